@@ -1,0 +1,164 @@
+/*
+ * sarlabtdc_amd.h -- C ABI of the MI355X time-domain-correlation (TDC) SAR
+ * image former.
+ *
+ * The reference exposes a C++ class (tdc-lib/include/tdcprocessor.h:42-135)
+ * and a pybind11 module (py-sarlabtdc/src/interface.cpp:9-23) and has no C
+ * ABI. These entry points carry exactly the arguments of that class, as plain
+ * pointers and sizes, so any FFI (ctypes, cgo, JNI, N-API) can bind them. The
+ * C++ class in tdcprocessor.h and the Python module _sarlabtdc are thin
+ * layers over this header.
+ *
+ * Conventions
+ *   - Every function returns an int status (TDC_OK or a TDC_ERR_* code).
+ *     No exception crosses this boundary. tdc_last_error() returns a
+ *     thread-local message for the last failing call on this thread.
+ *   - Complex data are interleaved float pairs (re, im), the layout of
+ *     std::complex<float> and of HIP's float2.
+ *   - Host arrays passed to tdc_set_* are borrowed, not copied, until the
+ *     next tdc_start / tdc_prepare on the handle (the reference keeps the
+ *     same non-owning pointers, tdcprocessor.cpp:169-217).
+ *   - A handle is not thread safe; one handle per host thread.
+ */
+#ifndef SARLABTDC_AMD_H
+#define SARLABTDC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The C++ layer maps INVALID_ARGUMENT to std::invalid_argument
+ * and every other error to std::runtime_error, as the reference throws
+ * (tdcprocessor.cpp:48-52, :189-195, :149-154). */
+#define TDC_OK 0
+#define TDC_ERR_INVALID_ARGUMENT 1
+#define TDC_ERR_RUNTIME 2
+#define TDC_ERR_DEVICE 3
+#define TDC_ERR_STATE 4
+#define TDC_ERR_UNSUPPORTED 5
+#define TDC_ERR_OUT_OF_MEMORY 6
+
+/* Image-formation forms (additive; the reference has only the plain one). */
+#define TDC_FORM_PLAIN 0     /* direct per-term FP32 reference phase: the drop-in */
+#define TDC_FORM_PRECISION 1 /* FP64 range/anchor + FP32 rotation recurrence */
+
+/* Raw sample storage on the device for the precision form (additive). */
+#define TDC_RAW_C64 0  /* complex64 */
+#define TDC_RAW_F16 1  /* two IEEE half values per sample */
+#define TDC_RAW_BF16 2 /* two bfloat16 values per sample */
+
+/* Range window families (additive; the reference hard-codes the periodic
+ * Hamming window with A = 25/46, tdc-lib/include/tuning.h:31). */
+#define TDC_WIN_HAMMING 0 /* A - (1-A) cos(2 pi m / Ns), p0 = A (25/46 default) */
+#define TDC_WIN_HANN 1    /* A = 0.5 */
+#define TDC_WIN_TAYLOR 2  /* p0 = nbar, p1 = peak sidelobe level in dB (positive) */
+
+typedef struct tdc_processor tdc_processor;
+
+typedef struct tdc_stats {
+    double kernel_ms;       /* device time of the last run's hot-path kernels (HIP events) */
+    double main_kernel_ms;  /* device time of the correlate-sum kernel alone */
+    double end_to_end_ms;   /* host wall time of the last tdc_start (H2D + compute + D2H) */
+    double terms;           /* pixels * chirps * samples of the last run */
+    int32_t form;           /* TDC_FORM_* actually run */
+    int32_t raw_storage;    /* TDC_RAW_* actually run */
+    int32_t doppler_active; /* 1 when the per-term Doppler window / skip test ran */
+    int32_t chirp_splits;   /* chirp ranges the pixels were split over */
+    int32_t pixels_per_lane;
+    int32_t workgroups;
+    int64_t skipped_chunks;   /* (pixel, chunk) pairs skipped; only counted when doppler_active */
+    int64_t processed_chunks; /* (pixel, chunk) pairs processed */
+} tdc_stats;
+
+/* Message for the last failing call on this thread ("" when none). */
+const char *tdc_last_error(void);
+
+/* ABI version: major * 10000 + minor * 100 + patch. */
+int tdc_abi_version(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int tdc_device_count(void);
+
+/* Replaces TdcProcessor::TdcProcessor(int gpuNum), tdcprocessor.cpp:39-55:
+ * selects the device, fails with TDC_ERR_DEVICE when it cannot. */
+int tdc_create(int gpu_num, tdc_processor **out);
+
+/* Replaces TdcProcessor::~TdcProcessor, tdcprocessor.cpp:57. NULL is a no-op. */
+int tdc_destroy(tdc_processor *h);
+
+/* Replaces TdcProcessor::setRawData, tdcprocessor.cpp:169-209 (declared
+ * tdcprocessor.h:104-107). raw_iq [n_pri][n_samples][2], pri_times [n_pri]
+ * (kept for parity; the reference uploads but never reads it),
+ * sample_times [n_samples], position/velocity [n_pri][n_samples][3],
+ * attitude [n_pri][n_samples][4] quaternions (scalar first, body to local),
+ * body_boresight [3]. n_pri <= 0 or n_samples <= 0 gives
+ * TDC_ERR_INVALID_ARGUMENT. Computes the centre wavelength
+ * c / ((f0 + f_end) / 2) in double, as the reference does. */
+int tdc_set_raw_data(tdc_processor *h, const float *raw_iq, const float *pri_times,
+                     const float *sample_times, const float *position, const float *velocity,
+                     const float *attitude, int n_pri, int n_samples, float mod_rate,
+                     float start_freq, const float *body_boresight);
+
+/* Replaces TdcProcessor::setFocusGrid, tdcprocessor.cpp:211-217:
+ * grid [n_rows][n_cols][3] (x, y, z) in the local frame. */
+int tdc_set_focus_grid(tdc_processor *h, const float *grid, int n_rows, int n_cols);
+
+/* Replaces TdcProcessor::start, tdcprocessor.cpp:59-167 (tdcprocessor.h:67-68).
+ * Forms the image: upload, one fused correlate-sum launch (plus a split
+ * reduction), download into the processor-owned image buffer. */
+int tdc_start(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
+              int dop_centroid_win, int apply_range_win, int doppler_win_taper);
+
+/* Replaces TdcProcessor::imageBuffer, tdcprocessor.cpp:240-243: *image_iq
+ * receives the processor-owned [n_rows][n_cols][2] image, valid until the
+ * next start or destroy; NULL before the first start (as the reference). */
+int tdc_image_buffer(const tdc_processor *h, const float **image_iq);
+
+/* Shape of the focus grid last set (no reference counterpart; the reference's
+ * Python wrapper keeps it itself, tdcprocessor_wrapper.h:56-57). */
+int tdc_image_shape(const tdc_processor *h, int *n_rows, int *n_cols);
+
+/* ---- additive API: no counterpart in the reference ------------------------ */
+
+/* Range window used when start(..., apply_range_win = 1). */
+int tdc_set_range_window(tdc_processor *h, int kind, double p0, double p1);
+
+/* Form and device raw storage used by the next start / prepare. */
+int tdc_set_form(tdc_processor *h, int form);
+int tdc_set_raw_storage(tdc_processor *h, int storage);
+
+/* FP64 inputs for the precision form: raw_iq complex64 [n_pri][n_samples],
+ * sample_times [n_samples] (uniformly spaced), chirp_position [n_pri][3]
+ * (the phase centre of each chirp: stop-and-hop), grid set by
+ * tdc_set_focus_grid_f64. Selects TDC_FORM_PRECISION. */
+int tdc_set_raw_data_f64(tdc_processor *h, const float *raw_iq, const double *pri_times,
+                         const double *sample_times, const double *chirp_position, int n_pri,
+                         int n_samples, double mod_rate, double start_freq);
+int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int n_cols);
+
+/* Split of tdc_start for callers that keep inputs resident in HBM:
+ * tdc_prepare uploads and stages the inputs; tdc_run forms the image on the
+ * device `iterations` times (each run recomputes it from the staged inputs);
+ * tdc_fetch_image copies it into the image buffer. tdc_start is exactly
+ * prepare + run(1) + fetch. */
+int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
+                int dop_centroid_win, int apply_range_win, int doppler_win_taper);
+int tdc_run(tdc_processor *h, int iterations);
+int tdc_fetch_image(tdc_processor *h);
+
+/* Statistics of the last run (timings from HIP events on the processor's
+ * stream). */
+int tdc_get_stats(const tdc_processor *h, tdc_stats *out);
+
+/* Force a launch configuration for tuning and tests: pixels per lane
+ * (1, 2 or 4) and the chirp split count (0 = automatic). */
+int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SARLABTDC_AMD_H */

@@ -1,0 +1,795 @@
+// tdc_capi.cpp -- host side of the C ABI declared in include/sarlabtdc_amd.h.
+//
+// Replaces the reference's TdcProcessor driver (tdc-lib/src/tdcprocessor.cpp:
+// 59-167): instead of staging 128-chirp chunks through pinned buffers and
+// launching five kernels per (pixel, chunk) on the legacy default stream,
+// the whole job is uploaded once, staged by one kernel into the layout the
+// correlate-sum kernel reads, formed by one correlate-sum launch (plus a
+// fixed-order split reduction), and copied back. Device buffers are kept and
+// reused across calls (the reference reallocates every buffer on every
+// start(), tdcprocessor.cpp:65-66).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "sarlabtdc_amd.h"
+#include "tdc_internal.h"
+
+namespace {
+
+constexpr double kSpeedOfLight = 299792458.0; // tdcprocessor.cpp:37
+constexpr double kPi = 3.14159265358979323846;
+
+thread_local std::string t_err;
+
+int fail(int code, std::string msg)
+{
+    t_err = std::move(msg);
+    return code;
+}
+
+int ok()
+{
+    t_err.clear();
+    return TDC_OK;
+}
+
+#define TDC_HIP(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(TDC_ERR_DEVICE, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes == 0) bytes = 256;
+        if (bytes <= n) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Periodic (DFT-even) range windows, evaluated in FP64 and rounded once.
+void range_window(int kind, double p0, double p1, int ns, bool apply, std::vector<float> &out)
+{
+    out.assign(ns, 1.0F);
+    if (!apply) return;
+    if (kind == TDC_WIN_HAMMING || kind == TDC_WIN_HANN) {
+        // tdckernels.cu:43-46: A - (1-A) cos(2 pi m / Ns)
+        const double a = (kind == TDC_WIN_HANN) ? 0.5 : p0;
+        for (int m = 0; m < ns; ++m)
+            out[m] = static_cast<float>(a - (1.0 - a) * std::cos(2.0 * kPi * m / ns));
+        return;
+    }
+    // Taylor window (T. T. Taylor 1955), normalised to 1 at the centre,
+    // periodic: the (ns+1)-point symmetric window without its last point.
+    const int nbar = std::max(1, std::min(64, static_cast<int>(std::lround(p0))));
+    const double sll = p1;
+    const int nf = ns + 1;
+    const double b = std::pow(10.0, sll / 20.0);
+    const double a = std::acosh(b) / kPi;
+    const double s2 = static_cast<double>(nbar) * nbar / (a * a + (nbar - 0.5) * (nbar - 0.5));
+    std::vector<double> fm(std::max(0, nbar - 1));
+    for (int m = 1; m < nbar; ++m) {
+        double num = (m % 2 == 1) ? 1.0 : -1.0;
+        double den = 2.0;
+        for (int i = 1; i < nbar; ++i) {
+            num *= 1.0 - static_cast<double>(m) * m / s2 / (a * a + (i - 0.5) * (i - 0.5));
+            if (i != m) den *= 1.0 - static_cast<double>(m) * m / (static_cast<double>(i) * i);
+        }
+        fm[m - 1] = num / den;
+    }
+    auto w = [&](double n) {
+        double s = 1.0;
+        for (int m = 1; m < nbar; ++m)
+            s += 2.0 * fm[m - 1] * std::cos(2.0 * kPi * m * (n - nf / 2.0 + 0.5) / nf);
+        return s;
+    };
+    const double centre = w((nf - 1) / 2.0);
+    for (int m = 0; m < ns; ++m) out[m] = static_cast<float>(w(m) / centre);
+}
+
+struct PosKey {
+    uint32_t x, y, z;
+    bool operator==(const PosKey &o) const { return x == o.x && y == o.y && z == o.z; }
+};
+struct PosKeyHash {
+    size_t operator()(const PosKey &k) const
+    {
+        uint64_t h = k.x * 0x9E3779B97F4A7C15ULL;
+        h ^= (k.y + 0x7F4A7C159E3779B9ULL) + (h << 6) + (h >> 2);
+        h ^= (k.z + 0x94D049BB133111EBULL) + (h << 6) + (h >> 2);
+        return static_cast<size_t>(h);
+    }
+};
+PosKey key_of(float x, float y, float z)
+{
+    // +0.0 and -0.0 compare equal, so normalise the sign of zero
+    auto bits = [](float v) {
+        if (v == 0.0F) v = 0.0F;
+        uint32_t b;
+        std::memcpy(&b, &v, 4);
+        return b;
+    };
+    return PosKey{bits(x), bits(y), bits(z)};
+}
+
+bool all_finite(const float *p, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+} // namespace
+
+struct tdc_processor {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
+
+    // FP32 drop-in inputs (borrowed)
+    const float *raw = nullptr;
+    const float *pri_times = nullptr;
+    const float *sample_times = nullptr;
+    const float *pos = nullptr;
+    const float *vel = nullptr;
+    const float *att = nullptr;
+    int npri = 0, ns = 0;
+    float mod_rate = 0.0F, start_freq = 0.0F;
+    float bore[3] = {0.0F, 0.0F, 0.0F};
+    float lambda_c = 0.0F;
+    bool have_raw32 = false;
+
+    // FP64 inputs of the precision form (borrowed)
+    const double *sample_times64 = nullptr;
+    const double *cpos64 = nullptr;
+    double mod_rate64 = 0.0, start_freq64 = 0.0;
+    bool have_raw64 = false;
+
+    // focus grid (borrowed)
+    const float *grid = nullptr;
+    const double *grid64 = nullptr;
+    int rows = 0, cols = 0;
+
+    // options
+    int form = TDC_FORM_PLAIN;
+    int storage = TDC_RAW_C64;
+    int win_kind = TDC_WIN_HAMMING;
+    double win_p0 = 25.0 / 46.0, win_p1 = 0.0; // tuning.h:31
+    int force_ppl = 0, force_split = 0;
+
+    // device buffers
+    DevBuf d_raw, d_pos, d_vel, d_att, d_kt, d_wr, d_rec, d_grid, d_part, d_img, d_cpos,
+        d_kanchor, d_counters;
+
+    // prepared state
+    bool prepared = false;
+    int run_form = TDC_FORM_PLAIN;
+    int run_storage = TDC_RAW_C64;
+    bool doppler = false;
+    tdc::Geom geom{};
+    tdc::DopParams dp{};
+    int ppl = 1;
+    double dk = 0.0;
+    int64_t trivial_processed = 0;
+
+    std::vector<float> image;
+    bool has_image = false;
+    tdc_stats stats{};
+};
+
+namespace {
+
+int ensure_device(tdc_processor *h)
+{
+    TDC_HIP(hipSetDevice(h->device));
+    return TDC_OK;
+}
+
+// Power-of-two chirp split so that the launch has enough workgroups to fill
+// 256 CUs several times over, while each split keeps a useful chirp count.
+int choose_split(int ntiles, int npri, int force)
+{
+    if (force > 0) return force;
+    const int target_wgs = 2048;
+    const int min_cps = 16;
+    int s = 1;
+    while (s < 64 && ntiles * s < target_wgs && (npri + 2 * s - 1) / (2 * s) >= min_cps) s *= 2;
+    return s;
+}
+
+void make_geom(tdc::Geom &g, int npx, int npri, int ns, int ppl, int nsplit)
+{
+    g.npx = npx;
+    g.npri = npri;
+    g.ns = ns;
+    g.ntiles = (npx + tdc::kBlock * ppl - 1) / (tdc::kBlock * ppl);
+    g.nsplit = nsplit;
+    g.cps = (npri + nsplit - 1) / nsplit;
+    const bool pow2 = (nsplit & (nsplit - 1)) == 0;
+    g.xcd_map = pow2 ? 1 : 0;
+    if (g.xcd_map && nsplit < 8) {
+        const int per = 8 / nsplit;
+        g.tiles_padded = (g.ntiles + per - 1) / per * per;
+    } else {
+        g.tiles_padded = g.ntiles;
+    }
+}
+
+// True when every term's Doppler window value is exactly 1 and no (pixel,
+// chunk) can be skipped, so the per-term window and the skip test can be
+// left out without changing any result. Uses |f_D| <= (2/lambda)|v| and
+// |f_c| <= (2/lambda)|v| |q|^2 |b| (a rotation by a non-unit quaternion
+// scales by |q|^2), and excludes pixels that coincide with an antenna
+// position (0/0 there makes the reference's window NaN, hence 0).
+bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool centroid,
+                           bool taper)
+{
+    if (!(bw > 0.0F) || !std::isfinite(center)) return false;
+    const size_t nterm = static_cast<size_t>(h->npri) * h->ns;
+    const size_t npx = static_cast<size_t>(h->rows) * h->cols;
+    if (!h->vel || !h->att || !h->pos) return false;
+    if (!all_finite(h->pos, nterm * 3) || !all_finite(h->vel, nterm * 3) ||
+        !all_finite(h->att, nterm * 4) || !all_finite(h->grid, npx * 3))
+        return false;
+    double vmax2 = 0.0, qmax2 = 0.0;
+    for (size_t e = 0; e < nterm; ++e) {
+        const float *v = h->vel + 3 * e;
+        vmax2 = std::max(vmax2, double(v[0]) * v[0] + double(v[1]) * v[1] + double(v[2]) * v[2]);
+        const float *q = h->att + 4 * e;
+        qmax2 = std::max(qmax2, double(q[0]) * q[0] + double(q[1]) * q[1] + double(q[2]) * q[2] +
+                                    double(q[3]) * q[3]);
+    }
+    const double tol = 2.0 / h->lambda_c;
+    const double vmax = std::sqrt(vmax2);
+    const double bmag = std::sqrt(double(h->bore[0]) * h->bore[0] + double(h->bore[1]) * h->bore[1] +
+                                  double(h->bore[2]) * h->bore[2]);
+    const double fd_max = tol * vmax;
+    const double fc_max = centroid ? tol * vmax * qmax2 * bmag : std::fabs(double(center));
+    const double bound = (fd_max + fc_max) * (1.0 + 1e-4) + 1e-30;
+    if (!std::isfinite(bound) || bound > double(bw) / 2.0) return false;
+    // with the taper on, cos(2 pi d / bw - pi) must round to exactly -1
+    if (taper && 2.0 * kPi * bound / double(bw) > 1e-5) return false;
+    std::unordered_set<PosKey, PosKeyHash> seen;
+    seen.insert(key_of(0.0F, 0.0F, 0.0F)); // zero-filled probe rows
+    const float *last = nullptr;
+    for (size_t e = 0; e < nterm; ++e) {
+        const float *p = h->pos + 3 * e;
+        if (last && p[0] == last[0] && p[1] == last[1] && p[2] == last[2]) continue;
+        seen.insert(key_of(p[0], p[1], p[2]));
+        last = p;
+    }
+    for (size_t i = 0; i < npx; ++i) {
+        const float *g = h->grid + 3 * i;
+        if (seen.count(key_of(g[0], g[1], g[2]))) return false;
+    }
+    return true;
+}
+
+template <class T> int upload(DevBuf &b, const T *src, size_t count, hipStream_t s)
+{
+    TDC_HIP(b.ensure(sizeof(T) * count));
+    if (count) TDC_HIP(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    return TDC_OK;
+}
+
+int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroid, bool apply_rw,
+                  bool taper)
+{
+    if (!h->have_raw32)
+        return fail(TDC_ERR_STATE, "the plain form needs setRawData (FP32 inputs) before start()");
+    if (!h->grid) return fail(TDC_ERR_STATE, "setFocusGrid must be called before start()");
+    const int npri = h->npri, ns = h->ns, npx = h->rows * h->cols;
+    const size_t nterm = static_cast<size_t>(npri) * ns;
+    std::vector<float> wr, kt(ns);
+    range_window(h->win_kind, h->win_p0, h->win_p1, ns, apply_rw, wr);
+    for (int m = 0; m < ns; ++m) {
+        const double f = double(h->start_freq) + double(h->mod_rate) * double(h->sample_times[m]);
+        kt[m] = static_cast<float>(2.0 * f / kSpeedOfLight);
+    }
+    h->doppler = !doppler_window_is_one(h, dop_center, dop_bw, centroid, taper);
+    hipStream_t s = h->stream;
+    int rc;
+    if ((rc = upload(h->d_raw, h->raw, nterm * 2, s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_pos, h->pos, nterm * 3, s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_kt, kt.data(), ns, s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_wr, wr.data(), ns, s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_grid, h->grid, static_cast<size_t>(npx) * 3, s)) != TDC_OK) return rc;
+    if (h->doppler) {
+        if (!h->vel || !h->att)
+            return fail(TDC_ERR_INVALID_ARGUMENT, "the Doppler window needs velocity and attitude");
+        if ((rc = upload(h->d_vel, h->vel, nterm * 3, s)) != TDC_OK) return rc;
+        if ((rc = upload(h->d_att, h->att, nterm * 4, s)) != TDC_OK) return rc;
+        TDC_HIP(h->d_rec.ensure(sizeof(tdc::DopRec) * nterm));
+        const float tol = 2.0F / h->lambda_c; // windowing.h:21, :33
+        TDC_HIP(tdc::launch_stage_doppler(h->d_raw.as<float>(), h->d_pos.as<float>(),
+                                          h->d_vel.as<float>(), h->d_att.as<float>(),
+                                          h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns, tol,
+                                          h->bore, centroid ? 1 : 0, dop_center,
+                                          h->d_rec.as<tdc::DopRec>(), s));
+        h->dp.half_bw = dop_bw / 2.0F;
+        h->dp.inv_bw = 1.0F / dop_bw;
+        h->dp.az_a = static_cast<float>(25.0 / 46.0); // tuning.h:38
+        h->dp.taper = taper ? 1 : 0;
+        h->dp.centroid = centroid ? 1 : 0;
+        h->dp.fc_fixed = dop_center;
+        TDC_HIP(h->d_counters.ensure(2 * sizeof(unsigned long long)));
+    } else {
+        TDC_HIP(h->d_rec.ensure(sizeof(tdc::PlainRec) * nterm));
+        TDC_HIP(tdc::launch_stage_plain(h->d_raw.as<float>(), h->d_pos.as<float>(),
+                                        h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
+                                        h->d_rec.as<tdc::PlainRec>(), s));
+    }
+    h->ppl = h->force_ppl ? h->force_ppl : (h->doppler ? 1 : 2);
+    const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
+    const int nsplit = choose_split(ntiles, npri, h->force_split);
+    make_geom(h->geom, npx, npri, ns, h->ppl, nsplit);
+    TDC_HIP(h->d_img.ensure(sizeof(float2) * static_cast<size_t>(npx)));
+    if (nsplit > 1) TDC_HIP(h->d_part.ensure(sizeof(float2) * static_cast<size_t>(npx) * nsplit));
+    h->run_form = TDC_FORM_PLAIN;
+    h->run_storage = TDC_RAW_C64;
+    return TDC_OK;
+}
+
+int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool centroid,
+                      bool apply_rw, bool taper)
+{
+    const int npri = h->npri, ns = h->ns, npx = h->rows * h->cols;
+    const size_t nterm = static_cast<size_t>(npri) * ns;
+    std::vector<double> t(ns), cpos(static_cast<size_t>(npri) * 3), grid64;
+    double f0, krate;
+    const double *gridp = nullptr;
+    if (h->have_raw64) {
+        if (!h->grid64 && !h->grid)
+            return fail(TDC_ERR_STATE, "setFocusGrid must be called before start()");
+        // the FP64 API carries no velocity: the Doppler window must be identically 1
+        if (taper || !(double(dop_bw) / 2.0 >= (centroid ? 0.0 : std::fabs(double(dop_center)))))
+            return fail(TDC_ERR_UNSUPPORTED,
+                        "the precision form applies no Doppler window; pass a band that "
+                        "contains the window centre with the taper off");
+        std::copy(h->sample_times64, h->sample_times64 + ns, t.begin());
+        std::copy(h->cpos64, h->cpos64 + 3 * static_cast<size_t>(npri), cpos.begin());
+        f0 = h->start_freq64;
+        krate = h->mod_rate64;
+    } else {
+        if (!h->have_raw32) return fail(TDC_ERR_STATE, "setRawData must be called before start()");
+        if (!h->grid) return fail(TDC_ERR_STATE, "setFocusGrid must be called before start()");
+        if (!doppler_window_is_one(h, dop_center, dop_bw, centroid, taper))
+            return fail(TDC_ERR_UNSUPPORTED,
+                        "the precision form needs a Doppler window that is identically 1 over "
+                        "the data (use the plain form for Doppler windowing)");
+        for (int n = 0; n < npri; ++n) {
+            const float *p0 = h->pos + static_cast<size_t>(n) * ns * 3;
+            for (int m = 1; m < ns; ++m) {
+                const float *pm = p0 + 3 * static_cast<size_t>(m);
+                if (pm[0] != p0[0] || pm[1] != p0[1] || pm[2] != p0[2])
+                    return fail(TDC_ERR_UNSUPPORTED,
+                                "the precision form needs one phase-centre position per chirp "
+                                "(stop-and-hop); positions vary within chirp " +
+                                    std::to_string(n));
+            }
+            cpos[3 * n] = p0[0];
+            cpos[3 * n + 1] = p0[1];
+            cpos[3 * n + 2] = p0[2];
+        }
+        for (int m = 0; m < ns; ++m) t[m] = h->sample_times[m];
+        f0 = h->start_freq;
+        krate = h->mod_rate;
+    }
+    if (h->grid64) {
+        gridp = h->grid64;
+    } else {
+        grid64.resize(static_cast<size_t>(npx) * 3);
+        for (size_t i = 0; i < grid64.size(); ++i) grid64[i] = h->grid[i];
+        gridp = grid64.data();
+    }
+    // uniform sampling: the rotation recurrence advances by a constant step
+    // (a few float32 ulps of the largest time are allowed, so that sample times
+    // given in float32 qualify; the anchor phases use the exact anchor times)
+    double dt = ns > 1 ? (t[ns - 1] - t[0]) / (ns - 1) : 0.0;
+    const double tmax = std::max(std::fabs(t[0]), std::fabs(t[ns - 1]));
+    const double ttol = 1e-6 * std::fabs(dt) + 0x1p-21 * tmax + 1e-300;
+    for (int m = 0; m < ns; ++m) {
+        if (std::fabs(t[m] - (t[0] + m * dt)) > ttol)
+            return fail(TDC_ERR_UNSUPPORTED,
+                        "the precision form needs uniformly spaced sample times");
+    }
+    const int nblk = (ns + tdc::kAnchorBlock - 1) / tdc::kAnchorBlock;
+    std::vector<double> kanchor(nblk);
+    for (int b = 0; b < nblk; ++b)
+        kanchor[b] = 2.0 * (f0 + krate * t[b * tdc::kAnchorBlock]) / kSpeedOfLight;
+    h->dk = 2.0 * krate * dt / kSpeedOfLight;
+    std::vector<tdc::ChirpPos> cp(npri);
+    for (int n = 0; n < npri; ++n) cp[n] = tdc::ChirpPos{cpos[3 * n], cpos[3 * n + 1], cpos[3 * n + 2], 0.0};
+    std::vector<float> wr;
+    range_window(h->win_kind, h->win_p0, h->win_p1, ns, apply_rw, wr);
+
+    hipStream_t s = h->stream;
+    int rc;
+    if ((rc = upload(h->d_raw, h->raw, nterm * 2, s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_wr, wr.data(), ns, s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_cpos, cp.data(), cp.size(), s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_kanchor, kanchor.data(), kanchor.size(), s)) != TDC_OK) return rc;
+    if ((rc = upload(h->d_grid, gridp, static_cast<size_t>(npx) * 3, s)) != TDC_OK) return rc;
+    const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
+    TDC_HIP(h->d_rec.ensure(esz * nterm));
+    TDC_HIP(tdc::launch_stage_precision(h->storage, h->d_raw.as<float>(), h->d_wr.as<float>(), npri,
+                                        ns, h->d_rec.p, s));
+    h->doppler = false;
+    h->ppl = h->force_ppl ? h->force_ppl : 4;
+    const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
+    const int nsplit = choose_split(ntiles, npri, h->force_split);
+    make_geom(h->geom, npx, npri, ns, h->ppl, nsplit);
+    TDC_HIP(h->d_img.ensure(sizeof(float2) * static_cast<size_t>(npx)));
+    TDC_HIP(h->d_part.ensure(sizeof(double2) * static_cast<size_t>(npx) * nsplit));
+    h->run_form = TDC_FORM_PRECISION;
+    h->run_storage = h->storage;
+    return TDC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *tdc_last_error(void) { return t_err.c_str(); }
+
+int tdc_abi_version(void) { return 10000; }
+
+int tdc_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int tdc_create(int gpu_num, tdc_processor **out)
+{
+    if (!out) return fail(TDC_ERR_INVALID_ARGUMENT, "out handle pointer is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || gpu_num < 0 || gpu_num >= n) {
+        return fail(TDC_ERR_DEVICE, std::string("Failed to set HIP device ") +
+                                        std::to_string(gpu_num) + ": " +
+                                        (e != hipSuccess ? hipGetErrorString(e)
+                                                         : "no such device"));
+    }
+    auto *h = new tdc_processor();
+    h->device = gpu_num;
+    e = hipSetDevice(gpu_num);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev_a);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev_b);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev_c);
+    if (e != hipSuccess) {
+        tdc_destroy(h);
+        return fail(TDC_ERR_DEVICE, std::string("Failed to set HIP device: ") + hipGetErrorString(e));
+    }
+    *out = h;
+    return ok();
+}
+
+int tdc_destroy(tdc_processor *h)
+{
+    if (!h) return ok();
+    (void) hipSetDevice(h->device);
+    if (h->stream) (void) hipStreamSynchronize(h->stream);
+    if (h->ev_a) (void) hipEventDestroy(h->ev_a);
+    if (h->ev_b) (void) hipEventDestroy(h->ev_b);
+    if (h->ev_c) (void) hipEventDestroy(h->ev_c);
+    if (h->stream) (void) hipStreamDestroy(h->stream);
+    delete h;
+    return ok();
+}
+
+int tdc_set_raw_data(tdc_processor *h, const float *raw_iq, const float *pri_times,
+                     const float *sample_times, const float *position, const float *velocity,
+                     const float *attitude, int n_pri, int n_samples, float mod_rate,
+                     float start_freq, const float *body_boresight)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    // tdcprocessor.cpp:189-195
+    if (n_pri <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nPri must be greater than 0");
+    if (n_samples <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nSamples must be greater than 0");
+    if (!raw_iq || !sample_times || !position)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "rawData, sampleTimes and position must not be NULL");
+    h->raw = raw_iq;
+    h->pri_times = pri_times;
+    h->sample_times = sample_times;
+    h->pos = position;
+    h->vel = velocity;
+    h->att = attitude;
+    h->npri = n_pri;
+    h->ns = n_samples;
+    h->mod_rate = mod_rate;
+    h->start_freq = start_freq;
+    for (int i = 0; i < 3; ++i) h->bore[i] = body_boresight ? body_boresight[i] : 0.0F;
+    // tdcprocessor.cpp:204-208: centre wavelength in double, stored as float
+    const double chirp_time = sample_times[n_samples - 1] - sample_times[0];
+    const double end_freq = start_freq + static_cast<double>(mod_rate) * chirp_time;
+    const double center_freq = (start_freq + end_freq) / 2.0;
+    h->lambda_c = static_cast<float>(kSpeedOfLight / center_freq);
+    h->have_raw32 = true;
+    h->have_raw64 = false;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_set_focus_grid(tdc_processor *h, const float *grid, int n_rows, int n_cols)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n_rows < 0 || n_cols < 0)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "grid dimensions must not be negative");
+    if (!grid && n_rows * n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
+    h->grid = grid;
+    h->grid64 = nullptr;
+    h->rows = n_rows;
+    h->cols = n_cols;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_set_range_window(tdc_processor *h, int kind, double p0, double p1)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (kind == TDC_WIN_HAMMING) {
+        if (!(p0 >= 0.0 && p0 <= 1.0)) return fail(TDC_ERR_INVALID_ARGUMENT, "Hamming A must be in [0, 1]");
+    } else if (kind == TDC_WIN_TAYLOR) {
+        if (!(p0 >= 1.0 && p0 <= 64.0) || !(p1 > 0.0))
+            return fail(TDC_ERR_INVALID_ARGUMENT, "Taylor needs 1 <= nbar <= 64 and sll > 0 dB");
+    } else if (kind != TDC_WIN_HANN) {
+        return fail(TDC_ERR_INVALID_ARGUMENT, "unknown window kind");
+    }
+    h->win_kind = kind;
+    h->win_p0 = p0;
+    h->win_p1 = p1;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_set_form(tdc_processor *h, int form)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (form != TDC_FORM_PLAIN && form != TDC_FORM_PRECISION)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "unknown form");
+    h->form = form;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_set_raw_storage(tdc_processor *h, int storage)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (storage != TDC_RAW_C64 && storage != TDC_RAW_F16 && storage != TDC_RAW_BF16)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "unknown raw storage");
+    h->storage = storage;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_set_raw_data_f64(tdc_processor *h, const float *raw_iq, const double *pri_times,
+                         const double *sample_times, const double *chirp_position, int n_pri,
+                         int n_samples, double mod_rate, double start_freq)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n_pri <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nPri must be greater than 0");
+    if (n_samples <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nSamples must be greater than 0");
+    if (!raw_iq || !sample_times || !chirp_position)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "rawData, sampleTimes and position must not be NULL");
+    (void) pri_times;
+    h->raw = raw_iq;
+    h->sample_times64 = sample_times;
+    h->cpos64 = chirp_position;
+    h->npri = n_pri;
+    h->ns = n_samples;
+    h->mod_rate64 = mod_rate;
+    h->start_freq64 = start_freq;
+    h->have_raw64 = true;
+    h->have_raw32 = false;
+    h->form = TDC_FORM_PRECISION;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int n_cols)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (n_rows < 0 || n_cols < 0)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "grid dimensions must not be negative");
+    if (!grid && n_rows * n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
+    h->grid64 = grid;
+    h->grid = nullptr;
+    h->rows = n_rows;
+    h->cols = n_cols;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (pixels_per_lane != 0 && pixels_per_lane != 1 && pixels_per_lane != 2 && pixels_per_lane != 4)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "pixels per lane must be 0, 1, 2 or 4");
+    if (chirp_splits < 0 || chirp_splits > 4096)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "chirp splits must be in [0, 4096]");
+    h->force_ppl = pixels_per_lane;
+    h->force_split = chirp_splits;
+    h->prepared = false;
+    return ok();
+}
+
+int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
+                int dop_centroid_win, int apply_range_win, int doppler_win_taper)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (!h->have_raw32 && !h->have_raw64)
+        return fail(TDC_ERR_STATE, "setRawData must be called before start()");
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    h->prepared = false;
+    if (h->form == TDC_FORM_PRECISION || h->have_raw64) {
+        rc = prepare_precision(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
+                               apply_range_win != 0, doppler_win_taper != 0);
+    } else {
+        rc = prepare_plain(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
+                           apply_range_win != 0, doppler_win_taper != 0);
+    }
+    if (rc != TDC_OK) return rc;
+    const int nchunks = (h->npri + tdc::kChunk - 1) / tdc::kChunk;
+    h->trivial_processed = static_cast<int64_t>(h->rows) * h->cols * nchunks;
+    h->prepared = true;
+    return ok();
+}
+
+int tdc_run(tdc_processor *h, int iterations)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (!h->prepared) return fail(TDC_ERR_STATE, "tdc_prepare must succeed before tdc_run");
+    if (iterations < 1) return fail(TDC_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    const tdc::Geom &g = h->geom;
+    hipStream_t s = h->stream;
+    double main_ms = 0.0, total_ms = 0.0;
+    int wgs = 0;
+    for (int it = 0; it < iterations; ++it) {
+        if (g.npx == 0) break;
+        if (h->doppler) TDC_HIP(hipMemsetAsync(h->d_counters.p, 0, 2 * sizeof(unsigned long long), s));
+        TDC_HIP(hipEventRecord(h->ev_a, s));
+        if (h->run_form == TDC_FORM_PLAIN) {
+            float2 *dst = g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>();
+            if (h->doppler) {
+                TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(), h->d_grid.as<float>(),
+                                            g, h->dp, dst, h->d_counters.as<unsigned long long>(), s,
+                                            &wgs));
+            } else {
+                TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), h->d_grid.as<float>(),
+                                          g, dst, s, &wgs));
+            }
+            TDC_HIP(hipEventRecord(h->ev_b, s));
+            if (g.nsplit > 1)
+                TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
+                                                 h->d_img.as<float2>(), s));
+        } else {
+            TDC_HIP(tdc::launch_precision(h->ppl, h->run_storage, h->d_rec.p,
+                                          h->d_cpos.as<tdc::ChirpPos>(), h->d_grid.as<double>(),
+                                          h->d_kanchor.as<double>(), h->dk, g,
+                                          h->d_part.as<double2>(), s, &wgs));
+            TDC_HIP(hipEventRecord(h->ev_b, s));
+            TDC_HIP(tdc::launch_finalize_f64(h->d_part.as<double2>(), g.nsplit, g.npx,
+                                             h->d_img.as<float2>(), s));
+        }
+        TDC_HIP(hipEventRecord(h->ev_c, s));
+        TDC_HIP(hipEventSynchronize(h->ev_c));
+        float a = 0.0F, b = 0.0F;
+        TDC_HIP(hipEventElapsedTime(&a, h->ev_a, h->ev_b));
+        TDC_HIP(hipEventElapsedTime(&b, h->ev_a, h->ev_c));
+        main_ms += a;
+        total_ms += b;
+    }
+    TDC_HIP(hipStreamSynchronize(s));
+    tdc_stats &st = h->stats;
+    st.main_kernel_ms = main_ms / iterations;
+    st.kernel_ms = total_ms / iterations;
+    st.terms = double(g.npx) * g.npri * g.ns;
+    st.form = h->run_form;
+    st.raw_storage = h->run_storage;
+    st.doppler_active = h->doppler ? 1 : 0;
+    st.chirp_splits = g.nsplit;
+    st.pixels_per_lane = h->ppl;
+    st.workgroups = wgs;
+    if (h->doppler && g.npx > 0) {
+        unsigned long long c[2] = {0, 0};
+        TDC_HIP(hipMemcpy(c, h->d_counters.p, sizeof c, hipMemcpyDeviceToHost));
+        st.processed_chunks = static_cast<int64_t>(c[0]);
+        st.skipped_chunks = static_cast<int64_t>(c[1]);
+    } else {
+        st.processed_chunks = h->trivial_processed;
+        st.skipped_chunks = 0;
+    }
+    return ok();
+}
+
+int tdc_fetch_image(tdc_processor *h)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (!h->prepared) return fail(TDC_ERR_STATE, "nothing has been run");
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    const size_t npx = static_cast<size_t>(h->rows) * h->cols;
+    h->image.assign(npx * 2, 0.0F);
+    if (npx) {
+        TDC_HIP(hipMemcpyAsync(h->image.data(), h->d_img.p, npx * sizeof(float2),
+                               hipMemcpyDeviceToHost, h->stream));
+        TDC_HIP(hipStreamSynchronize(h->stream));
+    }
+    h->has_image = true;
+    return ok();
+}
+
+int tdc_start(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
+              int dop_centroid_win, int apply_range_win, int doppler_win_taper)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = tdc_prepare(h, doppler_win_center, doppler_bandwidth, dop_centroid_win,
+                         apply_range_win, doppler_win_taper);
+    if (rc == TDC_OK) rc = tdc_run(h, 1);
+    if (rc == TDC_OK) rc = tdc_fetch_image(h);
+    if (rc != TDC_OK) return rc;
+    const auto t1 = std::chrono::steady_clock::now();
+    h->stats.end_to_end_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return ok();
+}
+
+int tdc_image_buffer(const tdc_processor *h, const float **image_iq)
+{
+    if (!h || !image_iq) return fail(TDC_ERR_INVALID_ARGUMENT, "NULL argument");
+    static const float empty_image[2] = {0.0F, 0.0F};
+    if (!h->has_image) *image_iq = nullptr;
+    else *image_iq = h->image.empty() ? empty_image : h->image.data();
+    return ok();
+}
+
+int tdc_image_shape(const tdc_processor *h, int *n_rows, int *n_cols)
+{
+    if (!h || !n_rows || !n_cols) return fail(TDC_ERR_INVALID_ARGUMENT, "NULL argument");
+    *n_rows = h->rows;
+    *n_cols = h->cols;
+    return ok();
+}
+
+int tdc_get_stats(const tdc_processor *h, tdc_stats *out)
+{
+    if (!h || !out) return fail(TDC_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = h->stats;
+    return ok();
+}
+
+} // extern "C"

@@ -1,0 +1,89 @@
+// tdc_internal.h -- shared between the HIP kernels and the host C ABI.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace tdc {
+
+// Chirps per skip-test chunk (reference tdc-lib/include/tuning.h:18).
+constexpr int kChunk = 128;
+// Threads per workgroup: four wave64 wavefronts.
+constexpr int kBlock = 256;
+// Samples per FP64 re-anchor block of the precision form.
+constexpr int kAnchorBlock = 64;
+
+// One (chirp, sample) term of the plain form, staged once per job.
+// Every field is wave-uniform in the correlate-sum kernel, so a record is
+// read with one scalar load (s_load_dwordx8) into SGPRs.
+struct alignas(32) PlainRec {
+    float rr, ri; // raw sample times the range window
+    float x, y, z; // antenna phase centre
+    float k;      // 2 (f0 + K t_m) / c, cycles per metre of range
+    float pad0, pad1;
+};
+
+// Plain form with the per-term Doppler window (reference
+// tdc-lib/src/tdckernels.cu:98-130): adds the velocity pre-scaled by
+// 2 / lambda_c and the Doppler window centre of the term.
+struct alignas(16) DopRec {
+    float rr, ri;
+    float x, y, z;
+    float k;
+    float vx, vy, vz; // (2 / lambda_c) * velocity
+    float fc;         // Doppler window centre (centroid or fixed), Hz
+    float pad0, pad1;
+};
+
+// Chirp phase centre of the precision form (FP64, padded to 32 bytes).
+struct alignas(32) ChirpPos {
+    double x, y, z, pad;
+};
+
+// Launch geometry shared by the correlate-sum kernels.
+struct Geom {
+    int npx;           // pixels
+    int npri;          // chirps
+    int ns;            // samples per chirp
+    int ntiles;        // pixel tiles of kBlock * P pixels
+    int nsplit;        // chirp ranges
+    int cps;           // chirps per split
+    int xcd_map;       // 1: XCD-aware block mapping (nsplit a power of two)
+    int tiles_padded;  // tiles including the padding of the XCD mapping
+};
+
+struct DopParams {
+    float half_bw;   // dopplerBandwidth / 2
+    float inv_bw;    // 1 / dopplerBandwidth
+    float az_a;      // azimuth window parameter A (25/46)
+    int taper;       // 1: raised-cosine taper, 0: rectangle
+    int centroid;    // fc from the centroid (1) or fixed (0)
+    float fc_fixed;  // dopplerWinCenter
+};
+
+// Kernel launch wrappers (defined in tdc_kernels.hip). All enqueue on `s`.
+hipError_t launch_stage_plain(const float *raw, const float *pos, const float *kt, const float *wr,
+                              int npri, int ns, PlainRec *out, hipStream_t s);
+hipError_t launch_stage_doppler(const float *raw, const float *pos, const float *vel,
+                                const float *att, const float *kt, const float *wr, int npri,
+                                int ns, float two_over_lambda, const float *bore, int centroid,
+                                float fc_fixed, DopRec *out, hipStream_t s);
+hipError_t launch_plain(int ppl, const PlainRec *rec, const float *grid, const Geom &g,
+                        float2 *out, hipStream_t s, int *workgroups);
+hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const Geom &g,
+                          const DopParams &dp, float2 *out, unsigned long long *counters,
+                          hipStream_t s, int *workgroups);
+hipError_t launch_finalize_f32(const float2 *partial, int nsplit, int npx, float2 *img,
+                               hipStream_t s);
+
+hipError_t launch_stage_precision(int storage, const float *raw, const float *wr, int npri, int ns,
+                                  void *out, hipStream_t s);
+hipError_t launch_precision(int ppl, int storage, const void *raw, const ChirpPos *cpos,
+                            const double *grid, const double *kanchor, double dk, const Geom &g,
+                            double2 *out, hipStream_t s, int *workgroups);
+hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
+                               hipStream_t s);
+
+} // namespace tdc

@@ -1,0 +1,682 @@
+// tdc_kernels.hip -- gfx950 kernels of the TDC correlate-sum path.
+//
+// Reference path (tdc-lib/src/tdckernels.cu): for every pixel and every
+// 128-chirp chunk the reference launches createWindowKernel (:70-132),
+// referenceResponseKernel (:178-217), correlateWithReference (:250-274), an
+// NPP complex sum and addToImage (:276-279, :309-314), each materialising a
+// chunk-sized array in global memory. Here one launch forms the whole image:
+//
+//   * every lane owns P pixels (coordinates and accumulators in VGPRs);
+//   * the loop runs over (chirp, sample) terms, which are the same for every
+//     lane of a wavefront, so each term's record arrives through one scalar
+//     load into SGPRs and is consumed as the scalar operand of the VALU ops;
+//   * chirps are split into ranges across workgroups when the grid alone
+//     cannot fill the chip; the partial images are summed in a fixed order
+//     by a second small kernel, so results do not depend on scheduling.
+//
+// Plain form (the drop-in): per term the direct reference phase
+//   theta = k_m * |pos - p|  (cycles, k_m = 2 (f0 + K t_m) / c)
+// with hardware sqrt / fract / sin / cos (inputs in revolutions), then a
+// complex multiply-accumulate of raw * exp(+j 2 pi theta) -- the conjugate
+// product of tdckernels.cu:270-272 with ref = w exp(-j 4 pi f R / c).
+//
+// Precision form: per (pixel, chirp) the range R is computed in FP64; per
+// 64-sample block the anchor phase k_a R is reduced modulo one cycle in
+// FP64; inside the block the phase advances by the constant rotation
+// w = exp(j 2 pi dk R), applied in nested (Horner) form to the running block
+// sum, S <- S w + raw, which is one complex rotate-and-add (4 FMAs) per term.
+// Block sums are rotated by the anchor phasor and accumulated with an FP32
+// two-sum compensation.
+#include <hip/hip_runtime.h>
+
+#include "tdc_internal.h"
+
+namespace tdc {
+namespace {
+
+// Workgroup -> (pixel tile, chirp split). Workgroups are dispatched to the
+// eight XCDs round-robin by id, so with xcd_map the workgroups of one XCD
+// share a chirp range and that range's records stay in the XCD's own L2.
+__device__ __forceinline__ void map_block(const Geom &g, int &tile, int &split)
+{
+    const int b = static_cast<int>(blockIdx.x);
+    if (g.xcd_map) {
+        const int xcd = b & 7;
+        const int slot = b >> 3;
+        if (g.nsplit >= 8) {
+            const int r = g.nsplit >> 3;
+            split = xcd + 8 * (slot % r);
+            tile = slot / r;
+        } else {
+            const int per = 8 / g.nsplit;
+            split = xcd % g.nsplit;
+            tile = slot * per + xcd / g.nsplit;
+        }
+    } else {
+        split = b % g.nsplit;
+        tile = b / g.nsplit;
+    }
+}
+
+__device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
+__device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
+
+// Error-free accumulation: acc + comp carries the running sum.
+__device__ __forceinline__ void two_sum_acc(float &acc, float &comp, float x)
+{
+    const float s = acc + x;
+    const float bp = s - acc;
+    const float err = (acc - (s - bp)) + (x - bp);
+    acc = s;
+    comp += err;
+}
+
+// Quaternion product / rotation in FP32 (reference tdc-lib/include/gpumath.h:66-87).
+__device__ __forceinline__ float4 q_prod(float4 a, float4 b)
+{
+    return make_float4((a.x * b.x) - (a.y * b.y) - (a.z * b.z) - (a.w * b.w),
+                       (a.x * b.y) + (a.y * b.x) + (a.z * b.w) - (a.w * b.z),
+                       (a.x * b.z) - (a.y * b.w) + (a.z * b.x) + (a.w * b.y),
+                       (a.x * b.w) + (a.y * b.z) - (a.z * b.y) + (a.w * b.x));
+}
+
+__device__ __forceinline__ float3 q_rot(float4 q, float3 v)
+{
+    const float4 pv = make_float4(0.0F, v.x, v.y, v.z);
+    const float4 qc = make_float4(q.x, -q.y, -q.z, -q.w);
+    const float4 r = q_prod(q_prod(q, pv), qc);
+    return make_float3(r.y, r.z, r.w);
+}
+
+// ---------------------------------------------------------------------------
+// Staging: raw * range window, positions and k_m packed per term.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void stage_plain_kernel(
+    const float2 *__restrict__ raw, const float *__restrict__ pos, const float *__restrict__ kt,
+    const float *__restrict__ wr, int npri, int ns, PlainRec *__restrict__ out)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    for (size_t e = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * kBlock) {
+        const int m = static_cast<int>(e % ns);
+        const float2 v = raw[e];
+        PlainRec r;
+        r.rr = v.x * wr[m];
+        r.ri = v.y * wr[m];
+        r.x = pos[3 * e];
+        r.y = pos[3 * e + 1];
+        r.z = pos[3 * e + 2];
+        r.k = kt[m];
+        r.pad0 = 0.0F;
+        r.pad1 = 0.0F;
+        out[e] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void stage_doppler_kernel(
+    const float2 *__restrict__ raw, const float *__restrict__ pos, const float *__restrict__ vel,
+    const float4 *__restrict__ att, const float *__restrict__ kt, const float *__restrict__ wr,
+    int npri, int ns, float two_over_lambda, float3 bore, int centroid, float fc_fixed,
+    DopRec *__restrict__ out)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    for (size_t e = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * kBlock) {
+        const int m = static_cast<int>(e % ns);
+        const float2 v = raw[e];
+        const float3 vv = make_float3(vel[3 * e], vel[3 * e + 1], vel[3 * e + 2]);
+        DopRec r;
+        r.rr = v.x * wr[m];
+        r.ri = v.y * wr[m];
+        r.x = pos[3 * e];
+        r.y = pos[3 * e + 1];
+        r.z = pos[3 * e + 2];
+        r.k = kt[m];
+        r.vx = two_over_lambda * vv.x;
+        r.vy = two_over_lambda * vv.y;
+        r.vz = two_over_lambda * vv.z;
+        if (centroid) {
+            // windowing.h:19-24 with the antenna pointing of gpumath.h:77-87
+            const float3 ant = q_rot(att[e], bore);
+            r.fc = two_over_lambda * ((vv.x * ant.x) + (vv.y * ant.y) + (vv.z * ant.z));
+        } else {
+            r.fc = fc_fixed;
+        }
+        r.pad0 = 0.0F;
+        r.pad1 = 0.0F;
+        out[e] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain form, Doppler window identically one (proved on the host).
+// ---------------------------------------------------------------------------
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void plain_kernel(const PlainRec *__restrict__ rec,
+                                                       const float *__restrict__ grid, Geom g,
+                                                       float2 *__restrict__ out)
+{
+    int tile, split;
+    map_block(g, tile, split);
+    if (tile >= g.ntiles) return;
+    const int n0 = split * g.cps;
+    const int n1 = min(g.npri, n0 + g.cps);
+
+    float px[P], py[P], pz[P], ar[P], ai[P];
+    int pix[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
+        const int q = min(pix[j], g.npx - 1);
+        px[j] = grid[3 * q];
+        py[j] = grid[3 * q + 1];
+        pz[j] = grid[3 * q + 2];
+        ar[j] = 0.0F;
+        ai[j] = 0.0F;
+    }
+
+    const PlainRec *__restrict__ r = rec + static_cast<size_t>(max(n0, 0)) * g.ns;
+    const int nterms = (n1 > n0) ? (n1 - n0) * g.ns : 0;
+#pragma unroll 4
+    for (int e = 0; e < nterms; ++e) {
+        const PlainRec t = r[e];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const float dx = px[j] - t.x;
+            const float dy = py[j] - t.y;
+            const float dz = pz[j] - t.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float th = __builtin_amdgcn_fractf(__builtin_amdgcn_sqrtf(r2) * t.k);
+            const float sn = sin_rev(th);
+            const float cs = cos_rev(th);
+            ar[j] = fmaf(t.rr, cs, ar[j]);
+            ar[j] = fmaf(-t.ri, sn, ar[j]);
+            ai[j] = fmaf(t.rr, sn, ai[j]);
+            ai[j] = fmaf(t.ri, cs, ai[j]);
+        }
+    }
+
+    float2 *o = out + static_cast<size_t>(split) * g.npx;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (pix[j] < g.npx) o[pix[j]] = make_float2(ar[j], ai[j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain form with the per-term Doppler window and the per-chunk skip test.
+// ---------------------------------------------------------------------------
+
+// Skip test for one pixel and chunk (reference tdcprocessor.cpp:347-383):
+// probes rows max(i*64-1,0) and samples max(j*(ns/2)-1,0), i,j in {0,1,2};
+// beyond the valid rows of a partial last chunk the reference's staging
+// buffer still holds the previous chunk's rows, and a partial first chunk
+// reads freshly allocated (zero) page-locked memory. NaN in the first probe
+// makes the minimum NaN (std::min_element semantics) and skips the chunk.
+__device__ __forceinline__ bool chunk_processed(const DopRec *__restrict__ rec, const Geom &g,
+                                                const DopParams &dp, int chunk, float px, float py,
+                                                float pz)
+{
+    const int valid = min(kChunk, g.npri - chunk * kChunk);
+    const int sstep = g.ns / 2;
+    float mn = 0.0F;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int row = max(i * (kChunk / 2) - 1, 0);
+        int pri = -1;
+        if (row < valid) pri = chunk * kChunk + row;
+        else if (chunk > 0) pri = (chunk - 1) * kChunk + row;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int smp = max(j * sstep - 1, 0);
+            float x = 0.0F, y = 0.0F, z = 0.0F, vx = 0.0F, vy = 0.0F, vz = 0.0F;
+            float fc = dp.centroid ? 0.0F : dp.fc_fixed;
+            if (pri >= 0) {
+                const DopRec t = rec[static_cast<size_t>(pri) * g.ns + smp];
+                x = t.x;
+                y = t.y;
+                z = t.z;
+                vx = t.vx;
+                vy = t.vy;
+                vz = t.vz;
+                fc = t.fc;
+            }
+            const float dx = px - x, dy = py - y, dz = pz - z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float fd = ((vx * dx) + (vy * dy) + (vz * dz)) / __builtin_amdgcn_sqrtf(r2);
+            const float mag = fabsf(fd - fc);
+            if (i == 0 && j == 0) mn = mag;
+            else if (mag < mn) mn = mag;
+        }
+    }
+    return mn <= dp.half_bw;
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restrict__ rec,
+                                                         const float *__restrict__ grid, Geom g,
+                                                         DopParams dp, float2 *__restrict__ out,
+                                                         unsigned long long *__restrict__ counters)
+{
+    int tile, split;
+    map_block(g, tile, split);
+    if (tile >= g.ntiles) return;
+    const int n0 = split * g.cps;
+    const int n1 = min(g.npri, n0 + g.cps);
+
+    float px[P], py[P], pz[P], ar[P], ai[P];
+    int pix[P];
+    bool live[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
+        live[j] = pix[j] < g.npx;
+        const int q = min(pix[j], g.npx - 1);
+        px[j] = grid[3 * q];
+        py[j] = grid[3 * q + 1];
+        pz[j] = grid[3 * q + 2];
+        ar[j] = 0.0F;
+        ai[j] = 0.0F;
+    }
+    unsigned long long processed = 0, skipped = 0;
+
+    int n = n0;
+    while (n < n1) {
+        const int chunk = n / kChunk;
+        const int nend = min(n1, (chunk + 1) * kChunk);
+        bool on[P];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            on[j] = chunk_processed(rec, g, dp, chunk, px[j], py[j], pz[j]);
+            if (live[j] && n == chunk * kChunk) {
+                // count each (pixel, chunk) once: in the split holding its first chirp
+                if (on[j]) ++processed;
+                else ++skipped;
+            }
+            any |= on[j];
+        }
+        if (__ballot(any) != 0ULL) {
+            const DopRec *__restrict__ r = rec + static_cast<size_t>(n) * g.ns;
+            const int nterms = (nend - n) * g.ns;
+#pragma unroll 2
+            for (int e = 0; e < nterms; ++e) {
+                const DopRec t = r[e];
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const float dx = px[j] - t.x;
+                    const float dy = py[j] - t.y;
+                    const float dz = pz[j] - t.z;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float rs = __builtin_amdgcn_rsqf(r2);
+                    // windowing.h:29-37: (2/lambda) v . (p - pos) / |p - pos|
+                    const float fd = ((t.vx * dx) + (t.vy * dy) + (t.vz * dz)) * rs;
+                    const float delta = fd - t.fc;
+                    // windowing.h:43-58; a NaN difference fails the test
+                    const bool inband = on[j] && (fabsf(delta) <= dp.half_bw);
+                    const float wa =
+                        dp.taper ? fmaf(1.0F - dp.az_a, cos_rev(delta * dp.inv_bw), dp.az_a) : 1.0F;
+                    const float th = __builtin_amdgcn_fractf((r2 * rs) * t.k);
+                    const float cs = inband ? cos_rev(th) * wa : 0.0F;
+                    const float sn = inband ? sin_rev(th) * wa : 0.0F;
+                    ar[j] = fmaf(t.rr, cs, ar[j]);
+                    ar[j] = fmaf(-t.ri, sn, ar[j]);
+                    ai[j] = fmaf(t.rr, sn, ai[j]);
+                    ai[j] = fmaf(t.ri, cs, ai[j]);
+                }
+            }
+        }
+        n = nend;
+    }
+
+    float2 *o = out + static_cast<size_t>(split) * g.npx;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (live[j]) o[pix[j]] = make_float2(ar[j], ai[j]);
+    }
+    if (processed) atomicAdd(&counters[0], processed);
+    if (skipped) atomicAdd(&counters[1], skipped);
+}
+
+// Sum of the per-split partial images in split order.
+__global__ __launch_bounds__(kBlock) void finalize_f32_kernel(const float2 *__restrict__ partial,
+                                                              int nsplit, int npx,
+                                                              float2 *__restrict__ img)
+{
+    const int p = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    if (p >= npx) return;
+    float sr = 0.0F, si = 0.0F;
+    for (int s = 0; s < nsplit; ++s) {
+        const float2 v = partial[static_cast<size_t>(s) * npx + p];
+        sr += v.x;
+        si += v.y;
+    }
+    img[p] = make_float2(sr, si);
+}
+
+// ---------------------------------------------------------------------------
+// Precision form.
+// ---------------------------------------------------------------------------
+
+// Raw sample loaders for the three storage formats; the address is uniform.
+struct RawC64 {
+    using T = float2;
+    static __device__ __forceinline__ void get(const T *p, int i, float &re, float &im)
+    {
+        const float2 v = p[i];
+        re = v.x;
+        im = v.y;
+    }
+};
+struct RawF16 {
+    using T = unsigned int;
+    static __device__ __forceinline__ void get(const T *p, int i, float &re, float &im)
+    {
+        const unsigned int v = p[i];
+        re = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v & 0xffffU)));
+        im = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v >> 16)));
+    }
+};
+struct RawBF16 {
+    using T = unsigned int;
+    static __device__ __forceinline__ void get(const T *p, int i, float &re, float &im)
+    {
+        const unsigned int v = p[i];
+        re = __uint_as_float(v << 16);
+        im = __uint_as_float(v & 0xffff0000U);
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void stage_precision_kernel(const float2 *__restrict__ raw,
+                                                                 const float *__restrict__ wr,
+                                                                 int npri, int ns, int storage,
+                                                                 void *__restrict__ out)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    for (size_t e = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * kBlock) {
+        const int m = static_cast<int>(e % ns);
+        const float2 v = raw[e];
+        const float re = v.x * wr[m], im = v.y * wr[m];
+        if (storage == 0) {
+            static_cast<float2 *>(out)[e] = make_float2(re, im);
+        } else if (storage == 1) {
+            const unsigned short hr = __builtin_bit_cast(unsigned short, static_cast<_Float16>(re));
+            const unsigned short hi = __builtin_bit_cast(unsigned short, static_cast<_Float16>(im));
+            static_cast<unsigned int *>(out)[e] = static_cast<unsigned int>(hr) |
+                                                  (static_cast<unsigned int>(hi) << 16);
+        } else {
+            // round to nearest even (finite inputs)
+            unsigned int br = __float_as_uint(re), bi = __float_as_uint(im);
+            br += 0x7fffU + ((br >> 16) & 1U);
+            bi += 0x7fffU + ((bi >> 16) & 1U);
+            static_cast<unsigned int *>(out)[e] = (br >> 16) | (bi & 0xffff0000U);
+        }
+    }
+}
+
+template <int P, class RAW>
+__global__ __launch_bounds__(kBlock) void precision_kernel(
+    const typename RAW::T *__restrict__ raw, const ChirpPos *__restrict__ cpos,
+    const double *__restrict__ grid, const double *__restrict__ kanchor, double dk, Geom g,
+    double2 *__restrict__ out)
+{
+    int tile, split;
+    map_block(g, tile, split);
+    if (tile >= g.ntiles) return;
+    const int n0 = split * g.cps;
+    const int n1 = min(g.npri, n0 + g.cps);
+    const int nblk = (g.ns + kAnchorBlock - 1) / kAnchorBlock;
+
+    double px[P], py[P], pz[P];
+    float accr[P], acci[P], cmpr[P], cmpi[P];
+    int pix[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
+        const int q = min(pix[j], g.npx - 1);
+        px[j] = grid[3 * q];
+        py[j] = grid[3 * q + 1];
+        pz[j] = grid[3 * q + 2];
+        accr[j] = 0.0F;
+        acci[j] = 0.0F;
+        cmpr[j] = 0.0F;
+        cmpi[j] = 0.0F;
+    }
+
+    for (int n = n0; n < n1; ++n) {
+        const ChirpPos c = cpos[n];
+        double rng[P];
+        float wr[P], wi[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const double dx = c.x - px[j], dy = c.y - py[j], dz = c.z - pz[j];
+            rng[j] = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
+            double rot = dk * rng[j];
+            rot -= floor(rot);
+            sincospif(2.0F * static_cast<float>(rot), &wi[j], &wr[j]);
+        }
+        const typename RAW::T *__restrict__ row = raw + static_cast<size_t>(n) * g.ns;
+        for (int b = 0; b < nblk; ++b) {
+            const int a = b * kAnchorBlock;
+            const typename RAW::T *__restrict__ blk = row + a;
+            float sr[P], si[P];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                sr[j] = 0.0F;
+                si[j] = 0.0F;
+            }
+            const int len = min(kAnchorBlock, g.ns - a);
+            if (len == kAnchorBlock) {
+#pragma unroll
+                for (int i = kAnchorBlock - 1; i >= 0; --i) {
+                    float vr, vi;
+                    RAW::get(blk, i, vr, vi);
+#pragma unroll
+                    for (int j = 0; j < P; ++j) {
+                        const float nr = fmaf(sr[j], wr[j], fmaf(-si[j], wi[j], vr));
+                        const float ni = fmaf(sr[j], wi[j], fmaf(si[j], wr[j], vi));
+                        sr[j] = nr;
+                        si[j] = ni;
+                    }
+                }
+            } else {
+                for (int i = len - 1; i >= 0; --i) {
+                    float vr, vi;
+                    RAW::get(blk, i, vr, vi);
+#pragma unroll
+                    for (int j = 0; j < P; ++j) {
+                        const float nr = fmaf(sr[j], wr[j], fmaf(-si[j], wi[j], vr));
+                        const float ni = fmaf(sr[j], wi[j], fmaf(si[j], wr[j], vi));
+                        sr[j] = nr;
+                        si[j] = ni;
+                    }
+                }
+            }
+            const double ka = kanchor[b];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                double th = ka * rng[j];
+                th -= floor(th);
+                const float tf = static_cast<float>(th);
+                const float zr = cos_rev(tf), zi = sin_rev(tf);
+                two_sum_acc(accr[j], cmpr[j], fmaf(zr, sr[j], -(zi * si[j])));
+                two_sum_acc(acci[j], cmpi[j], fmaf(zr, si[j], zi * sr[j]));
+            }
+        }
+    }
+
+    double2 *o = out + static_cast<size_t>(split) * g.npx;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (pix[j] < g.npx) {
+            o[pix[j]] = make_double2(static_cast<double>(accr[j]) + static_cast<double>(cmpr[j]),
+                                     static_cast<double>(acci[j]) + static_cast<double>(cmpi[j]));
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const double2 *__restrict__ partial,
+                                                              int nsplit, int npx,
+                                                              float2 *__restrict__ img)
+{
+    const int p = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    if (p >= npx) return;
+    double sr = 0.0, si = 0.0;
+    for (int s = 0; s < nsplit; ++s) {
+        const double2 v = partial[static_cast<size_t>(s) * npx + p];
+        sr += v.x;
+        si += v.y;
+    }
+    img[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+}
+
+int stage_grid(size_t total)
+{
+    const size_t want = (total + kBlock - 1) / kBlock;
+    return static_cast<int>(want < 4096 ? (want > 0 ? want : 1) : 4096);
+}
+
+int correlate_grid(const Geom &g) { return g.tiles_padded * g.nsplit; }
+
+} // namespace
+
+hipError_t launch_stage_plain(const float *raw, const float *pos, const float *kt, const float *wr,
+                              int npri, int ns, PlainRec *out, hipStream_t s)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    hipLaunchKernelGGL(stage_plain_kernel, dim3(stage_grid(total)), dim3(kBlock), 0, s,
+                       reinterpret_cast<const float2 *>(raw), pos, kt, wr, npri, ns, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_doppler(const float *raw, const float *pos, const float *vel,
+                                const float *att, const float *kt, const float *wr, int npri,
+                                int ns, float two_over_lambda, const float *bore, int centroid,
+                                float fc_fixed, DopRec *out, hipStream_t s)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    hipLaunchKernelGGL(stage_doppler_kernel, dim3(stage_grid(total)), dim3(kBlock), 0, s,
+                       reinterpret_cast<const float2 *>(raw), pos, vel,
+                       reinterpret_cast<const float4 *>(att), kt, wr, npri, ns, two_over_lambda,
+                       make_float3(bore[0], bore[1], bore[2]), centroid, fc_fixed, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain(int ppl, const PlainRec *rec, const float *grid, const Geom &g,
+                        float2 *out, hipStream_t s, int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    switch (ppl) {
+    case 1:
+        hipLaunchKernelGGL(plain_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL(plain_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, out);
+        break;
+    case 4:
+        hipLaunchKernelGGL(plain_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, out);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const Geom &g,
+                          const DopParams &dp, float2 *out, unsigned long long *counters,
+                          hipStream_t s, int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    switch (ppl) {
+    case 1:
+        hipLaunchKernelGGL(doppler_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp, out,
+                           counters);
+        break;
+    case 2:
+        hipLaunchKernelGGL(doppler_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp, out,
+                           counters);
+        break;
+    case 4:
+        hipLaunchKernelGGL(doppler_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp, out,
+                           counters);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_f32(const float2 *partial, int nsplit, int npx, float2 *img,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(finalize_f32_kernel, dim3((npx + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       partial, nsplit, npx, img);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_precision(int storage, const float *raw, const float *wr, int npri, int ns,
+                                  void *out, hipStream_t s)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    hipLaunchKernelGGL(stage_precision_kernel, dim3(stage_grid(total)), dim3(kBlock), 0, s,
+                       reinterpret_cast<const float2 *>(raw), wr, npri, ns, storage, out);
+    return hipGetLastError();
+}
+
+template <class RAW>
+static hipError_t launch_precision_t(int ppl, const void *raw, const ChirpPos *cpos,
+                                     const double *grid, const double *kanchor, double dk,
+                                     const Geom &g, double2 *out, hipStream_t s)
+{
+    const int nb = correlate_grid(g);
+    const auto *r = static_cast<const typename RAW::T *>(raw);
+    switch (ppl) {
+    case 1:
+        hipLaunchKernelGGL((precision_kernel<1, RAW>), dim3(nb), dim3(kBlock), 0, s, r, cpos, grid,
+                           kanchor, dk, g, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL((precision_kernel<2, RAW>), dim3(nb), dim3(kBlock), 0, s, r, cpos, grid,
+                           kanchor, dk, g, out);
+        break;
+    case 4:
+        hipLaunchKernelGGL((precision_kernel<4, RAW>), dim3(nb), dim3(kBlock), 0, s, r, cpos, grid,
+                           kanchor, dk, g, out);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_precision(int ppl, int storage, const void *raw, const ChirpPos *cpos,
+                            const double *grid, const double *kanchor, double dk, const Geom &g,
+                            double2 *out, hipStream_t s, int *workgroups)
+{
+    *workgroups = correlate_grid(g);
+    switch (storage) {
+    case 0:
+        return launch_precision_t<RawC64>(ppl, raw, cpos, grid, kanchor, dk, g, out, s);
+    case 1:
+        return launch_precision_t<RawF16>(ppl, raw, cpos, grid, kanchor, dk, g, out, s);
+    case 2:
+        return launch_precision_t<RawBF16>(ppl, raw, cpos, grid, kanchor, dk, g, out, s);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(finalize_f64_kernel, dim3((npx + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       partial, nsplit, npx, img);
+    return hipGetLastError();
+}
+
+} // namespace tdc

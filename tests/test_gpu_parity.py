@@ -1,0 +1,425 @@
+"""GPU parity: the HIP correlate-sum path against the FP64 oracle.
+
+Tolerances (relative L2 over the compared pixels, and the phase error at the
+brightest oracle pixel):
+  plain form      2e-3 / 2e-3 rad -- FP32 range and phase: the phase k*R of
+                  ~1e3 cycles carries ~1e-4 cycles of rounding per term.
+  precision form  1e-4 / 1e-4 rad -- the north-star bound; FP64 range/anchor.
+  FP16 raw        3e-3            -- storage rounding 2^-11 per component.
+  BF16 raw        2e-2            -- storage rounding 2^-8 per component.
+  Doppler window  5e-3            -- terms within rounding of the band edge may
+                  fall either side in FP32 vs FP64.
+"""
+import numpy as np
+import pytest
+
+from conftest import subset
+
+pytestmark = pytest.mark.gpu
+
+PLAIN_TOL = 2e-3
+PREC_TOL = 1e-4
+
+
+def form(tdc, d=None, f64=None, center=0.0, bw=1e30, centroid=False, rw=False, taper=False,
+         kind="plain", storage="c64", ppl=0, splits=0):
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    if f64 is None:
+        p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"], d["velocity"],
+                      d["attitude"], d["modRate"], d["startFreq"], d["bodyBoresight"])
+        p.setFocusGrid(d["focusGrid"])
+    else:
+        p.setRawDataF64(f64["rawData"], f64["priTimes"], f64["sampleTimes"], f64["position"],
+                        f64["modRate"], f64["startFreq"])
+        p.setFocusGridF64(f64["focusGrid"])
+    p.setForm(kind)
+    p.setRawStorage(storage)
+    p.setLaunch(ppl, splits)
+    p.start(center, bw, centroid, rw, taper)
+    return p.getFocusedImage(), p.stats()
+
+
+def oracle_dropin(orc, d, rw=None, doppler=None, pixels=None):
+    return orc.image_f64(d["rawData"], d["sampleTimes"], d["position"], d["focusGrid"],
+                         float(d["modRate"]), float(d["startFreq"]), range_win=rw,
+                         doppler=doppler, pixels=pixels)
+
+
+def oracle_f64(orc, s, pixels=None, rw=None):
+    return orc.image_f64(s.raw, s.sample_times, s.track, s.grid, s.mod_rate, s.f0,
+                         range_win=rw, pixels=pixels)
+
+
+def check(orc, img, ref, tol, ptol=None):
+    e = orc.rel_l2(img, ref)
+    ph = orc.peak_phase_error(img, ref)
+    print(f"rel-L2 {e:.3e}  peak phase {ph:.3e} rad")
+    assert e <= tol, e
+    assert ph <= (ptol if ptol is not None else tol), ph
+    return e, ph
+
+
+def ragged_scene(npri=130, ns=100, rows=13, cols=21, seed=3):
+    from sarlabtdc import scene
+
+    rng = np.random.default_rng(seed)
+    track = np.zeros((npri, 3))
+    track[:, 0] = np.linspace(-0.2, 0.2, npri)
+    track += rng.normal(scale=1e-4, size=track.shape)
+    grid = scene.plane_grid(rows, cols, 4e-3, (0.01, 0.6, 0.0))
+    scat = scene._uniform_on_grid(rng, grid, 6)
+    amp = scene._cn(rng, 6, 1.0)
+    return scene.synthesize("ragged", track, ns, grid, scat, amp, 30.0, rng, speed=10.0)
+
+
+# ---------------------------------------------------------------------------- plain form
+
+
+def test_plain_config1(tdc, orc):
+    from sarlabtdc import scene
+
+    s = scene.config1()
+    d = s.dropin()
+    img, st = form(tdc, d)
+    assert st["form"] == "plain" and not st["doppler_active"]
+    assert img.shape == (32, 32) and img.dtype == np.complex64
+    check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
+
+
+def test_plain_config1_hamming_range_window(tdc, orc):
+    from sarlabtdc import scene
+
+    d = scene.config1().dropin()
+    img, _ = form(tdc, d, rw=True)
+    ref = oracle_dropin(orc, d, rw=orc.range_window(d["rawData"].shape[1]))
+    check(orc, img, ref, PLAIN_TOL)
+
+
+@pytest.mark.parametrize("window", [("hann", 0.0, 0.0), ("taylor", 4.0, 30.0)])
+def test_plain_other_range_windows(tdc, orc, window):
+    from sarlabtdc import scene
+
+    d = scene.config1().dropin()
+    ns = d["rawData"].shape[1]
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"], d["velocity"],
+                 d["attitude"], d["modRate"], d["startFreq"], d["bodyBoresight"])
+    p.setFocusGrid(d["focusGrid"])
+    p.setRangeWindow(*window)
+    p.start(0.0, 1e30, False, True, False)
+    img = p.getFocusedImage()
+    if window[0] == "hann":
+        rw = orc.range_window(ns, 0.5)
+    else:
+        rw = orc.taylor_window(ns, 4, 30.0)
+    check(orc, img, oracle_dropin(orc, d, rw=rw), PLAIN_TOL)
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("splits", [0, 1, 3, 8])
+def test_plain_launch_shapes_ragged(tdc, orc, ppl, splits):
+    s = ragged_scene()
+    d = s.dropin()
+    img, st = form(tdc, d, ppl=ppl, splits=splits)
+    assert st["pixels_per_lane"] == ppl
+    if splits:
+        assert st["chirp_splits"] == splits
+    check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
+
+
+def test_plain_config2_subset(tdc, orc):
+    from sarlabtdc import scene
+
+    s = scene.config2()
+    d = s.dropin()
+    img, st = form(tdc, d)
+    flat = img.reshape(-1)
+    px = subset(flat.size, 768, extra=np.argsort(np.abs(flat))[-16:])
+    ref = oracle_dropin(orc, d, pixels=px)
+    check(orc, flat[px], ref, PLAIN_TOL)
+
+
+def test_plain_point_target_closed_form(tdc):
+    """One noiseless scatterer on a pixel, windows off: I(p0) = a * Nc * Ns."""
+    from sarlabtdc import scene
+
+    s = scene.config1()
+    p0 = s.grid[11, 17]
+    amp = 0.8 - 0.6j
+    raw = scene.simulate_beat(s.track, s.sample_times, p0[None], [amp]).astype(np.complex64)
+    s.raw = raw
+    d = s.dropin()
+    d["focusGrid"] = np.ascontiguousarray(p0[None, None, :].astype(np.float32))
+    # the oracle convention rounds the track to float32 too; use the float32 pixel
+    img, _ = form(tdc, d)
+    want = amp * raw.size
+    assert abs(img[0, 0] - want) / abs(want) < 2e-3
+
+
+def test_plain_linearity(tdc):
+    from sarlabtdc import scene
+
+    s = scene.config1()
+    d = s.dropin()
+    rng = np.random.default_rng(11)
+    r2 = (rng.normal(size=d["rawData"].shape) + 1j * rng.normal(size=d["rawData"].shape)).astype(
+        np.complex64)
+    i1, _ = form(tdc, d)
+    d2 = dict(d, rawData=r2)
+    i2, _ = form(tdc, d2)
+    d3 = dict(d, rawData=(2.0 * d["rawData"] + r2).astype(np.complex64))
+    i3, _ = form(tdc, d3)
+    lhs = i3.astype(np.complex128)
+    rhs = 2.0 * i1.astype(np.complex128) + i2
+    assert np.linalg.norm(lhs - rhs) / np.linalg.norm(rhs) < 1e-5
+
+
+def test_plain_deterministic_reruns(tdc):
+    from sarlabtdc import scene
+
+    d = scene.config1().dropin()
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"], d["velocity"],
+                 d["attitude"], d["modRate"], d["startFreq"], d["bodyBoresight"])
+    p.setFocusGrid(d["focusGrid"])
+    p.start(0.0, 1e30, False, False, False)
+    a = p.getFocusedImage()
+    p.start(0.0, 1e30, False, False, False)
+    b = p.getFocusedImage()
+    p.prepare(0.0, 1e30, False, False, False)
+    p.run(3)
+    p.fetchImage()
+    c = p.getFocusedImage()
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+# ---------------------------------------------------------------------------- precision form
+
+
+def test_precision_config2_c64(tdc, orc):
+    from sarlabtdc import scene
+
+    s = scene.config2()
+    img, st = form(tdc, f64=s.f64(), kind="precision")
+    assert st["form"] == "precision" and st["raw_storage"] == "c64"
+    flat = img.reshape(-1)
+    px = subset(flat.size, 768, extra=np.argsort(np.abs(flat))[-16:])
+    check(orc, flat[px], oracle_f64(orc, s, pixels=px), PREC_TOL)
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 3e-3), ("bf16", 2e-2)])
+def test_precision_config2_half_storage(tdc, orc, storage, tol):
+    from sarlabtdc import scene
+
+    s = scene.config2()
+    img, st = form(tdc, f64=s.f64(), kind="precision", storage=storage)
+    assert st["raw_storage"] == storage
+    flat = img.reshape(-1)
+    px = subset(flat.size, 512, extra=np.argsort(np.abs(flat))[-16:])
+    check(orc, flat[px], oracle_f64(orc, s, pixels=px), tol)
+
+
+def test_precision_from_float_api(tdc, orc):
+    """Precision form selected on the reference's float32 inputs (stop-and-hop)."""
+    from sarlabtdc import scene
+
+    d = scene.config1().dropin()
+    img, st = form(tdc, d, kind="precision")
+    assert st["form"] == "precision"
+    check(orc, img, oracle_dropin(orc, d), PREC_TOL)
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4])
+@pytest.mark.parametrize("splits", [0, 3])
+def test_precision_ragged(tdc, orc, ppl, splits):
+    """130 chirps, 100 samples (a 36-sample tail block), 13 x 21 pixels."""
+    s = ragged_scene()
+    img, _ = form(tdc, f64=s.f64(), kind="precision", ppl=ppl, splits=splits)
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
+def test_precision_range_window(tdc, orc):
+    s = ragged_scene()
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    f = s.f64()
+    p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"], f["modRate"],
+                    f["startFreq"])
+    p.setFocusGridF64(f["focusGrid"])
+    p.start(0.0, 1e30, False, True, False)
+    img = p.getFocusedImage()
+    rw = orc.range_window(s.n_samples)
+    check(orc, img, oracle_f64(orc, s, rw=rw), PREC_TOL)
+
+
+def test_precision_point_target_closed_form(tdc):
+    from sarlabtdc import scene
+
+    s = scene.config2(nc=128, ns=128, side=8)
+    p0 = s.grid[3, 5]
+    amp = -0.3 + 0.9j
+    s.raw = scene.simulate_beat(s.track, s.sample_times, p0[None], [amp]).astype(np.complex64)
+    s.grid = p0[None, None, :].copy()
+    img, _ = form(tdc, f64=s.f64(), kind="precision")
+    want = amp * s.raw.size
+    assert abs(img[0, 0] - want) / abs(want) < 1e-5
+
+
+def test_precision_rejects_nonuniform_times(tdc):
+    s = ragged_scene()
+    f = s.f64()
+    f["sampleTimes"] = f["sampleTimes"] ** 1.01
+    with pytest.raises(RuntimeError, match="uniformly spaced"):
+        form(tdc, f64=f, kind="precision")
+
+
+def test_precision_rejects_moving_phase_centre(tdc):
+    d = ragged_scene().dropin()
+    d["position"] = d["position"].copy()
+    d["position"][:, :, 0] += np.linspace(0, 1e-4, d["position"].shape[1])[None, :]
+    with pytest.raises(RuntimeError, match="stop-and-hop"):
+        form(tdc, d, kind="precision")
+
+
+# ---------------------------------------------------------------------------- Doppler window
+
+
+def doppler_scene(npri=300, ns=64, side=16, speed=40.0, seed=5):
+    from sarlabtdc import scene
+
+    rng = np.random.default_rng(seed)
+    track = np.zeros((npri, 3))
+    track[:, 0] = np.linspace(-0.5, 0.5, npri)
+    grid = scene.plane_grid(side, side, 5e-3, (0.0, 0.5, 0.0))
+    scat = scene._uniform_on_grid(rng, grid, 5)
+    amp = scene._cn(rng, 5, 1.0)
+    s = scene.synthesize("doppler", track, ns, grid, scat, amp, 30.0, rng, speed=speed)
+    return s
+
+
+def doppler_dict(d, center, bw, centroid, taper):
+    return dict(velocity=d["velocity"], attitude=d["attitude"], boresight=d["bodyBoresight"],
+                center=center, bandwidth=bw, centroid=centroid, taper=taper)
+
+
+@pytest.mark.parametrize("centroid,taper,center,bw", [
+    (True, True, 0.0, 8e3),
+    (False, False, 2e3, 6e3),
+    (True, False, 0.0, 12e3),
+])
+def test_doppler_window_and_skip(tdc, orc, centroid, taper, center, bw):
+    s = doppler_scene()
+    d = s.dropin()
+    img, st = form(tdc, d, center=center, bw=bw, centroid=centroid, taper=taper)
+    assert st["doppler_active"]
+    dop = doppler_dict(d, center, bw, centroid, taper)
+    ref = oracle_dropin(orc, d, doppler=dop)
+    # skip counts against the oracle's own skip test
+    g = d["focusGrid"].reshape(-1, 3).astype(np.float64)
+    nchunks = (d["rawData"].shape[0] + 127) // 128
+    processed = sum(
+        orc.chunk_processed(d["rawData"], d["sampleTimes"], d["position"], float(d["modRate"]),
+                            float(d["startFreq"]), dop, c, g[i])
+        for i in range(g.shape[0]) for c in range(nchunks))
+    assert st["processed_chunks"] + st["skipped_chunks"] == g.shape[0] * nchunks
+    assert st["skipped_chunks"] > 0
+    assert st["processed_chunks"] == processed
+    check(orc, img, ref, 5e-3)
+
+
+def test_doppler_matches_reference_method(tdc, orc):
+    """GPU Doppler path vs the FP32 restatement of the reference method."""
+    s = doppler_scene()
+    d = s.dropin()
+    img, _ = form(tdc, d, center=0.0, bw=8e3, centroid=True, taper=True, rw=True)
+    rm = orc.refmethod_f32(d, 0.0, 8e3, True, True, True)
+    assert orc.rel_l2(img, rm) < 5e-3
+
+
+def test_doppler_trivial_window_uses_plain_kernel(tdc, orc):
+    s = doppler_scene()
+    d = s.dropin()
+    img, st = form(tdc, d, center=0.0, bw=1e9, centroid=True, taper=False)
+    assert not st["doppler_active"]
+    check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
+
+
+def test_pixel_on_antenna_position_takes_exact_path(tdc, orc):
+    """A pixel exactly at an antenna position makes the reference's window 0/0."""
+    s = doppler_scene(npri=140, side=4)
+    d = s.dropin()
+    g = d["focusGrid"].copy()
+    g[0, 0] = d["position"][3, 0]
+    d["focusGrid"] = g
+    img, st = form(tdc, d, center=0.0, bw=1e9, centroid=True, taper=False)
+    assert st["doppler_active"]
+    dop = doppler_dict(d, 0.0, 1e9, True, False)
+    ref = oracle_dropin(orc, d, doppler=dop)
+    assert np.isfinite(img).all()
+    assert orc.rel_l2(img, ref) < 5e-3
+
+
+# ---------------------------------------------------------------------------- boundary behaviour
+
+
+def test_empty_grid(tdc):
+    d = ragged_scene().dropin()
+    d["focusGrid"] = np.zeros((0, 5, 3), dtype=np.float32)
+    img, _ = form(tdc, d)
+    assert img.shape == (0, 5)
+
+
+def test_single_term(tdc, orc):
+    s = ragged_scene(npri=1, ns=1, rows=1, cols=1)
+    d = s.dropin()
+    img, _ = form(tdc, d)
+    check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
+
+
+def test_errors_mirror_reference(tdc):
+    d = ragged_scene().dropin()
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    with pytest.raises(RuntimeError):
+        p.getFocusedImage()
+    with pytest.raises(RuntimeError, match="rawData must be 2D"):
+        p.setRawData(d["rawData"][None], d["priTimes"], d["sampleTimes"], d["position"],
+                     d["velocity"], d["attitude"], d["modRate"], d["startFreq"],
+                     d["bodyBoresight"])
+    with pytest.raises(RuntimeError, match="position shape is incorrect"):
+        p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"][:, :-1],
+                     d["velocity"], d["attitude"], d["modRate"], d["startFreq"],
+                     d["bodyBoresight"])
+    with pytest.raises(RuntimeError, match="focusGrid must be 3D"):
+        p.setFocusGrid(d["focusGrid"][0])
+    with pytest.raises(ValueError, match="nPri must be greater than 0"):
+        z = np.zeros((0, 4), np.complex64)
+        p.setRawData(z, np.zeros(0, np.float32), np.zeros(4, np.float32),
+                     np.zeros((0, 4, 3), np.float32), np.zeros((0, 4, 3), np.float32),
+                     np.zeros((0, 4, 4), np.float32), 1.0, 1.0, d["bodyBoresight"])
+    with pytest.raises(TypeError):
+        p.setFocusGrid(d["focusGrid"].astype(np.float64))
+    with pytest.raises(RuntimeError):
+        tdc.TdcProcessor(9999)
+
+
+def test_reference_smoke_all_zero(tdc):
+    """The reference's SmokeTest (tdctest/tdctest.cpp:21-59): all-zero inputs."""
+    n_rows = n_cols = 20
+    npri, ns = 750, 1500
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setFocusGrid(np.zeros((n_rows, n_cols, 3), np.float32))
+    p.setRawData(np.zeros((npri, ns), np.complex64), np.zeros(npri, np.float32),
+                 np.zeros(ns, np.float32), np.zeros((npri, ns, 3), np.float32),
+                 np.zeros((npri, ns, 3), np.float32), np.zeros((npri, ns, 4), np.float32),
+                 1.0, 1.0, np.array([0, 1, 0], np.float32))
+    p.start(0.0, 5.0, True, True)
+    img = p.getFocusedImage()
+    st = p.stats()
+    # pos = target = 0 makes every probe NaN: every (pixel, chunk) is skipped
+    assert st["skipped_chunks"] == n_rows * n_cols * 6 and st["processed_chunks"] == 0
+    assert np.array_equal(img, np.zeros((n_rows, n_cols), np.complex64))
