@@ -1,0 +1,121 @@
+"""The C ABI library: it loads, exports every symbol its header declares, and
+reports errors through status codes and tdc_last_error() (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sarlabtdc_amd.h")
+LIB = os.path.join(ROOT, "sarlab-tdc_amd", "sarlabtdc", "libsarlabtdc_amd.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tdc_[a-z0-9_]+)\s*\(", text)))
+
+
+def status_codes():
+    text = open(HEADER).read()
+    return {k: int(v) for k, v in re.findall(r"#define (TDC_[A-Z0-9_]+) (\d+)", text)}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    lib = ctypes.CDLL(LIB)
+    lib.tdc_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def test_header_declares_the_reference_boundary():
+    names = declared_functions()
+    for n in ("tdc_create", "tdc_destroy", "tdc_set_raw_data", "tdc_set_focus_grid",
+              "tdc_start", "tdc_image_buffer", "tdc_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_abi_version_and_device_count(lib):
+    assert lib.tdc_abi_version() == 10000
+    assert lib.tdc_device_count() >= 0
+
+
+def test_null_handle_is_invalid_argument(lib):
+    codes = status_codes()
+    bad = codes["TDC_ERR_INVALID_ARGUMENT"]
+    assert lib.tdc_set_focus_grid(None, None, 0, 0) == bad
+    assert b"NULL" in lib.tdc_last_error()
+    assert lib.tdc_start(None, ctypes.c_float(0), ctypes.c_float(1), 0, 0, 0) == bad
+    assert lib.tdc_run(None, 1) == bad
+    assert lib.tdc_destroy(None) == codes["TDC_OK"]
+    assert lib.tdc_last_error() == b""
+
+
+def test_create_without_device_reports_device_error(lib):
+    if lib.tdc_device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    h = ctypes.c_void_p()
+    assert lib.tdc_create(0, ctypes.byref(h)) == status_codes()["TDC_ERR_DEVICE"]
+    assert h.value is None
+    assert b"Failed to set HIP device" in lib.tdc_last_error()
+
+
+def test_python_module_without_device_raises_runtime_error():
+    import sarlabtdc
+
+    if sarlabtdc.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(RuntimeError, match="Failed to set HIP device"):
+        sarlabtdc.TdcProcessor(0)
+
+
+def test_python_surface_matches_reference():
+    """Method and keyword names of py-sarlabtdc/src/interface.cpp:15-22."""
+    import sarlabtdc
+
+    cls = sarlabtdc.TdcProcessor
+    for name in ("start", "setRawData", "setFocusGrid", "getFocusedImage"):
+        assert hasattr(cls, name)
+    doc = cls.start.__doc__
+    for kw in ("dopplerWinCenter", "dopplerBandwidth", "dopCentroidWin", "applyRangeWin",
+               "dopplerWinTaper: bool = True"):
+        assert kw in doc
+
+
+def test_cpp_header_compiles_against_library(tmp_path):
+    """A C++ user of tdcprocessor.h (the reference's tdctest SmokeTest shape) builds."""
+    src = tmp_path / "user.cpp"
+    src.write_text(r'''
+#include <complex>
+#include <vector>
+#include "tdcprocessor.h"
+int main() {
+    TdcProcessor tdc(0);
+    std::vector<float> grid(20 * 20 * 3);
+    tdc.setFocusGrid(grid.data(), 20, 20);
+    const int nPri = 750, nSamples = 1500;
+    std::vector<std::complex<float>> raw(nPri * nSamples);
+    std::vector<float> priTimes(nPri), sampleTimes(nSamples), position(raw.size() * 3),
+        velocity(raw.size() * 3), attitude(raw.size() * 4);
+    float3 bore; bore.x = 0; bore.y = 1; bore.z = 0;
+    tdc.setRawData(raw.data(), priTimes.data(), sampleTimes.data(), position.data(),
+                   velocity.data(), attitude.data(), nPri, nSamples, 1.0f, 1.0f, bore);
+    tdc.setLoggerSinks({std::make_shared<tdclog::stdout_sink>()});
+    tdc.start(0.0f, 5.0f, true, true);
+    return tdc.imageBuffer() == nullptr;
+}
+''')
+    import subprocess
+
+    pkg = os.path.join(ROOT, "sarlab-tdc_amd", "sarlabtdc")
+    cmd = ["/opt/rocm/bin/hipcc", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I/opt/rocm/include", str(src), "-L", pkg,
+           "-lsarlabtdc_amd", "-Wl,-rpath," + pkg, "-o", str(tmp_path / "user")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
