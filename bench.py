@@ -85,6 +85,97 @@ def time_variant(tdc, s, kind, storage, steps, warmup):
                 workgroups=st["workgroups"], image=img, keep=keep)
 
 
+def serve_stream(tdc, scenes, rate, form, storage="c64", seed=0, budget=200e9):
+    """Poisson arrivals at `rate` jobs/s into one server; returns per-job records."""
+    import threading
+
+    rng = np.random.default_rng([5, seed, 1])
+    gaps = rng.exponential(1.0 / rate, size=len(scenes))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    srv = tdc.Server(0, budget)
+    # warm-up: one job per grid bucket shape class, not counted
+    for s in scenes[:3]:
+        f = s.f64()
+        srv.wait(srv.submit(f["rawData"], f["sampleTimes"], f["position"], f["focusGrid"],
+                            f["modRate"], f["startFreq"], form=form, storage=storage))
+    inputs = [s.f64() for s in scenes]
+    ids = [None] * len(scenes)
+    results = [None] * len(scenes)
+    ready = threading.Semaphore(0)
+
+    def waiter():
+        for k in range(len(scenes)):
+            ready.acquire()
+            results[k] = srv.wait(ids[k])
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    for k, f in enumerate(inputs):
+        dt = arrivals[k] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids[k] = srv.submit(f["rawData"], f["sampleTimes"], f["position"], f["focusGrid"],
+                            f["modRate"], f["startFreq"], form=form, storage=storage)
+        ready.release()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r[1]["latency_ms"] for r in results])
+    t_sub = min(r[1]["t_submit"] for r in results)
+    t_end = max(r[1]["t_done"] for r in results)
+    terms = sum(r[1]["terms"] for r in results)
+    makespan = t_end - t_sub
+    return dict(form=form, storage=storage, jobs=len(scenes), rate=rate,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_ms=float(lat.mean()), max_ms=float(lat.max()), makespan_s=makespan,
+                gterms=terms / makespan / 1e9, terms=terms, steps=st["steps"],
+                device_busy_s=st["device_busy_ms"] / 1e3, bytes_peak=st["bytes_peak"],
+                graphs=st["graphs"]), results
+
+
+def serve_leg(tdc, args):
+    from sarlabtdc import scene
+
+    shapes = scene.config5_job_shapes(args.serve_jobs)
+    t0 = time.perf_counter()
+    scenes = []
+    for j, sh in enumerate(shapes):
+        scenes.append(scene.config5_job(j, sh))
+        if (j + 1) % 200 == 0:
+            log(f"serving: generated {j + 1}/{len(shapes)} jobs ({time.perf_counter() - t0:.1f} s)")
+    out = {}
+    rec, res = serve_stream(tdc, scenes, args.serve_rate, "precision")
+    log(f"serving precision: {rec['jobs']} jobs at {rec['rate']:.0f}/s: p50 {rec['p50_ms']:.2f} ms, "
+        f"p99 {rec['p99_ms']:.2f} ms, {rec['gterms']:.1f} G terms/s over {rec['makespan_s']:.2f} s "
+        f"(device busy {rec['device_busy_s']:.2f} s, {rec['steps']} steps)")
+    out["precision"] = rec
+    n_plain = min(len(scenes), args.serve_plain_jobs)
+    if n_plain > 0:
+        rec_p, _ = serve_stream(tdc, scenes[:n_plain], args.serve_rate, "plain")
+        log(f"serving plain: {rec_p['jobs']} jobs at {rec_p['rate']:.0f}/s: p50 "
+            f"{rec_p['p50_ms']:.2f} ms, p99 {rec_p['p99_ms']:.2f} ms, {rec_p['gterms']:.1f} G terms/s")
+        out["plain"] = rec_p
+    # accuracy of a few served jobs against the FP64 oracle (checker only)
+    if not args.no_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        rng = np.random.default_rng(55)
+        errs = []
+        for k in rng.choice(len(scenes), size=min(4, len(scenes)), replace=False):
+            s = scenes[k]
+            img = res[k][0].reshape(-1)
+            px = np.unique(np.concatenate([rng.choice(img.size, size=min(256, img.size),
+                                                      replace=False), [int(np.argmax(np.abs(img)))]]))
+            ref = oracle.image_f64(s.raw, s.sample_times, s.track, s.grid, s.mod_rate, s.f0,
+                                   pixels=px)
+            errs.append(oracle.rel_l2(img[px], ref))
+        out["precision"]["rel_l2_max_checked"] = float(max(errs))
+        log(f"serving precision: rel-L2 of {len(errs)} checked jobs <= {max(errs):.3e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +187,10 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-jobs", type=int, default=1000, help="config 5 jobs (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="offered jobs/s")
+    ap.add_argument("--serve-plain-jobs", type=int, default=200,
+                    help="jobs of the stream also served in the plain form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -183,6 +278,8 @@ def main():
                           f"{s.n_samples} samples, FP32 reference method, {dt:.1f} s")
         log(f"cpu baseline: {cpu['value']:.4f} G terms/s on {cores} cores ({dt:.1f} s)")
 
+    serving = serve_leg(tdc, args) if args.serve_jobs > 0 else None
+
     head = results.get("c64") or next(iter(results.values()))
     kind = head["kind"]
     flop = FLOP_PER_TERM[kind]
@@ -230,6 +327,15 @@ def main():
             for v, r in results.items()
         },
     }
+    if serving:
+        out["serving"] = {
+            k: {kk: (float(f"{vv:.5g}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
+            for k, v in serving.items()}
+        pr = serving["precision"]
+        out["p50_ms"] = round(pr["p50_ms"], 3)
+        out["p99_ms"] = round(pr["p99_ms"], 3)
+        out["serving"]["precision"]["fp32_peak_frac"] = round(
+            pr["gterms"] * 1e9 * FLOP_PER_TERM["precision"] / (PEAK_FP32_TFLOPS * 1e12), 4)
     if "plain" in results and "c64" in results:
         out["precision_vs_plain"] = round(results["c64"]["gterms"] / results["plain"]["gterms"], 3)
     print(json.dumps(out), flush=True)

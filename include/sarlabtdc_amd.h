@@ -154,8 +154,78 @@ int tdc_fetch_image(tdc_processor *h);
 int tdc_get_stats(const tdc_processor *h, tdc_stats *out);
 
 /* Force a launch configuration for tuning and tests: pixels per lane
- * (1, 2 or 4) and the chirp split count (0 = automatic). */
+ * (0 = automatic; 1, 2 or 4 for the plain form, up to 8 for the precision
+ * form) and the chirp split count (0 = automatic). */
 int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits);
+
+/* ---- serving form (additive; no counterpart in the reference) ------------- */
+/* A server owns one GPU stream and a scheduler thread. Jobs of any shape are
+ * submitted at any time; the scheduler admits them in arrival order while
+ * their device footprint fits the HBM budget, keeps their inputs, partial and
+ * final images resident, and advances all admitted jobs together in short
+ * steps (continuous batching): each step is one persistent correlate-sum
+ * kernel draining a table of (job, pixel tile, chirp range) tiles, replayed
+ * as a hipGraph per grid-size bucket. A job's tile decomposition and
+ * reduction order depend only on its own shape, so its image is the same
+ * bit for bit alone or in any mix. */
+
+typedef struct tdc_server tdc_server;
+
+#define TDC_JOB_QUEUED 0
+#define TDC_JOB_ADMITTED 1
+#define TDC_JOB_DONE 2
+#define TDC_JOB_FAILED 3
+
+typedef struct tdc_job_desc {
+    const float *raw_iq;          /* [n_pri][n_samples][2] complex64, windows applied */
+    const double *sample_times;   /* [n_samples], uniformly spaced */
+    const double *chirp_position; /* [n_pri][3] phase centre per chirp */
+    const double *grid;           /* [n_rows][n_cols][3] */
+    int32_t n_pri, n_samples, n_rows, n_cols;
+    double mod_rate, start_freq;
+    int32_t form;                 /* TDC_FORM_PLAIN or TDC_FORM_PRECISION */
+    int32_t raw_storage;          /* TDC_RAW_* (precision form) */
+} tdc_job_desc;
+
+typedef struct tdc_job_info {
+    int64_t id;
+    int32_t state;        /* TDC_JOB_* */
+    int32_t status;       /* TDC_OK, or the error that failed the job */
+    double t_submit_s;    /* server clock, seconds since tdc_server_create */
+    double t_admit_s;
+    double t_first_step_s;
+    double t_done_s;      /* image copied to host memory */
+    double terms;         /* pixels * chirps * samples */
+    int64_t device_bytes; /* HBM footprint charged against the budget */
+    int32_t tiles;
+    int32_t steps;        /* steps the job's tiles ran in */
+} tdc_job_info;
+
+typedef struct tdc_server_stats {
+    int64_t submitted, completed, failed, steps;
+    double device_busy_ms; /* summed device time of the steps (HIP events) */
+    double terms_done;
+    int64_t bytes_in_use, bytes_peak;
+    int32_t queued, active, graphs;
+} tdc_server_stats;
+
+/* hbm_budget_bytes: device memory the admitted jobs may hold together;
+ * step_terms: work per step (pixels*chirps*samples; 0 = default 4e10). */
+int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, tdc_server **out);
+/* Queues a job and returns at once. The arrays stay borrowed until the job
+ * is DONE or FAILED. A job larger than the whole budget fails at submit. */
+int tdc_server_submit(tdc_server *s, const tdc_job_desc *job, int64_t *job_id);
+/* *done = 1 once the job is DONE or FAILED. */
+int tdc_server_poll(tdc_server *s, int64_t job_id, int *done);
+/* Blocks until the job ends, copies its [n_rows][n_cols][2] image into
+ * image_iq (may be NULL) and its record into info (may be NULL), then
+ * forgets the job. Returns the job's status. */
+int tdc_server_wait(tdc_server *s, int64_t job_id, float *image_iq, tdc_job_info *info);
+int tdc_server_get_stats(tdc_server *s, tdc_server_stats *out);
+/* Seconds on the server clock (the clock of tdc_job_info). */
+int tdc_server_clock(tdc_server *s, double *seconds);
+/* Finishes the steps in flight, fails the jobs still queued, frees everything. */
+int tdc_server_destroy(tdc_server *s);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@
 
 #include <complex>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -231,6 +232,134 @@ private:
     bool started_ = false;
 };
 
+void check_status(int rc)
+{
+    if (rc == TDC_OK) return;
+    if (rc == TDC_ERR_INVALID_ARGUMENT) throw std::invalid_argument(tdc_last_error());
+    throw std::runtime_error(tdc_last_error());
+}
+
+// Serving form: continuously batched jobs on one GPU.
+class PyServer
+{
+public:
+    PyServer(int gpuNum, double hbmBudgetBytes, double stepTerms)
+    {
+        check_status(tdc_server_create(gpuNum, hbmBudgetBytes, stepTerms, &s_));
+    }
+    ~PyServer() { close(); }
+
+    int64_t submit(c64_array rawData, f64_array sampleTimes, f64_array position, f64_array focusGrid,
+                   double modRate, double startFreq, const py::object &form,
+                   const py::object &storage)
+    {
+        if (!s_) throw std::runtime_error("the server is closed");
+        auto di = rawData.request(), si = sampleTimes.request(), po = position.request(),
+             gi = focusGrid.request();
+        if (di.ndim != 2) throw std::runtime_error("rawData must be 2D");
+        if (si.ndim != 1 || si.shape[0] != di.shape[1])
+            throw std::runtime_error("sampleTimes shape is incorrect");
+        if (po.ndim != 2 || po.shape[0] != di.shape[0] || po.shape[1] != 3)
+            throw std::runtime_error("position shape is incorrect");
+        if (gi.ndim != 3 || gi.shape[2] != 3) throw std::runtime_error("focusGrid shape is incorrect");
+        tdc_job_desc d{};
+        d.raw_iq = static_cast<const float *>(di.ptr);
+        d.sample_times = static_cast<const double *>(si.ptr);
+        d.chirp_position = static_cast<const double *>(po.ptr);
+        d.grid = static_cast<const double *>(gi.ptr);
+        d.n_pri = static_cast<int32_t>(di.shape[0]);
+        d.n_samples = static_cast<int32_t>(di.shape[1]);
+        d.n_rows = static_cast<int32_t>(gi.shape[0]);
+        d.n_cols = static_cast<int32_t>(gi.shape[1]);
+        d.mod_rate = modRate;
+        d.start_freq = startFreq;
+        d.form = parse_form(form);
+        d.raw_storage = parse_storage(storage);
+        int64_t id = 0;
+        check_status(tdc_server_submit(s_, &d, &id));
+        keep_[id] = Keep{py::make_tuple(rawData, sampleTimes, position, focusGrid), d.n_rows, d.n_cols};
+        return id;
+    }
+
+    bool poll(int64_t id)
+    {
+        int done = 0;
+        check_status(tdc_server_poll(s_, id, &done));
+        return done != 0;
+    }
+
+    py::tuple wait(int64_t id)
+    {
+        auto it = keep_.find(id);
+        if (it == keep_.end()) throw std::invalid_argument("unknown job id");
+        c64_array img({it->second.rows, it->second.cols});
+        tdc_job_info info{};
+        int rc;
+        {
+            float *dst = reinterpret_cast<float *>(img.mutable_data());
+            py::gil_scoped_release nogil;
+            rc = tdc_server_wait(s_, id, dst, &info);
+        }
+        keep_.erase(it);
+        check_status(rc);
+        py::dict d;
+        d["id"] = info.id;
+        d["t_submit"] = info.t_submit_s;
+        d["t_admit"] = info.t_admit_s;
+        d["t_first_step"] = info.t_first_step_s;
+        d["t_done"] = info.t_done_s;
+        d["latency_ms"] = (info.t_done_s - info.t_submit_s) * 1e3;
+        d["terms"] = info.terms;
+        d["device_bytes"] = info.device_bytes;
+        d["tiles"] = info.tiles;
+        d["steps"] = info.steps;
+        return py::make_tuple(img, d);
+    }
+
+    py::dict stats()
+    {
+        tdc_server_stats st{};
+        check_status(tdc_server_get_stats(s_, &st));
+        py::dict d;
+        d["submitted"] = st.submitted;
+        d["completed"] = st.completed;
+        d["failed"] = st.failed;
+        d["steps"] = st.steps;
+        d["device_busy_ms"] = st.device_busy_ms;
+        d["terms_done"] = st.terms_done;
+        d["bytes_in_use"] = st.bytes_in_use;
+        d["bytes_peak"] = st.bytes_peak;
+        d["queued"] = st.queued;
+        d["active"] = st.active;
+        d["graphs"] = st.graphs;
+        return d;
+    }
+
+    double clock()
+    {
+        double t = 0.0;
+        check_status(tdc_server_clock(s_, &t));
+        return t;
+    }
+
+    void close()
+    {
+        if (!s_) return;
+        tdc_server *s = s_;
+        s_ = nullptr;
+        py::gil_scoped_release nogil;
+        tdc_server_destroy(s);
+    }
+
+private:
+    struct Keep {
+        py::object arrays;
+        int rows, cols;
+    };
+    tdc_server *s_ = nullptr;
+    std::map<int64_t, Keep> keep_;
+};
+
 } // namespace
 
 PYBIND11_MODULE(_sarlabtdc, m)
@@ -263,4 +392,16 @@ PYBIND11_MODULE(_sarlabtdc, m)
         .def("run", &PyTdcProcessor::run, py::arg("iterations") = 1)
         .def("fetchImage", &PyTdcProcessor::fetchImage)
         .def("stats", &PyTdcProcessor::stats);
+
+    py::class_<PyServer>(m, "Server")
+        .def(py::init<int, double, double>(), py::arg("gpuNum") = 0,
+             py::arg("hbmBudgetBytes") = 200e9, py::arg("stepTerms") = 0.0)
+        .def("submit", &PyServer::submit, py::arg("rawData"), py::arg("sampleTimes"),
+             py::arg("position"), py::arg("focusGrid"), py::arg("modRate"), py::arg("startFreq"),
+             py::arg("form") = "precision", py::arg("storage") = "c64")
+        .def("poll", &PyServer::poll, py::arg("jobId"))
+        .def("wait", &PyServer::wait, py::arg("jobId"))
+        .def("stats", &PyServer::stats)
+        .def("clock", &PyServer::clock)
+        .def("close", &PyServer::close);
 }

@@ -217,8 +217,10 @@ int ensure_device(tdc_processor *h)
 int choose_split(int ntiles, int npri, int force)
 {
     if (force > 0) return force;
-    const int target_wgs = 2048;
-    const int min_cps = 16;
+    // measured on config 3: ~16k workgroups of 4 waves (8 rounds of 2048
+    // resident ones) beat one resident round by 10-12%
+    const int target_wgs = 16384;
+    const int min_cps = 8;
     int s = 1;
     while (s < 64 && ntiles * s < target_wgs && (npri + 2 * s - 1) / (2 * s) >= min_cps) s *= 2;
     return s;
@@ -346,7 +348,9 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
                                         h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
                                         h->d_rec.as<tdc::PlainRec>(), s));
     }
-    h->ppl = h->force_ppl ? h->force_ppl : (h->doppler ? 1 : 2);
+    h->ppl = h->force_ppl ? h->force_ppl : 1;
+    if (h->ppl > 4)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "the plain form runs 1, 2 or 4 pixels per lane");
     const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
     const int nsplit = choose_split(ntiles, npri, h->force_split);
     make_geom(h->geom, npx, npri, ns, h->ppl, nsplit);
@@ -409,22 +413,10 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
         for (size_t i = 0; i < grid64.size(); ++i) grid64[i] = h->grid[i];
         gridp = grid64.data();
     }
-    // uniform sampling: the rotation recurrence advances by a constant step
-    // (a few float32 ulps of the largest time are allowed, so that sample times
-    // given in float32 qualify; the anchor phases use the exact anchor times)
-    double dt = ns > 1 ? (t[ns - 1] - t[0]) / (ns - 1) : 0.0;
-    const double tmax = std::max(std::fabs(t[0]), std::fabs(t[ns - 1]));
-    const double ttol = 1e-6 * std::fabs(dt) + 0x1p-21 * tmax + 1e-300;
-    for (int m = 0; m < ns; ++m) {
-        if (std::fabs(t[m] - (t[0] + m * dt)) > ttol)
-            return fail(TDC_ERR_UNSUPPORTED,
-                        "the precision form needs uniformly spaced sample times");
-    }
-    const int nblk = (ns + tdc::kAnchorBlock - 1) / tdc::kAnchorBlock;
-    std::vector<double> kanchor(nblk);
-    for (int b = 0; b < nblk; ++b)
-        kanchor[b] = 2.0 * (f0 + krate * t[b * tdc::kAnchorBlock]) / kSpeedOfLight;
-    h->dk = 2.0 * krate * dt / kSpeedOfLight;
+    std::vector<double> kanchor;
+    std::string why;
+    if (!tdc::precision_tables(t.data(), ns, f0, krate, kanchor, h->dk, why))
+        return fail(TDC_ERR_UNSUPPORTED, why);
     std::vector<tdc::ChirpPos> cp(npri);
     for (int n = 0; n < npri; ++n) cp[n] = tdc::ChirpPos{cpos[3 * n], cpos[3 * n + 1], cpos[3 * n + 2], 0.0};
     std::vector<float> wr;
@@ -442,7 +434,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     TDC_HIP(tdc::launch_stage_precision(h->storage, h->d_raw.as<float>(), h->d_wr.as<float>(), npri,
                                         ns, h->d_rec.p, s));
     h->doppler = false;
-    h->ppl = h->force_ppl ? h->force_ppl : 4;
+    h->ppl = h->force_ppl ? h->force_ppl : 2;
     const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
     const int nsplit = choose_split(ntiles, npri, h->force_split);
     make_geom(h->geom, npx, npri, ns, h->ppl, nsplit);
@@ -454,6 +446,42 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
 }
 
 } // namespace
+
+namespace tdc {
+
+int set_last_error(int code, const std::string &msg)
+{
+    t_err = msg;
+    return code;
+}
+
+void clear_last_error() { t_err.clear(); }
+
+// Anchor table and rotation step of the precision form. The recurrence
+// advances by a constant phase step, so the sample times must be uniformly
+// spaced (a few float32 ulps of the largest time are allowed, so that times
+// given in float32 qualify; each anchor uses its own exact sample time).
+bool precision_tables(const double *t, int ns, double f0, double krate,
+                      std::vector<double> &kanchor, double &dk, std::string &why)
+{
+    const double dt = ns > 1 ? (t[ns - 1] - t[0]) / (ns - 1) : 0.0;
+    const double tmax = std::max(std::fabs(t[0]), std::fabs(t[ns - 1]));
+    const double ttol = 1e-6 * std::fabs(dt) + 0x1p-21 * tmax + 1e-300;
+    for (int m = 0; m < ns; ++m) {
+        if (std::fabs(t[m] - (t[0] + m * dt)) > ttol) {
+            why = "the precision form needs uniformly spaced sample times";
+            return false;
+        }
+    }
+    const int nblk = (ns + kAnchorBlock - 1) / kAnchorBlock;
+    kanchor.resize(nblk);
+    for (int b = 0; b < nblk; ++b)
+        kanchor[b] = 2.0 * (f0 + krate * t[b * kAnchorBlock]) / kSpeedOfLight;
+    dk = 2.0 * krate * dt / kSpeedOfLight;
+    return true;
+}
+
+} // namespace tdc
 
 extern "C" {
 
@@ -634,8 +662,9 @@ int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int
 int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
-    if (pixels_per_lane != 0 && pixels_per_lane != 1 && pixels_per_lane != 2 && pixels_per_lane != 4)
-        return fail(TDC_ERR_INVALID_ARGUMENT, "pixels per lane must be 0, 1, 2 or 4");
+    if (pixels_per_lane != 0 && pixels_per_lane != 1 && pixels_per_lane != 2 &&
+        pixels_per_lane != 4 && pixels_per_lane != 8)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "pixels per lane must be 0, 1, 2, 4 or 8");
     if (chirp_splits < 0 || chirp_splits > 4096)
         return fail(TDC_ERR_INVALID_ARGUMENT, "chirp splits must be in [0, 4096]");
     h->force_ppl = pixels_per_lane;

@@ -3,6 +3,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
+#include <vector>
 
 #include <hip/hip_runtime.h>
 
@@ -85,5 +87,12 @@ hipError_t launch_precision(int ppl, int storage, const void *raw, const ChirpPo
                             double2 *out, hipStream_t s, int *workgroups);
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
+
+// Host helpers shared by the processor and the server (tdc_capi.cpp).
+// The thread-local message returned by tdc_last_error().
+int set_last_error(int code, const std::string &msg);
+void clear_last_error();
+bool precision_tables(const double *t, int ns, double f0, double krate,
+                      std::vector<double> &kanchor, double &dk, std::string &why);
 
 } // namespace tdc

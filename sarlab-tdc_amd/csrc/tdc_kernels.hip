@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tdc_internal.h"
+#include "tdc_tiles.h"
 
 namespace tdc {
 namespace {
@@ -56,19 +57,6 @@ __device__ __forceinline__ void map_block(const Geom &g, int &tile, int &split)
         split = b % g.nsplit;
         tile = b / g.nsplit;
     }
-}
-
-__device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
-__device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
-
-// Error-free accumulation: acc + comp carries the running sum.
-__device__ __forceinline__ void two_sum_acc(float &acc, float &comp, float x)
-{
-    const float s = acc + x;
-    const float bp = s - acc;
-    const float err = (acc - (s - bp)) + (x - bp);
-    acc = s;
-    comp += err;
 }
 
 // Quaternion product / rotation in FP32 (reference tdc-lib/include/gpumath.h:66-87).
@@ -163,46 +151,8 @@ __global__ __launch_bounds__(kBlock) void plain_kernel(const PlainRec *__restric
     if (tile >= g.ntiles) return;
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
-
-    float px[P], py[P], pz[P], ar[P], ai[P];
-    int pix[P];
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
-        const int q = min(pix[j], g.npx - 1);
-        px[j] = grid[3 * q];
-        py[j] = grid[3 * q + 1];
-        pz[j] = grid[3 * q + 2];
-        ar[j] = 0.0F;
-        ai[j] = 0.0F;
-    }
-
-    const PlainRec *__restrict__ r = rec + static_cast<size_t>(max(n0, 0)) * g.ns;
-    const int nterms = (n1 > n0) ? (n1 - n0) * g.ns : 0;
-#pragma unroll 4
-    for (int e = 0; e < nterms; ++e) {
-        const PlainRec t = r[e];
-#pragma unroll
-        for (int j = 0; j < P; ++j) {
-            const float dx = px[j] - t.x;
-            const float dy = py[j] - t.y;
-            const float dz = pz[j] - t.z;
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float th = __builtin_amdgcn_fractf(__builtin_amdgcn_sqrtf(r2) * t.k);
-            const float sn = sin_rev(th);
-            const float cs = cos_rev(th);
-            ar[j] = fmaf(t.rr, cs, ar[j]);
-            ar[j] = fmaf(-t.ri, sn, ar[j]);
-            ai[j] = fmaf(t.rr, sn, ai[j]);
-            ai[j] = fmaf(t.ri, cs, ai[j]);
-        }
-    }
-
-    float2 *o = out + static_cast<size_t>(split) * g.npx;
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        if (pix[j] < g.npx) o[pix[j]] = make_float2(ar[j], ai[j]);
-    }
+    plain_tile<P>(rec, grid, g.npx, g.ns, n0, n1, tile * P * kBlock,
+                  out + static_cast<size_t>(split) * g.npx);
 }
 
 // ---------------------------------------------------------------------------
@@ -360,35 +310,6 @@ __global__ __launch_bounds__(kBlock) void finalize_f32_kernel(const float2 *__re
 // Precision form.
 // ---------------------------------------------------------------------------
 
-// Raw sample loaders for the three storage formats; the address is uniform.
-struct RawC64 {
-    using T = float2;
-    static __device__ __forceinline__ void get(const T *p, int i, float &re, float &im)
-    {
-        const float2 v = p[i];
-        re = v.x;
-        im = v.y;
-    }
-};
-struct RawF16 {
-    using T = unsigned int;
-    static __device__ __forceinline__ void get(const T *p, int i, float &re, float &im)
-    {
-        const unsigned int v = p[i];
-        re = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v & 0xffffU)));
-        im = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v >> 16)));
-    }
-};
-struct RawBF16 {
-    using T = unsigned int;
-    static __device__ __forceinline__ void get(const T *p, int i, float &re, float &im)
-    {
-        const unsigned int v = p[i];
-        re = __uint_as_float(v << 16);
-        im = __uint_as_float(v & 0xffff0000U);
-    }
-};
-
 __global__ __launch_bounds__(kBlock) void stage_precision_kernel(const float2 *__restrict__ raw,
                                                                  const float *__restrict__ wr,
                                                                  int npri, int ns, int storage,
@@ -428,94 +349,8 @@ __global__ __launch_bounds__(kBlock) void precision_kernel(
     if (tile >= g.ntiles) return;
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
-    const int nblk = (g.ns + kAnchorBlock - 1) / kAnchorBlock;
-
-    double px[P], py[P], pz[P];
-    float accr[P], acci[P], cmpr[P], cmpi[P];
-    int pix[P];
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
-        const int q = min(pix[j], g.npx - 1);
-        px[j] = grid[3 * q];
-        py[j] = grid[3 * q + 1];
-        pz[j] = grid[3 * q + 2];
-        accr[j] = 0.0F;
-        acci[j] = 0.0F;
-        cmpr[j] = 0.0F;
-        cmpi[j] = 0.0F;
-    }
-
-    for (int n = n0; n < n1; ++n) {
-        const ChirpPos c = cpos[n];
-        double rng[P];
-        float wr[P], wi[P];
-#pragma unroll
-        for (int j = 0; j < P; ++j) {
-            const double dx = c.x - px[j], dy = c.y - py[j], dz = c.z - pz[j];
-            rng[j] = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
-            double rot = dk * rng[j];
-            rot -= floor(rot);
-            sincospif(2.0F * static_cast<float>(rot), &wi[j], &wr[j]);
-        }
-        const typename RAW::T *__restrict__ row = raw + static_cast<size_t>(n) * g.ns;
-        for (int b = 0; b < nblk; ++b) {
-            const int a = b * kAnchorBlock;
-            const typename RAW::T *__restrict__ blk = row + a;
-            float sr[P], si[P];
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                sr[j] = 0.0F;
-                si[j] = 0.0F;
-            }
-            const int len = min(kAnchorBlock, g.ns - a);
-            if (len == kAnchorBlock) {
-#pragma unroll
-                for (int i = kAnchorBlock - 1; i >= 0; --i) {
-                    float vr, vi;
-                    RAW::get(blk, i, vr, vi);
-#pragma unroll
-                    for (int j = 0; j < P; ++j) {
-                        const float nr = fmaf(sr[j], wr[j], fmaf(-si[j], wi[j], vr));
-                        const float ni = fmaf(sr[j], wi[j], fmaf(si[j], wr[j], vi));
-                        sr[j] = nr;
-                        si[j] = ni;
-                    }
-                }
-            } else {
-                for (int i = len - 1; i >= 0; --i) {
-                    float vr, vi;
-                    RAW::get(blk, i, vr, vi);
-#pragma unroll
-                    for (int j = 0; j < P; ++j) {
-                        const float nr = fmaf(sr[j], wr[j], fmaf(-si[j], wi[j], vr));
-                        const float ni = fmaf(sr[j], wi[j], fmaf(si[j], wr[j], vi));
-                        sr[j] = nr;
-                        si[j] = ni;
-                    }
-                }
-            }
-            const double ka = kanchor[b];
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-                double th = ka * rng[j];
-                th -= floor(th);
-                const float tf = static_cast<float>(th);
-                const float zr = cos_rev(tf), zi = sin_rev(tf);
-                two_sum_acc(accr[j], cmpr[j], fmaf(zr, sr[j], -(zi * si[j])));
-                two_sum_acc(acci[j], cmpi[j], fmaf(zr, si[j], zi * sr[j]));
-            }
-        }
-    }
-
-    double2 *o = out + static_cast<size_t>(split) * g.npx;
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-        if (pix[j] < g.npx) {
-            o[pix[j]] = make_double2(static_cast<double>(accr[j]) + static_cast<double>(cmpr[j]),
-                                     static_cast<double>(acci[j]) + static_cast<double>(cmpi[j]));
-        }
-    }
+    precision_tile<P, RAW>(raw, cpos, grid, kanchor, dk, g.npx, g.ns, n0, n1, tile * P * kBlock,
+                           out + static_cast<size_t>(split) * g.npx);
 }
 
 __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const double2 *__restrict__ partial,
@@ -646,6 +481,10 @@ static hipError_t launch_precision_t(int ppl, const void *raw, const ChirpPos *c
         break;
     case 4:
         hipLaunchKernelGGL((precision_kernel<4, RAW>), dim3(nb), dim3(kBlock), 0, s, r, cpos, grid,
+                           kanchor, dk, g, out);
+        break;
+    case 8:
+        hipLaunchKernelGGL((precision_kernel<8, RAW>), dim3(nb), dim3(kBlock), 0, s, r, cpos, grid,
                            kanchor, dk, g, out);
         break;
     default:

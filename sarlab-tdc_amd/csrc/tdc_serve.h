@@ -1,0 +1,57 @@
+// tdc_serve.h -- device-side tables of the serving form (shared host/device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tdc_internal.h"
+
+namespace tdc {
+
+// Pixels per lane of the serving tiles (the one-job kernels' measured best):
+// a precision tile covers 512 pixels, a plain tile 256.
+constexpr int kServePixelsPrecision = 2;
+constexpr int kServePixelsPlain = 1;
+// Capacity of one step.
+constexpr int kMaxStepTiles = 32768;
+constexpr int kMaxStepFin = 32768;
+constexpr int kMaxJobSlots = 8192;
+
+// One admitted job as the kernels see it.
+struct alignas(16) ServeJob {
+    const void *raw;          // precision: staged raw samples; plain: PlainRec records
+    const ChirpPos *cpos;     // precision: FP64 phase centre per chirp
+    const double *grid64;     // precision: FP64 pixel coordinates
+    const float *grid32;      // plain: FP32 pixel coordinates
+    const double *kanchor;    // precision: 2 (f0 + K t_a) / c per 64-sample block
+    void *partial;            // [nsplit][npx]: double2 (precision) or float2 (plain)
+    float2 *image;            // [npx]
+    double dk;                // precision: 2 K dt / c
+    int npx, npri, ns, nsplit, cps, form, storage, pad;
+};
+
+struct ServeTile {
+    int job;   // job slot
+    int pix0;  // first pixel of the tile
+    int split; // chirp range index
+    int pad;
+};
+
+struct FinTile {
+    int job;
+    int pix0; // first of up to kBlock pixels
+};
+
+struct StepTable {
+    int ntiles;
+    int nfin;
+    int pad[2];
+    ServeTile tiles[kMaxStepTiles];
+    FinTile fin[kMaxStepFin];
+};
+
+hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
+                             int workgroups, int fin_workgroups, hipStream_t s);
+hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,
+                                    int npri, int ns, PlainRec *out, hipStream_t s);
+
+} // namespace tdc

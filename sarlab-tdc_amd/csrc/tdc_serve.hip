@@ -1,0 +1,149 @@
+// tdc_serve.hip -- kernels of the serving form.
+//
+// A step of the server runs three kernels, captured once per grid-size bucket
+// in a hipGraph and replayed:
+//   1. serve_reset_kernel   zeroes the step's tile counter (a kernel, not a
+//                           memset node, so the graph replays cleanly);
+//   2. serve_kernel         persistent: every workgroup takes the next tile
+//                           from the step table with one atomic, runs the
+//                           tile body of its job (plain or precision, any raw
+//                           storage) and writes the job's partial image for
+//                           that (pixel tile, chirp range); all workgroups
+//                           leave once the counter passes the tile count;
+//   3. serve_finalize_kernel sums each finished job's partial images in
+//                           chirp-range order into its image.
+// A job's tiles, their chirp ranges and the finalize order depend only on
+// the job's own shape, so its image is bit-identical whatever else shares the
+// step, and whichever workgroup runs which tile.
+#include <hip/hip_runtime.h>
+
+#include "tdc_serve.h"
+#include "tdc_tiles.h"
+
+namespace tdc {
+namespace {
+
+__global__ void serve_reset_kernel(unsigned int *counter) { *counter = 0U; }
+
+template <int PP, int PL>
+__global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restrict__ jobs,
+                                                       const StepTable *__restrict__ table,
+                                                       unsigned int *__restrict__ counter)
+{
+    __shared__ int s_tile;
+    const cptr<StepTable> tab = as_const(table);
+    const int ntiles = tab->ntiles;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(counter, 1U));
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (t >= ntiles) break; // every wave of the workgroup takes this exit
+        const ServeTile tl = tab->tiles[t];
+        const ServeJob j = as_const(jobs)[tl.job];
+        const int n0 = tl.split * j.cps;
+        const int n1 = min(j.npri, n0 + j.cps);
+        if (j.form == 0) {
+            plain_tile<PL>(static_cast<const PlainRec *>(j.raw), j.grid32, j.npx, j.ns, n0, n1,
+                           tl.pix0,
+                           static_cast<float2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx);
+        } else {
+            double2 *out = static_cast<double2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx;
+            if (j.storage == 0) {
+                precision_tile<PP, RawC64>(static_cast<const float2 *>(j.raw), j.cpos, j.grid64,
+                                           j.kanchor, j.dk, j.npx, j.ns, n0, n1, tl.pix0, out);
+            } else if (j.storage == 1) {
+                precision_tile<PP, RawF16>(static_cast<const unsigned int *>(j.raw), j.cpos,
+                                           j.grid64, j.kanchor, j.dk, j.npx, j.ns, n0, n1, tl.pix0,
+                                           out);
+            } else {
+                precision_tile<PP, RawBF16>(static_cast<const unsigned int *>(j.raw), j.cpos,
+                                            j.grid64, j.kanchor, j.dk, j.npx, j.ns, n0, n1,
+                                            tl.pix0, out);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const ServeJob *__restrict__ jobs,
+                                                                const StepTable *__restrict__ table)
+{
+    const cptr<StepTable> tab = as_const(table);
+    const int nfin = tab->nfin;
+    for (int f = static_cast<int>(blockIdx.x); f < nfin; f += static_cast<int>(gridDim.x)) {
+        const FinTile ft = tab->fin[f];
+        const ServeJob j = as_const(jobs)[ft.job];
+        const int p = ft.pix0 + static_cast<int>(threadIdx.x);
+        if (p >= j.npx) continue;
+        if (j.form == 0) {
+            const float2 *part = static_cast<const float2 *>(j.partial);
+            float sr = 0.0F, si = 0.0F;
+            for (int s = 0; s < j.nsplit; ++s) {
+                const float2 v = part[static_cast<size_t>(s) * j.npx + p];
+                sr += v.x;
+                si += v.y;
+            }
+            j.image[p] = make_float2(sr, si);
+        } else {
+            const double2 *part = static_cast<const double2 *>(j.partial);
+            double sr = 0.0, si = 0.0;
+            for (int s = 0; s < j.nsplit; ++s) {
+                const double2 v = part[static_cast<size_t>(s) * j.npx + p];
+                sr += v.x;
+                si += v.y;
+            }
+            j.image[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+        }
+    }
+}
+
+// Plain-form records from per-chirp positions (stop-and-hop), for served jobs.
+__global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(
+    const float2 *__restrict__ raw, const ChirpPos *__restrict__ cpos, const float *__restrict__ kt,
+    int npri, int ns, PlainRec *__restrict__ out)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    for (size_t e = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * kBlock) {
+        const int n = static_cast<int>(e / ns);
+        const int m = static_cast<int>(e % ns);
+        const float2 v = raw[e];
+        const ChirpPos c = cpos[n];
+        PlainRec r;
+        r.rr = v.x;
+        r.ri = v.y;
+        r.x = static_cast<float>(c.x);
+        r.y = static_cast<float>(c.y);
+        r.z = static_cast<float>(c.z);
+        r.k = kt[m];
+        r.pad0 = 0.0F;
+        r.pad1 = 0.0F;
+        out[e] = r;
+    }
+}
+
+} // namespace
+
+hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
+                             int workgroups, int fin_workgroups, hipStream_t s)
+{
+    hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL((serve_kernel<kServePixelsPrecision, kServePixelsPlain>), dim3(workgroups),
+                       dim3(kBlock), 0, s, jobs, table, counter);
+    hipLaunchKernelGGL(serve_finalize_kernel, dim3(fin_workgroups), dim3(kBlock), 0, s, jobs,
+                       table);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,
+                                    int npri, int ns, PlainRec *out, hipStream_t s)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    const size_t want = (total + kBlock - 1) / kBlock;
+    const int grid = static_cast<int>(want < 4096 ? (want > 0 ? want : 1) : 4096);
+    hipLaunchKernelGGL(stage_plain_chirp_kernel, dim3(grid), dim3(kBlock), 0, s,
+                       reinterpret_cast<const float2 *>(raw), cpos, kt, npri, ns, out);
+    return hipGetLastError();
+}
+
+} // namespace tdc

@@ -7,6 +7,6 @@ the reference's constructor, ``start``, ``setRawData``, ``setFocusGrid`` and
 without the built HIP library raises ImportError.
 """
 from ._sarlabtdc import *  # noqa: F401,F403
-from ._sarlabtdc import FORM_PLAIN, FORM_PRECISION, TdcProcessor, device_count
+from ._sarlabtdc import FORM_PLAIN, FORM_PRECISION, Server, TdcProcessor, device_count
 
-__all__ = ["TdcProcessor", "device_count", "FORM_PLAIN", "FORM_PRECISION"]
+__all__ = ["TdcProcessor", "Server", "device_count", "FORM_PLAIN", "FORM_PRECISION"]

@@ -1,0 +1,142 @@
+"""Serving form: continuously batched jobs, parity, determinism in a mix, admission."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import subset
+
+pytestmark = pytest.mark.gpu
+
+
+def job_scene(j, nc, ns, side, spacing=3e-3):
+    from sarlabtdc import scene
+
+    return scene.config5_job(j, (nc, ns, side, spacing))
+
+
+def submit(srv, s, form="precision", storage="c64"):
+    f = s.f64()
+    return srv.submit(f["rawData"], f["sampleTimes"], f["position"], f["focusGrid"],
+                      f["modRate"], f["startFreq"], form=form, storage=storage)
+
+
+def oracle_img(orc, s, px=None):
+    return orc.image_f64(s.raw, s.sample_times, s.track, s.grid, s.mod_rate, s.f0, pixels=px)
+
+
+@pytest.mark.parametrize("form,storage,tol", [("precision", "c64", 1e-4), ("precision", "bf16", 2e-2),
+                                              ("precision", "f16", 3e-3), ("plain", "c64", 2e-3)])
+def test_single_job_parity(tdc, orc, form, storage, tol):
+    s = job_scene(1, 200, 256, 70)
+    srv = tdc.Server(0, 8e9)
+    jid = submit(srv, s, form, storage)
+    img, info = srv.wait(jid)
+    srv.close()
+    assert img.shape == (70, 70)
+    px = subset(img.size, 600, extra=np.argsort(np.abs(img.reshape(-1)))[-8:])
+    e = orc.rel_l2(img.reshape(-1)[px], oracle_img(orc, s, px))
+    print(form, storage, "rel-L2", e, info)
+    assert e < tol
+    assert info["latency_ms"] > 0 and info["tiles"] >= 1
+
+
+def test_job_is_bit_identical_alone_and_in_a_mix(tdc):
+    target = job_scene(7, 300, 256, 90)
+    others = [job_scene(100 + k, nc, ns, side) for k, (nc, ns, side) in
+              enumerate([(700, 512, 200), (64, 128, 32), (1500, 128, 64), (333, 256, 150),
+                         (90, 512, 300)])]
+    srv = tdc.Server(0, 16e9)
+    alone, _ = srv.wait(submit(srv, target))
+    srv.close()
+    srv = tdc.Server(0, 16e9)
+    ids = [submit(srv, o) for o in others[:3]]
+    tid = submit(srv, target)
+    ids += [submit(srv, o) for o in others[3:]]
+    mixed, info = srv.wait(tid)
+    for i in ids:
+        srv.wait(i)
+    st = srv.stats()
+    srv.close()
+    assert np.array_equal(alone, mixed)
+    assert st["completed"] == 6 and st["failed"] == 0
+
+
+def test_plain_job_bit_identical_in_a_mix(tdc):
+    target = job_scene(8, 260, 128, 60)
+    srv = tdc.Server(0, 8e9)
+    alone, _ = srv.wait(submit(srv, target, form="plain"))
+    srv.close()
+    srv = tdc.Server(0, 8e9)
+    a = submit(srv, job_scene(9, 800, 256, 128))
+    b = submit(srv, target, form="plain")
+    c = submit(srv, job_scene(10, 100, 512, 256), form="plain")
+    mixed, _ = srv.wait(b)
+    srv.wait(a)
+    srv.wait(c)
+    srv.close()
+    assert np.array_equal(alone, mixed)
+
+
+def test_admission_respects_the_budget(tdc, orc):
+    jobs = [job_scene(20 + k, 256, 256, 128) for k in range(4)]
+    srv = tdc.Server(0, 1.0)  # probe: how big is one job
+    with pytest.raises(RuntimeError, match="more than the whole budget"):
+        submit(srv, jobs[0])
+    srv.close()
+    one = None
+    srv = tdc.Server(0, 64e9)
+    img0, info = srv.wait(submit(srv, jobs[0]))
+    one = info["device_bytes"]
+    srv.close()
+    # room for one job at a time: the others queue until memory frees up
+    srv = tdc.Server(0, one * 1.5)
+    ids = [submit(srv, s) for s in jobs]
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st["bytes_peak"] <= one * 1.5
+    assert np.array_equal(res[0][0], img0)
+    admits = [r[1]["t_admit"] for r in res]
+    dones = [r[1]["t_done"] for r in res]
+    for k in range(1, 4):
+        assert admits[k] >= dones[k - 1] - 1e-3  # admitted only after the previous one left
+    px = subset(128 * 128, 200)
+    e = orc.rel_l2(res[3][0].reshape(-1)[px], oracle_img(orc, jobs[3], px))
+    assert e < 1e-4
+
+
+def test_small_jobs_overtake_a_large_one(tdc):
+    """Jobs submitted while a large one runs join it on the device and finish first."""
+    big = job_scene(30, 2048, 512, 512)
+    small = [job_scene(31 + k, 64, 128, 32) for k in range(5)]
+    srv = tdc.Server(0, 64e9)
+    bid = submit(srv, big)
+    time.sleep(0.005)
+    sids = [submit(srv, s) for s in small]
+    res_small = [srv.wait(i) for i in sids]
+    _, binfo = srv.wait(bid)
+    srv.close()
+    for img, info in res_small:
+        assert info["t_submit"] < binfo["t_done"]
+        assert info["t_done"] < binfo["t_done"]
+        assert info["t_first_step"] < binfo["t_done"]
+    assert binfo["steps"] > 1
+
+
+def test_invalid_jobs(tdc):
+    s = job_scene(40, 64, 128, 32)
+    f = s.f64()
+    srv = tdc.Server(0, 8e9)
+    with pytest.raises(ValueError, match="plain form reads complex64"):
+        srv.submit(f["rawData"], f["sampleTimes"], f["position"], f["focusGrid"], f["modRate"],
+                   f["startFreq"], form="plain", storage="bf16")
+    bad_t = f["sampleTimes"] ** 1.01
+    jid = srv.submit(f["rawData"], bad_t, f["position"], f["focusGrid"], f["modRate"],
+                     f["startFreq"])
+    with pytest.raises(RuntimeError, match="uniformly spaced"):
+        srv.wait(jid)
+    ok = submit(srv, s)
+    img, _ = srv.wait(ok)
+    srv.close()
+    assert np.isfinite(img).all()
