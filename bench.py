@@ -191,7 +191,22 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="offered jobs/s")
     ap.add_argument("--serve-plain-jobs", type=int, default=200,
                     help="jobs of the stream also served in the plain form")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (the traffic calibration "
+                         "kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        # torch's copy kernel in a process that never loads the TDC library
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path shards by pixel tiles with no exchange; see the design notes)",
