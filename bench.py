@@ -2,11 +2,13 @@
 """Benchmark of TDC SAR image formation on one MI355X.
 
 Metric: G pixel*chirp*samples/s (terms/s / 1e9) at rel-L2 <= 1e-4 vs FP64,
-plain vs fast (precision) form. Workload (default): configuration 3 --
-1024 chirps x 512 samples, 512 x 512 pixels at 2 mm, 2 m slant range, 2000
-scatterers, 20 dB SNR, Hann x Hann windows pre-applied -- with complex64,
-FP16 and BF16 raw storage for the precision form, and the plain form (the
-drop-in) on the same data.
+plain vs fast (precision) form, and p50/p99 job latency of a serving stream.
+Workload (default, "arc"): 1024 chirps x 512 samples along a 1 m arc at 2 m
+slant range, 512 x 512 pixels at 2 mm, 2000 scatterers, 20 dB SNR, Hann x
+Hann windows pre-applied -- with complex64, FP16 and BF16 raw storage for the
+precision form, and the plain form (the drop-in) on the same data. Serving
+stream: 1000 seeded Poisson-arrival jobs (64-2048 chirps, 128/256/512
+samples, 32^2-512^2 pixels) against a 200 GB HBM budget.
 
 A step is one formation of the full image from inputs already resident in
 HBM (correlate-sum kernel plus the fixed-order split reduction). W warmup
@@ -137,11 +139,11 @@ def serve_stream(tdc, scenes, rate, form, storage="c64", seed=0, budget=200e9):
 def serve_leg(tdc, args):
     from sarlabtdc import scene
 
-    shapes = scene.config5_job_shapes(args.serve_jobs)
+    shapes = scene.stream_shapes(args.serve_jobs)
     t0 = time.perf_counter()
     scenes = []
     for j, sh in enumerate(shapes):
-        scenes.append(scene.config5_job(j, sh))
+        scenes.append(scene.stream_job(j, sh))
         if (j + 1) % 200 == 0:
             log(f"serving: generated {j + 1}/{len(shapes)} jobs ({time.perf_counter() - t0:.1f} s)")
     out = {}
@@ -181,13 +183,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=3, choices=[2, 3, 4])
+    ap.add_argument("--workload", default="arc", choices=["line_jitter", "arc", "helix"])
     ap.add_argument("--variants", default="c64,f16,bf16,plain",
                     help="comma list of c64, f16, bf16 (precision form) and plain")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--serve-jobs", type=int, default=1000, help="config 5 jobs (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=1000, help="serving-stream jobs (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="offered jobs/s")
     ap.add_argument("--serve-plain-jobs", type=int, default=200,
                     help="jobs of the stream also served in the plain form")
@@ -217,8 +219,8 @@ def main():
     from sarlabtdc import scene
 
     t0 = time.perf_counter()
-    s = scene.make(args.config)
-    log(f"scene config {args.config}: {s.n_chirps} x {s.n_samples}, {s.grid.shape[0]}x"
+    s = scene.make(args.workload)
+    log(f"scene {args.workload}: {s.n_chirps} x {s.n_samples}, {s.grid.shape[0]}x"
         f"{s.grid.shape[1]} px, {len(s.amplitudes)} scatterers "
         f"({time.perf_counter() - t0:.1f} s)")
 
@@ -289,7 +291,7 @@ def main():
         oracle.refmethod_f32(d, 0.0, 1e30, False, False, False, pixels=sample, nthreads=cores)
         dt = time.perf_counter() - t0
         cpu = dict(value=n_px * per_px / dt / 1e9, unit=UNIT, cores=cores, kind="port",
-                   sample=f"config {args.config}: {n_px} random pixels x {s.n_chirps} chirps x "
+                   sample=f"{args.workload}: {n_px} random pixels x {s.n_chirps} chirps x "
                           f"{s.n_samples} samples, FP32 reference method, {dt:.1f} s")
         log(f"cpu baseline: {cpu['value']:.4f} G terms/s on {cores} cores ({dt:.1f} s)")
 
@@ -300,7 +302,7 @@ def main():
     flop = FLOP_PER_TERM[kind]
     achieved = head["terms"] * flop / (head["main_kernel_ms"] * 1e-3) / 1e12
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{kind}_config{args.config}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kind}_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             tj = json.load(fh)
@@ -323,7 +325,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"config {args.config}: {s.n_chirps} chirps x {s.n_samples} samples, "
+            "workload": (f"{args.workload}: {s.n_chirps} chirps x {s.n_samples} samples, "
                          f"{s.grid.shape[0]}x{s.grid.shape[1]} pixels, {kind} form, "
                          f"{head['storage']} raw"),
             "chirps": s.n_chirps, "samples": s.n_samples,

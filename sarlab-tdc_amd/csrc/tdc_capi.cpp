@@ -217,7 +217,7 @@ int ensure_device(tdc_processor *h)
 int choose_split(int ntiles, int npri, int force)
 {
     if (force > 0) return force;
-    // measured on config 3: ~16k workgroups of 4 waves (8 rounds of 2048
+    // measured on 1024 chirps x 512 samples x 512^2 pixels: ~16k workgroups of 4 waves (8 rounds of 2048
     // resident ones) beat one resident round by 10-12%
     const int target_wgs = 16384;
     const int min_cps = 8;

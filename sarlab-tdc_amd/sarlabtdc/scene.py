@@ -1,13 +1,19 @@
 """Seeded synthetic FMCW SAR scenes for the TDC processor.
 
 Generates point-scatterer scenes, their linear-FMCW beat signals (FP64,
-stored complex64) and the focus grids for the five workloads:
+stored complex64) and the focus grids for these workloads:
 
-  1  plain, small:   64 x 128,   32^2 grid at 5 mm, 0.5 m range, 8 scatterers
-  2  plain on GPU:   256 x 256,  128^2 at 3 mm, 1 m range, 200 scatterers
-  3  precision:      1024 x 512, 512^2 at 2 mm, 2 m slant range, 2000 scatterers
-  4  precision, big: 2048 x 512, 768^2 at 2 mm on a tilted plane, 5000 scatterers
-  5  serving stream: jobs with 64-2048 chirps, 128/256/512 samples, 32^2-512^2 grids
+  line_small   64 x 128 samples, straight 0.3 m track, 32^2 grid at 5 mm,
+               0.5 m range, 8 scatterers, 30 dB, Hann in range
+  line_jitter  256 x 256, 0.5 m track with 0.2 mm jitter, 128^2 at 3 mm,
+               1 m range, 200 scatterers, 25 dB, Hann x Hann
+  arc          1024 x 512, 1 m arc at 2 m slant range with 0.5 mm jitter,
+               512^2 at 2 mm, 2000 log-normal scatterers, 20 dB, Hann x Hann
+  helix        2048 x 512, helical segment (radius 0.5 m), 768^2 at 2 mm on a
+               plane tilted 30 degrees, 5000 scatterers (10% bright), 20 dB,
+               Taylor (nbar 4, -30 dB) in range x Hann in azimuth
+  stream_job   jobs of a serving stream: 64-2048 chirps, 128/256/512
+               samples, 32^2-512^2 grids at 2-5 mm, drawn as line_jitter
 
 Common radar: f0 = 77 GHz, B = 4 GHz, chirp 40 us (K = 1e14 Hz/s), samples
 at t_m = m * Tc / Ns, c = 299792458 m/s. Positions are stop-and-hop (one
@@ -230,7 +236,7 @@ def _speed(track):
     return length / (track.shape[0] * PRI)
 
 
-def config1(seed=0):
+def line_small(seed=0):
     rng = np.random.default_rng([1, seed])
     nc, ns = 64, 128
     track = np.zeros((nc, 3))
@@ -238,11 +244,11 @@ def config1(seed=0):
     grid = plane_grid(32, 32, 5e-3, (0.0, 0.5, 0.0))
     scat = _uniform_on_grid(rng, grid, 8)
     amp = _cn(rng, 8, 1.0)
-    return synthesize("config1", track, ns, grid, scat, amp, 30.0, rng, range_win=hann(ns),
+    return synthesize("line_small", track, ns, grid, scat, amp, 30.0, rng, range_win=hann(ns),
                       speed=_speed(track))
 
 
-def config2(seed=0, nc=256, ns=256, side=128, spacing=3e-3, n_scat=200):
+def line_jitter(seed=0, nc=256, ns=256, side=128, spacing=3e-3, n_scat=200):
     rng = np.random.default_rng([2, seed])
     track = np.zeros((nc, 3))
     track[:, 0] = np.linspace(-0.25, 0.25, nc)
@@ -251,11 +257,11 @@ def config2(seed=0, nc=256, ns=256, side=128, spacing=3e-3, n_scat=200):
     grid = plane_grid(side, side, spacing, (0.0, 1.0, 0.0))
     scat = _uniform_on_grid(rng, grid, n_scat)
     amp = _cn(rng, n_scat, 1.0)
-    return synthesize("config2", track, ns, grid, scat, amp, 25.0, rng, range_win=hann(ns),
+    return synthesize("line_jitter", track, ns, grid, scat, amp, 25.0, rng, range_win=hann(ns),
                       az_win=hann(nc), speed=speed)
 
 
-def config3(seed=0, nc=1024, ns=512, side=512):
+def arc(seed=0, nc=1024, ns=512, side=512):
     rng = np.random.default_rng([3, seed])
     rho, height, length = 1.936, 0.5, 1.0
     phi = -np.pi / 2 + np.linspace(-length / (2 * rho), length / (2 * rho), nc)
@@ -266,11 +272,11 @@ def config3(seed=0, nc=1024, ns=512, side=512):
     n = 2000
     scat = _uniform_on_grid(rng, grid, n)
     amp = np.exp(rng.normal(size=n)) * np.exp(1j * rng.uniform(0, 2 * np.pi, size=n))
-    return synthesize("config3", track, ns, grid, scat, amp, 20.0, rng, range_win=hann(ns),
+    return synthesize("arc", track, ns, grid, scat, amp, 20.0, rng, range_win=hann(ns),
                       az_win=hann(nc), speed=speed)
 
 
-def config4(seed=0, nc=2048, ns=512, side=768):
+def helix(seed=0, nc=2048, ns=512, side=768):
     rng = np.random.default_rng([4, seed])
     phi = np.deg2rad(np.linspace(30.0, 150.0, nc))
     radius, pitch = 0.5, 0.2
@@ -285,12 +291,12 @@ def config4(seed=0, nc=2048, ns=512, side=768):
     n_bright = n // 10
     scat = _uniform_on_grid(rng, grid, n)
     amp = np.concatenate([_cn(rng, n - n_bright, 0.1), _cn(rng, n_bright, 10.0)])
-    return synthesize("config4", track, ns, grid, scat, amp, 20.0, rng,
+    return synthesize("helix", track, ns, grid, scat, amp, 20.0, rng,
                       range_win=taylor(ns, 4, 30.0), az_win=hann(nc), speed=speed)
 
 
-def config5_job_shapes(n_jobs=1000, seed=0):
-    """(chirps, samples, side, spacing) per job of the serving stream."""
+def stream_shapes(n_jobs=1000, seed=0):
+    """(chirps, samples, side, spacing) of each job of the serving stream."""
     rng = np.random.default_rng([5, seed, 0])
     nc = np.rint(np.exp(rng.uniform(np.log(64), np.log(2048), n_jobs))).astype(int)
     ns = rng.choice([128, 256, 512], size=n_jobs)
@@ -299,8 +305,9 @@ def config5_job_shapes(n_jobs=1000, seed=0):
     return [(int(a), int(b), int(c), float(d)) for a, b, c, d in zip(nc, ns, side, spacing)]
 
 
-def config5_job(j, shape):
-    """Scene of serving job j (drawn as config 2 with the job's own shape and seed)."""
+def stream_job(j, shape):
+    """Scene of stream job j: a jittered 0.5 m line track as line_jitter, with the job's own
+    shape and seed."""
     nc, ns, side, spacing = shape
     rng = np.random.default_rng([5, j])
     track = np.zeros((nc, 3))
@@ -314,8 +321,8 @@ def config5_job(j, shape):
                       az_win=hann(nc), speed=speed)
 
 
-CONFIGS = {1: config1, 2: config2, 3: config3, 4: config4}
+WORKLOADS = {"line_small": line_small, "line_jitter": line_jitter, "arc": arc, "helix": helix}
 
 
-def make(cfg, seed=0, **kw):
-    return CONFIGS[cfg](seed=seed, **kw)
+def make(name, seed=0, **kw):
+    return WORKLOADS[name](seed=seed, **kw)

@@ -3,8 +3,8 @@
 mathtest_vectors.json
     The reference's own known answers for the math helpers, restated as data
     from tdctest/mathtest.cpp:12-137 (inputs and expected outputs only).
-config1_plain.npz
-    Configuration 1 (64 chirps x 128 samples, 32 x 32 pixels) in the
+line_small_plain.npz
+    The line_small scene (64 chirps x 128 samples, 32 x 32 pixels) in the
     reference's float32 input layout, and its FP64 oracle image.
 doppler_small.npz
     A 300-chirp (three chunks, the last one partial) x 64-sample scene with a
@@ -78,14 +78,14 @@ def main():
     with open(os.path.join(HERE, "mathtest_vectors.json"), "w") as fh:
         json.dump(mathtest_vectors(), fh, indent=1)
 
-    # configuration 1, reference float32 layout (per-chirp track stored; the
+    # line_small scene, reference float32 layout (per-chirp track stored; the
     # per-sample arrays are broadcasts of it)
-    s = scene.config1()
+    s = scene.line_small()
     d = s.dropin()
     img = oracle.image_f64(d["rawData"], d["sampleTimes"], d["position"], d["focusGrid"],
                            float(d["modRate"]), float(d["startFreq"]))
     np.savez_compressed(
-        os.path.join(HERE, "config1_plain.npz"),
+        os.path.join(HERE, "line_small_plain.npz"),
         rawData=d["rawData"], priTimes=d["priTimes"], sampleTimes=d["sampleTimes"],
         track=d["position"][:, 0, :], velocity=d["velocity"][:, 0, :],
         modRate=d["modRate"], startFreq=d["startFreq"], bodyBoresight=d["bodyBoresight"],

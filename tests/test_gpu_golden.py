@@ -19,8 +19,8 @@ def _proc(tdc, d):
     return p
 
 
-def test_golden_config1_plain(tdc, orc):
-    d, z = load_dropin("config1_plain.npz")
+def test_golden_line_small_plain(tdc, orc):
+    d, z = load_dropin("line_small_plain.npz")
     p = _proc(tdc, d)
     p.start(0.0, 1e30, False, False, False)
     img = p.getFocusedImage()

@@ -4,7 +4,7 @@ Tolerances (relative L2 over the compared pixels, and the phase error at the
 brightest oracle pixel):
   plain form      2e-3 / 2e-3 rad -- FP32 range and phase: the phase k*R of
                   ~1e3 cycles carries ~1e-4 cycles of rounding per term.
-  precision form  1e-4 / 1e-4 rad -- the north-star bound; FP64 range/anchor.
+  precision form  1e-4 / 1e-4 rad -- the required bound; FP64 range/anchor.
   FP16 raw        3e-3            -- storage rounding 2^-11 per component.
   BF16 raw        2e-2            -- storage rounding 2^-8 per component.
   Doppler window  5e-3            -- terms within rounding of the band edge may
@@ -76,10 +76,10 @@ def ragged_scene(npri=130, ns=100, rows=13, cols=21, seed=3):
 # ---------------------------------------------------------------------------- plain form
 
 
-def test_plain_config1(tdc, orc):
+def test_plain_line_small(tdc, orc):
     from sarlabtdc import scene
 
-    s = scene.config1()
+    s = scene.line_small()
     d = s.dropin()
     img, st = form(tdc, d)
     assert st["form"] == "plain" and not st["doppler_active"]
@@ -87,10 +87,10 @@ def test_plain_config1(tdc, orc):
     check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
 
 
-def test_plain_config1_hamming_range_window(tdc, orc):
+def test_plain_line_small_hamming_range_window(tdc, orc):
     from sarlabtdc import scene
 
-    d = scene.config1().dropin()
+    d = scene.line_small().dropin()
     img, _ = form(tdc, d, rw=True)
     ref = oracle_dropin(orc, d, rw=orc.range_window(d["rawData"].shape[1]))
     check(orc, img, ref, PLAIN_TOL)
@@ -100,7 +100,7 @@ def test_plain_config1_hamming_range_window(tdc, orc):
 def test_plain_other_range_windows(tdc, orc, window):
     from sarlabtdc import scene
 
-    d = scene.config1().dropin()
+    d = scene.line_small().dropin()
     ns = d["rawData"].shape[1]
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
@@ -129,10 +129,10 @@ def test_plain_launch_shapes_ragged(tdc, orc, ppl, splits):
     check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
 
 
-def test_plain_config2_subset(tdc, orc):
+def test_plain_line_jitter_subset(tdc, orc):
     from sarlabtdc import scene
 
-    s = scene.config2()
+    s = scene.line_jitter()
     d = s.dropin()
     img, st = form(tdc, d)
     flat = img.reshape(-1)
@@ -145,7 +145,7 @@ def test_plain_point_target_closed_form(tdc):
     """One noiseless scatterer on a pixel, windows off: I(p0) = a * Nc * Ns."""
     from sarlabtdc import scene
 
-    s = scene.config1()
+    s = scene.line_small()
     p0 = s.grid[11, 17]
     amp = 0.8 - 0.6j
     raw = scene.simulate_beat(s.track, s.sample_times, p0[None], [amp]).astype(np.complex64)
@@ -161,7 +161,7 @@ def test_plain_point_target_closed_form(tdc):
 def test_plain_linearity(tdc):
     from sarlabtdc import scene
 
-    s = scene.config1()
+    s = scene.line_small()
     d = s.dropin()
     rng = np.random.default_rng(11)
     r2 = (rng.normal(size=d["rawData"].shape) + 1j * rng.normal(size=d["rawData"].shape)).astype(
@@ -179,7 +179,7 @@ def test_plain_linearity(tdc):
 def test_plain_deterministic_reruns(tdc):
     from sarlabtdc import scene
 
-    d = scene.config1().dropin()
+    d = scene.line_small().dropin()
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
     p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"], d["velocity"],
@@ -199,10 +199,10 @@ def test_plain_deterministic_reruns(tdc):
 # ---------------------------------------------------------------------------- precision form
 
 
-def test_precision_config2_c64(tdc, orc):
+def test_precision_line_jitter_c64(tdc, orc):
     from sarlabtdc import scene
 
-    s = scene.config2()
+    s = scene.line_jitter()
     img, st = form(tdc, f64=s.f64(), kind="precision")
     assert st["form"] == "precision" and st["raw_storage"] == "c64"
     flat = img.reshape(-1)
@@ -211,10 +211,10 @@ def test_precision_config2_c64(tdc, orc):
 
 
 @pytest.mark.parametrize("storage,tol", [("f16", 3e-3), ("bf16", 2e-2)])
-def test_precision_config2_half_storage(tdc, orc, storage, tol):
+def test_precision_line_jitter_half_storage(tdc, orc, storage, tol):
     from sarlabtdc import scene
 
-    s = scene.config2()
+    s = scene.line_jitter()
     img, st = form(tdc, f64=s.f64(), kind="precision", storage=storage)
     assert st["raw_storage"] == storage
     flat = img.reshape(-1)
@@ -226,7 +226,7 @@ def test_precision_from_float_api(tdc, orc):
     """Precision form selected on the reference's float32 inputs (stop-and-hop)."""
     from sarlabtdc import scene
 
-    d = scene.config1().dropin()
+    d = scene.line_small().dropin()
     img, st = form(tdc, d, kind="precision")
     assert st["form"] == "precision"
     check(orc, img, oracle_dropin(orc, d), PREC_TOL)
@@ -258,7 +258,7 @@ def test_precision_range_window(tdc, orc):
 def test_precision_point_target_closed_form(tdc):
     from sarlabtdc import scene
 
-    s = scene.config2(nc=128, ns=128, side=8)
+    s = scene.line_jitter(nc=128, ns=128, side=8)
     p0 = s.grid[3, 5]
     amp = -0.3 + 0.9j
     s.raw = scene.simulate_beat(s.track, s.sample_times, p0[None], [amp]).astype(np.complex64)

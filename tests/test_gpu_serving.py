@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 def job_scene(j, nc, ns, side, spacing=3e-3):
     from sarlabtdc import scene
 
-    return scene.config5_job(j, (nc, ns, side, spacing))
+    return scene.stream_job(j, (nc, ns, side, spacing))
 
 
 def submit(srv, s, form="precision", storage="c64"):

@@ -361,10 +361,10 @@ def test_skipped_chunk_contributes_nothing(orc):
 # ---------------------------------------------------------------------------- reference method
 
 
-def test_reference_method_agrees_with_oracle_config1(orc):
+def test_reference_method_agrees_with_oracle_line_small(orc):
     from golden.make_golden import load_dropin
 
-    d, z = load_dropin("config1_plain.npz")
+    d, z = load_dropin("line_small_plain.npz")
     rm = orc.refmethod_f32(d, 0.0, 1e30, False, False, False, doppler=False)
     assert orc.rel_l2(rm, z["image_f64"]) < 5e-4
     rm2 = orc.refmethod_f32(d, 0.0, 1e30, False, False, False, doppler=True)
@@ -382,10 +382,10 @@ def test_reference_method_agrees_with_oracle_doppler(orc):
 # ---------------------------------------------------------------------------- golden fixtures
 
 
-def test_golden_config1_reproduces(orc):
+def test_golden_line_small_reproduces(orc):
     from golden.make_golden import load_dropin
 
-    d, z = load_dropin("config1_plain.npz")
+    d, z = load_dropin("line_small_plain.npz")
     img = orc.image_f64(d["rawData"], d["sampleTimes"], d["position"], d["focusGrid"],
                         float(d["modRate"]), float(d["startFreq"]))
     assert np.allclose(img, z["image_f64"], rtol=0, atol=1e-9 * np.abs(img).max())

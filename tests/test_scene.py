@@ -38,14 +38,14 @@ def test_nonuniform_times_direct_path():
 def test_configs_shapes_and_seeds():
     from sarlabtdc import scene
 
-    a, b = scene.config1(), scene.config1()
+    a, b = scene.line_small(), scene.line_small()
     assert a.raw.shape == (64, 128) and a.grid.shape == (32, 32, 3)
     assert np.array_equal(a.raw, b.raw)
-    assert not np.array_equal(a.raw, scene.config1(seed=1).raw)
+    assert not np.array_equal(a.raw, scene.line_small(seed=1).raw)
     d = a.dropin()
     assert d["position"].shape == (64, 128, 3) and d["position"].dtype == np.float32
     assert d["attitude"].shape == (64, 128, 4)
-    shapes = scene.config5_job_shapes(1000)
+    shapes = scene.stream_shapes(1000)
     nc = np.array([s[0] for s in shapes])
     side = np.array([s[2] for s in shapes])
     assert nc.min() >= 64 and nc.max() <= 2048 and side.min() >= 32 and side.max() <= 512
