@@ -50,7 +50,9 @@ struct StepTable {
 };
 
 hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
-                             int workgroups, int fin_workgroups, hipStream_t s);
+                             int workgroups, hipStream_t s);
+hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,
+                                 hipStream_t s);
 hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,
                                     int npri, int ns, PlainRec *out, hipStream_t s);
 

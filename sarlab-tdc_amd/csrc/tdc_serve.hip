@@ -1,7 +1,7 @@
 // tdc_serve.hip -- kernels of the serving form.
 //
-// A step of the server runs three kernels, captured once per grid-size bucket
-// in a hipGraph and replayed:
+// A step of the server runs two kernels, captured once per grid-size bucket in
+// a hipGraph and replayed, and a finalize kernel on its own stream:
 //   1. serve_reset_kernel   zeroes the step's tile counter (a kernel, not a
 //                           memset node, so the graph replays cleanly);
 //   2. serve_kernel         persistent: every workgroup takes the next tile
@@ -125,13 +125,18 @@ __global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(
 } // namespace
 
 hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
-                             int workgroups, int fin_workgroups, hipStream_t s)
+                             int workgroups, hipStream_t s)
 {
     hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(1), 0, s, counter);
     hipLaunchKernelGGL((serve_kernel<kServePixelsPrecision, kServePixelsPlain>), dim3(workgroups),
                        dim3(kBlock), 0, s, jobs, table, counter);
-    hipLaunchKernelGGL(serve_finalize_kernel, dim3(fin_workgroups), dim3(kBlock), 0, s, jobs,
-                       table);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(serve_finalize_kernel, dim3(workgroups), dim3(kBlock), 0, s, jobs, table);
     return hipGetLastError();
 }
 

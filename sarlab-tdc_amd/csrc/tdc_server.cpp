@@ -1,21 +1,31 @@
 // tdc_server.cpp -- serving form: continuously batched image formation.
 //
 // No counterpart in the reference, whose TdcProcessor forms one image per
-// blocking start() call (tdc-lib/src/tdcprocessor.cpp:59-167). Here a
-// scheduler thread owns one HIP stream and:
-//   1. admits queued jobs in arrival order while their HBM footprint fits the
-//      budget, uploads their inputs and stages them (stream-ordered
-//      allocations, so nothing waits for the device);
-//   2. builds a step: tiles (job, pixel tile, chirp range) taken from the
-//      admitted jobs in arrival order up to a work budget, plus the finalize
-//      blocks of the jobs whose last tiles are in the step;
-//   3. copies the step table to the device and replays the step graph of the
-//      grid-size bucket (reset counter, persistent correlate-sum kernel,
-//      finalize), keeping two steps in flight;
-//   4. when a step's event completes, copies the finished images to host
-//      memory, frees their device memory and wakes their waiters.
-// New jobs join at the next step boundary while earlier ones are still
-// running, and their tiles share the device with them.
+// blocking start() call (tdc-lib/src/tdcprocessor.cpp:59-167). Two host
+// threads drive one GPU:
+//   uploader   takes admitted jobs in arrival order, packs each job's inputs
+//              into pinned staging memory and moves them with one large
+//              host-to-device copy (an SDMA transfer, no kernel), then marks
+//              the job resident;
+//   scheduler  admits queued jobs in arrival order while their HBM footprint
+//              fits the budget, builds steps and completes them.
+// A step is a table of tiles (job, pixel tile, chirp range) taken from the
+// resident jobs -- an equal share of the step's work for every job first,
+// then the rest in arrival order -- plus the finalize blocks of the jobs
+// whose last tiles are in it. Steps alternate between two compute streams,
+// each replaying the step graph of its grid-size bucket (counter reset +
+// persistent correlate-sum kernel), so the next step's workgroups fill the
+// CUs while the previous step drains. A job's finalize follows the step
+// holding its last tiles on that step's stream, after the step before it (on
+// the other stream) has ended too; that covers all of its tiles.
+//
+// Nothing besides the steps needs a CU while steps run back to back: small
+// host-to-device copies would run as blit kernels and wait behind the
+// persistent workgroups, so job descriptors and step tables live in pinned
+// host memory that the kernels read directly (a few hundred bytes per tile),
+// finished images are written by the finalize straight into pinned host
+// memory, and the staging kernels of a job (plain records, FP16/BF16
+// conversion) run on the step stream ahead of the job's first step.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -48,6 +58,54 @@ int sfail(int code, const std::string &msg) { return tdc::set_last_error(code, m
 
 size_t align256(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
 
+// Pinned host memory, recycled by power-of-two size class (pinning pages is
+// slow, so blocks are kept until the server is destroyed).
+class PinnedPool
+{
+public:
+    explicit PinnedPool(unsigned flags) : flags_(flags) {}
+    ~PinnedPool()
+    {
+        for (void *p : all_) (void) hipHostFree(p);
+    }
+    hipError_t get(size_t n, void **out)
+    {
+        const size_t c = size_class(n);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto &v = free_[c];
+            if (!v.empty()) {
+                *out = v.back();
+                v.pop_back();
+                return hipSuccess;
+            }
+        }
+        hipError_t e = hipHostMalloc(out, c, flags_);
+        if (e != hipSuccess) return e;
+        std::lock_guard<std::mutex> lk(mu_);
+        all_.push_back(*out);
+        return hipSuccess;
+    }
+    void put(void *p, size_t n)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        free_[size_class(n)].push_back(p);
+    }
+
+private:
+    static size_t size_class(size_t n)
+    {
+        size_t c = 4096;
+        while (c < n) c <<= 1;
+        return c;
+    }
+    unsigned flags_;
+    std::mutex mu_;
+    std::map<size_t, std::vector<void *>> free_;
+    std::vector<void *> all_;
+};
+
 struct Job {
     int64_t id = 0;
     tdc_job_desc d{};
@@ -58,16 +116,17 @@ struct Job {
     int npx = 0, tile_px = 0, npt = 0, cps = 0, nsplit = 0, nfin = 0;
     int64_t ntiles = 0, next_tile = 0;
     double terms = 0.0;
-    size_t bytes = 0;
-    // device
-    void *arena = nullptr;
+    // device arena layout: [upload region | staged records or raw | partials]
+    size_t off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
+           off_part = 0, up_bytes = 0, bytes = 0;
+    bool needs_stage = false;
+    char *arena = nullptr;
     int slot = -1;
-    float2 *d_image = nullptr;
+    float *h_image = nullptr; // pinned, written by the finalize kernel
     // host
     std::vector<float> image;
     double t_submit = 0, t_admit = 0, t_first = -1, t_done = 0;
     int steps = 0;
-    int64_t last_step = -1;
 };
 
 struct Step {
@@ -75,6 +134,7 @@ struct Step {
     int buf = 0;
     std::vector<Job *> finishing;
     std::vector<Job *> touched;
+    std::vector<Job *> stage; // jobs whose staging kernels precede this step
 };
 
 } // namespace
@@ -83,31 +143,41 @@ struct tdc_server {
     int device = 0;
     double budget = 0.0;
     double step_terms = kDefaultStepTerms;
-    hipStream_t stream = nullptr, copy_stream = nullptr;
+    hipStream_t step_stream[2] = {nullptr, nullptr}; // steps alternate between these
+    hipStream_t up_stream = nullptr;                 // uploads (uploader thread)
+    hipStream_t free_stream = nullptr;               // stream-ordered frees
     hipEvent_t ev_start[2] = {nullptr, nullptr}, ev_end[2] = {nullptr, nullptr};
-    hipEvent_t ev_upload = nullptr;
-    tdc::ServeJob *d_jobs = nullptr;
-    tdc::StepTable *d_table[2] = {nullptr, nullptr};
-    tdc::StepTable *h_table[2] = {nullptr, nullptr};
-    unsigned int *d_counter = nullptr;
+    hipEvent_t ev_fin[2] = {nullptr, nullptr}, ev_staged[2] = {nullptr, nullptr};
+    hipEvent_t ev_origin = nullptr;
+    tdc::ServeJob *h_jobs = nullptr;       // pinned job table
+    tdc::ServeJob *dev_jobs = nullptr;     // its device address
+    tdc::StepTable *h_table[2] = {nullptr, nullptr};   // pinned step tables
+    tdc::StepTable *dev_table[2] = {nullptr, nullptr}; // their device addresses
+    unsigned int *d_counter = nullptr; // two counters, 256 bytes apart
     hipGraphExec_t graph[kNumBuckets][2] = {};
     int ngraphs = 0;
+    // staging for SDMA uploads, and coherent (uncached by the GPU) memory for
+    // the images the finalize writes
+    PinnedPool pinned{hipHostMallocDefault};
+    PinnedPool coherent{hipHostMallocCoherent | hipHostMallocMapped};
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     std::mutex mu;
-    std::condition_variable cv_sched, cv_done;
-    std::deque<Job *> queued;
-    std::vector<Job *> active;            // admitted, tiles outstanding or finishing
+    std::condition_variable cv_sched, cv_done, cv_up;
+    std::deque<Job *> queued;    // submitted, not yet admitted
+    std::deque<Job *> uploading; // admitted, waiting for the uploader
+    std::vector<Job *> active;   // resident, tiles outstanding or finishing
     std::map<int64_t, std::unique_ptr<Job>> jobs;
     std::vector<int> free_slots;
     int64_t next_id = 1;
     bool stop = false;
+    bool stop_uploader = false;
     std::string fatal;
-    std::thread worker;
+    std::thread worker, uploader;
 
     // stats (guarded by mu)
     int64_t submitted = 0, completed = 0, failed = 0, nsteps = 0;
-    double busy_ms = 0.0, terms_done = 0.0;
+    double busy_ms = 0.0, terms_done = 0.0, last_end_ms = 0.0;
     size_t in_use = 0, peak = 0;
 
     double now() const
@@ -118,12 +188,13 @@ struct tdc_server {
 
 namespace {
 
-// Footprint and tile decomposition of a job.
+// Footprint, arena layout and tile decomposition of a job.
 void plan(Job &j)
 {
     const tdc_job_desc &d = j.d;
     const size_t nterm = static_cast<size_t>(d.n_pri) * d.n_samples;
-    j.npx = d.n_rows * d.n_cols;
+    const size_t npx = static_cast<size_t>(d.n_rows) * d.n_cols;
+    j.npx = static_cast<int>(npx);
     const bool prec = d.form == TDC_FORM_PRECISION;
     j.tile_px = tdc::kBlock * (prec ? tdc::kServePixelsPrecision : tdc::kServePixelsPlain);
     j.npt = (j.npx + j.tile_px - 1) / j.tile_px;
@@ -133,27 +204,34 @@ void plan(Job &j)
     j.nsplit = (d.n_pri + j.cps - 1) / j.cps;
     j.ntiles = static_cast<int64_t>(j.npt) * j.nsplit;
     j.nfin = (j.npx + tdc::kBlock - 1) / tdc::kBlock;
-    j.terms = static_cast<double>(j.npx) * d.n_pri * d.n_samples;
+    j.terms = static_cast<double>(npx) * d.n_pri * d.n_samples;
     const int nblk = (d.n_samples + tdc::kAnchorBlock - 1) / tdc::kAnchorBlock;
-    size_t b = 0;
-    b += align256(8 * nterm);                                   // raw as uploaded
-    b += align256(sizeof(tdc::ChirpPos) * d.n_pri);             // chirp positions
-    b += align256(sizeof(float2) * j.npx);                      // image
+    size_t o = align256(8 * nterm); // raw samples at offset 0
+    j.off_cpos = o;
+    o += align256(sizeof(tdc::ChirpPos) * d.n_pri);
     if (prec) {
+        j.off_ka = o;
+        o += align256(8 * static_cast<size_t>(nblk));
+        j.off_grid = o;
+        o += align256(24 * npx);
         if (d.raw_storage != TDC_RAW_C64) {
-            b += align256(4 * nterm);                           // converted raw
-            b += align256(4 * static_cast<size_t>(d.n_samples)); // unit window for the conversion
+            j.off_ones = o;
+            o += align256(4 * static_cast<size_t>(d.n_samples));
         }
-        b += align256(24 * static_cast<size_t>(j.npx));         // FP64 grid
-        b += align256(8 * static_cast<size_t>(nblk));           // anchor table
-        b += align256(sizeof(double2) * static_cast<size_t>(j.npx) * j.nsplit);
     } else {
-        b += align256(4 * static_cast<size_t>(d.n_samples));    // k table
-        b += align256(sizeof(tdc::PlainRec) * nterm);           // records
-        b += align256(12 * static_cast<size_t>(j.npx));         // FP32 grid
-        b += align256(sizeof(float2) * static_cast<size_t>(j.npx) * j.nsplit);
+        j.off_kt = o;
+        o += align256(4 * static_cast<size_t>(d.n_samples));
+        j.off_grid = o;
+        o += align256(12 * npx);
     }
-    j.bytes = b;
+    j.up_bytes = o;
+    j.needs_stage = !prec || d.raw_storage != TDC_RAW_C64;
+    j.off_stage = o;
+    if (!prec) o += align256(sizeof(tdc::PlainRec) * nterm);
+    else if (d.raw_storage != TDC_RAW_C64) o += align256(4 * nterm);
+    j.off_part = o;
+    o += align256((prec ? sizeof(double2) : sizeof(float2)) * npx * j.nsplit);
+    j.bytes = o;
 }
 
 int validate(const tdc_job_desc *d)
@@ -183,7 +261,7 @@ int validate(const tdc_job_desc *d)
         }                                                                                      \
     } while (0)
 
-// Upload and stage one admitted job (scheduler thread, compute stream).
+// Pack, upload and describe one admitted job (uploader thread).
 bool upload_job(tdc_server *s, Job &j, std::string &err)
 {
     const tdc_job_desc &d = j.d;
@@ -194,30 +272,51 @@ bool upload_job(tdc_server *s, Job &j, std::string &err)
     if (prec && !tdc::precision_tables(d.sample_times, d.n_samples, d.start_freq, d.mod_rate,
                                        kanchor, dk, err))
         return false;
-    // uploads and staging run on the copy stream, beside the step in flight;
-    // the compute stream waits for them through an event
-    hipStream_t cs = s->copy_stream;
-    SRV_HIP(hipMallocAsync(&j.arena, j.bytes, cs));
-    char *p = static_cast<char *>(j.arena);
-    auto take = [&](size_t n) {
-        char *q = p;
-        p += align256(n);
-        return static_cast<void *>(q);
-    };
-    void *raw = take(8 * nterm);
-    auto *cpos = static_cast<tdc::ChirpPos *>(take(sizeof(tdc::ChirpPos) * d.n_pri));
-    j.d_image = static_cast<float2 *>(take(sizeof(float2) * j.npx));
-    std::vector<tdc::ChirpPos> hc(d.n_pri);
+    void *stage_mem = nullptr;
+    SRV_HIP(s->pinned.get(j.up_bytes, &stage_mem));
+    char *h = static_cast<char *>(stage_mem);
+    std::memcpy(h, d.raw_iq, 8 * nterm);
+    auto *hc = reinterpret_cast<tdc::ChirpPos *>(h + j.off_cpos);
     for (int n = 0; n < d.n_pri; ++n) {
         const double *c = d.chirp_position + 3 * static_cast<size_t>(n);
         hc[n] = tdc::ChirpPos{c[0], c[1], c[2], 0.0};
     }
-    SRV_HIP(hipMemcpyAsync(raw, d.raw_iq, 8 * nterm, hipMemcpyHostToDevice, cs));
-    SRV_HIP(hipMemcpyAsync(cpos, hc.data(), sizeof(tdc::ChirpPos) * d.n_pri,
-                           hipMemcpyHostToDevice, cs));
+    if (prec) {
+        std::memcpy(h + j.off_ka, kanchor.data(), 8 * kanchor.size());
+        std::memcpy(h + j.off_grid, d.grid, 24 * static_cast<size_t>(j.npx));
+        if (d.raw_storage != TDC_RAW_C64) {
+            // served raw data arrive windowed: the conversion uses a unit window
+            auto *w = reinterpret_cast<float *>(h + j.off_ones);
+            for (int m = 0; m < d.n_samples; ++m) w[m] = 1.0F;
+        }
+    } else {
+        auto *kt = reinterpret_cast<float *>(h + j.off_kt);
+        for (int m = 0; m < d.n_samples; ++m)
+            kt[m] = static_cast<float>(2.0 * (d.start_freq + d.mod_rate * d.sample_times[m]) / kC);
+        auto *g = reinterpret_cast<float *>(h + j.off_grid);
+        for (size_t i = 0; i < 3 * static_cast<size_t>(j.npx); ++i) g[i] = static_cast<float>(d.grid[i]);
+    }
+    void *arena = nullptr;
+    hipError_t e = hipMallocAsync(&arena, j.bytes, s->up_stream);
+    if (e == hipSuccess) {
+        j.arena = static_cast<char *>(arena);
+        e = hipMemcpyAsync(arena, stage_mem, j.up_bytes, hipMemcpyHostToDevice, s->up_stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->up_stream);
+    s->pinned.put(stage_mem, j.up_bytes);
+    if (e != hipSuccess) {
+        err = std::string("job upload failed: ") + hipGetErrorString(e);
+        return false;
+    }
+    void *img = nullptr;
+    SRV_HIP(s->coherent.get(sizeof(float2) * j.npx, &img));
+    j.h_image = static_cast<float *>(img);
+    void *img_dev = nullptr;
+    SRV_HIP(hipHostGetDevicePointer(&img_dev, img, 0));
+
     tdc::ServeJob sj{};
-    sj.cpos = cpos;
-    sj.image = j.d_image;
+    sj.cpos = reinterpret_cast<const tdc::ChirpPos *>(j.arena + j.off_cpos);
+    sj.image = static_cast<float2 *>(img_dev);
     sj.npx = j.npx;
     sj.npri = d.n_pri;
     sj.ns = d.n_samples;
@@ -225,59 +324,37 @@ bool upload_job(tdc_server *s, Job &j, std::string &err)
     sj.cps = j.cps;
     sj.form = prec ? 1 : 0;
     sj.storage = d.raw_storage;
-    std::vector<float> ones;
+    sj.partial = j.arena + j.off_part;
     if (prec) {
-        const void *staged = raw;
-        if (d.raw_storage != TDC_RAW_C64) {
-            // served raw data arrive windowed: convert with a unit window
-            void *conv = take(4 * nterm);
-            auto *w = static_cast<float *>(take(4 * static_cast<size_t>(d.n_samples)));
-            ones.assign(d.n_samples, 1.0F);
-            SRV_HIP(hipMemcpyAsync(w, ones.data(), 4 * ones.size(), hipMemcpyHostToDevice, cs));
-            SRV_HIP(tdc::launch_stage_precision(d.raw_storage, static_cast<const float *>(raw), w,
-                                                d.n_pri, d.n_samples, conv, cs));
-            staged = conv;
-        }
-        auto *grid = static_cast<double *>(take(24 * static_cast<size_t>(j.npx)));
-        auto *ka = static_cast<double *>(take(8 * kanchor.size()));
-        void *part = take(sizeof(double2) * static_cast<size_t>(j.npx) * j.nsplit);
-        SRV_HIP(hipMemcpyAsync(grid, d.grid, 24 * static_cast<size_t>(j.npx),
-                               hipMemcpyHostToDevice, cs));
-        SRV_HIP(hipMemcpyAsync(ka, kanchor.data(), 8 * kanchor.size(), hipMemcpyHostToDevice, cs));
-        sj.raw = staged;
-        sj.grid64 = grid;
-        sj.kanchor = ka;
+        sj.raw = d.raw_storage == TDC_RAW_C64 ? static_cast<const void *>(j.arena)
+                                              : static_cast<const void *>(j.arena + j.off_stage);
+        sj.grid64 = reinterpret_cast<const double *>(j.arena + j.off_grid);
+        sj.kanchor = reinterpret_cast<const double *>(j.arena + j.off_ka);
         sj.dk = dk;
-        sj.partial = part;
     } else {
-        auto *kt = static_cast<float *>(take(4 * static_cast<size_t>(d.n_samples)));
-        auto *rec = static_cast<tdc::PlainRec *>(take(sizeof(tdc::PlainRec) * nterm));
-        auto *grid = static_cast<float *>(take(12 * static_cast<size_t>(j.npx)));
-        void *part = take(sizeof(float2) * static_cast<size_t>(j.npx) * j.nsplit);
-        std::vector<float> hk(d.n_samples), hg(3 * static_cast<size_t>(j.npx));
-        for (int m = 0; m < d.n_samples; ++m)
-            hk[m] = static_cast<float>(2.0 * (d.start_freq + d.mod_rate * d.sample_times[m]) / kC);
-        for (size_t i = 0; i < hg.size(); ++i) hg[i] = static_cast<float>(d.grid[i]);
-        SRV_HIP(hipMemcpyAsync(kt, hk.data(), 4 * hk.size(), hipMemcpyHostToDevice, cs));
-        SRV_HIP(hipMemcpyAsync(grid, hg.data(), 4 * hg.size(), hipMemcpyHostToDevice, cs));
-        SRV_HIP(tdc::launch_stage_plain_chirp(static_cast<const float *>(raw), cpos, kt, d.n_pri,
-                                              d.n_samples, rec, cs));
-        // the pageable host vectors above must outlive their copies
-        SRV_HIP(hipStreamSynchronize(cs));
-        sj.raw = rec;
-        sj.grid32 = grid;
-        sj.partial = part;
+        sj.raw = j.arena + j.off_stage;
+        sj.grid32 = reinterpret_cast<const float *>(j.arena + j.off_grid);
     }
-    if (static_cast<size_t>(p - static_cast<char *>(j.arena)) > j.bytes) {
-        err = "internal error: job footprint underestimated";
-        return false;
-    }
-    SRV_HIP(hipMemcpyAsync(s->d_jobs + j.slot, &sj, sizeof sj, hipMemcpyHostToDevice, cs));
-    SRV_HIP(hipEventRecord(s->ev_upload, cs));
-    SRV_HIP(hipStreamWaitEvent(s->stream, s->ev_upload, 0));
-    // the descriptor and the host-side tables are pageable: finish their copies
-    SRV_HIP(hipStreamSynchronize(cs));
+    // the kernels read the descriptor from pinned memory; it is written before
+    // any step that names the job is launched
+    s->h_jobs[j.slot] = sj;
     return true;
+}
+
+// Staging kernels of a job, on the stream of its first step (scheduler thread).
+hipError_t stage_job(const Job &j, hipStream_t ss)
+{
+    const tdc_job_desc &d = j.d;
+    if (d.form == TDC_FORM_PRECISION) {
+        return tdc::launch_stage_precision(d.raw_storage, reinterpret_cast<const float *>(j.arena),
+                                           reinterpret_cast<const float *>(j.arena + j.off_ones),
+                                           d.n_pri, d.n_samples, j.arena + j.off_stage, ss);
+    }
+    return tdc::launch_stage_plain_chirp(
+        reinterpret_cast<const float *>(j.arena),
+        reinterpret_cast<const tdc::ChirpPos *>(j.arena + j.off_cpos),
+        reinterpret_cast<const float *>(j.arena + j.off_kt), d.n_pri, d.n_samples,
+        reinterpret_cast<tdc::PlainRec *>(j.arena + j.off_stage), ss);
 }
 
 bool ensure_graph(tdc_server *s, int bucket, int buf, std::string &err)
@@ -288,8 +365,8 @@ bool ensure_graph(tdc_server *s, int bucket, int buf, std::string &err)
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        e = tdc::launch_serve_step(s->d_jobs, s->d_table[buf], s->d_counter, kBuckets[bucket],
-                                   kFinWorkgroups, cs);
+        e = tdc::launch_serve_step(s->dev_jobs, s->dev_table[buf], s->d_counter + 64 * buf,
+                                   kBuckets[bucket], cs);
         hipError_t e2 = hipStreamEndCapture(cs, &g);
         if (e == hipSuccess) e = e2;
     }
@@ -313,24 +390,27 @@ void fail_job_locked(tdc_server *s, Job *j, int code, const std::string &msg)
     ++s->failed;
 }
 
-// Finish the jobs of a completed step (scheduler thread).
+// Finish the jobs of a step whose finalize has ended (scheduler thread).
 bool complete_step(tdc_server *s, Step &st, std::string &err)
 {
-    SRV_HIP(hipEventSynchronize(s->ev_end[st.buf]));
-    float ms = 0.0F;
-    SRV_HIP(hipEventElapsedTime(&ms, s->ev_start[st.buf], s->ev_end[st.buf]));
+    SRV_HIP(hipEventSynchronize(s->ev_fin[st.buf]));
+    float t0 = 0.0F, t1 = 0.0F;
+    SRV_HIP(hipEventElapsedTime(&t0, s->ev_origin, s->ev_start[st.buf]));
+    SRV_HIP(hipEventElapsedTime(&t1, s->ev_origin, s->ev_end[st.buf]));
     for (Job *j : st.finishing) {
-        j->image.resize(2 * static_cast<size_t>(j->npx));
-        SRV_HIP(hipMemcpyAsync(j->image.data(), j->d_image, sizeof(float2) * j->npx,
-                               hipMemcpyDeviceToHost, s->copy_stream));
-    }
-    SRV_HIP(hipStreamSynchronize(s->copy_stream));
-    const double t = s->now();
-    for (Job *j : st.finishing) SRV_HIP(hipFreeAsync(j->arena, s->copy_stream));
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->busy_ms += ms;
-    for (Job *j : st.finishing) {
+        j->image.assign(j->h_image, j->h_image + 2 * static_cast<size_t>(j->npx));
+        s->coherent.put(j->h_image, sizeof(float2) * j->npx);
+        j->h_image = nullptr;
+        SRV_HIP(hipFreeAsync(j->arena, s->free_stream));
         j->arena = nullptr;
+    }
+    const double t = s->now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    // device busy time: union of the step intervals (consecutive steps overlap)
+    const double a = std::max(static_cast<double>(t0), s->last_end_ms);
+    if (t1 > a) s->busy_ms += t1 - a;
+    s->last_end_ms = std::max(s->last_end_ms, static_cast<double>(t1));
+    for (Job *j : st.finishing) {
         j->state = TDC_JOB_DONE;
         j->t_done = t;
         s->in_use -= j->bytes;
@@ -340,7 +420,40 @@ bool complete_step(tdc_server *s, Step &st, std::string &err)
         s->active.erase(std::find(s->active.begin(), s->active.end(), j));
     }
     s->cv_done.notify_all();
+    s->cv_sched.notify_all();
     return true;
+}
+
+// Uploads admitted jobs in arrival order (uploader thread).
+void uploader(tdc_server *s)
+{
+    if (hipSetDevice(s->device) != hipSuccess) return;
+    for (;;) {
+        Job *j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_up.wait(lk, [&] { return !s->uploading.empty() || s->stop_uploader; });
+            if (s->uploading.empty()) return;
+            j = s->uploading.front();
+        }
+        std::string why;
+        const bool okay = upload_job(s, *j, why);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->uploading.pop_front();
+        if (okay) {
+            s->active.push_back(j);
+        } else {
+            s->in_use -= j->bytes;
+            s->free_slots.push_back(j->slot);
+            if (j->arena) (void) hipFreeAsync(j->arena, s->up_stream);
+            j->arena = nullptr;
+            if (j->h_image) s->coherent.put(j->h_image, sizeof(float2) * j->npx);
+            j->h_image = nullptr;
+            fail_job_locked(s, j, TDC_ERR_UNSUPPORTED, why);
+            s->cv_done.notify_all();
+        }
+        s->cv_sched.notify_all();
+    }
 }
 
 void scheduler(tdc_server *s)
@@ -365,11 +478,10 @@ void scheduler(tdc_server *s)
         return;
     }
     for (;;) {
-        // 1. admission, in arrival order
-        std::vector<Job *> admit;
+        // 1. admission in arrival order: charge the budget, hand to the uploader
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->queued.empty() && s->active.empty() && inflight.empty()) {
+            if (s->queued.empty() && s->uploading.empty() && s->active.empty() && inflight.empty()) {
                 if (s->stop) break;
                 s->cv_sched.wait_for(lk, std::chrono::milliseconds(20));
                 continue;
@@ -384,32 +496,20 @@ void scheduler(tdc_server *s)
                 s->free_slots.pop_back();
                 j->state = TDC_JOB_ADMITTED;
                 j->t_admit = s->now();
-                admit.push_back(j);
+                s->uploading.push_back(j);
+                s->cv_up.notify_one();
             }
-        }
-        for (Job *j : admit) {
-            std::string why;
-            if (!upload_job(s, *j, why)) {
-                std::lock_guard<std::mutex> lk(s->mu);
-                s->in_use -= j->bytes;
-                s->free_slots.push_back(j->slot);
-                if (j->arena) (void) hipFreeAsync(j->arena, s->copy_stream);
-                j->arena = nullptr;
-                fail_job_locked(s, j, TDC_ERR_UNSUPPORTED, why);
-                s->cv_done.notify_all();
-                continue;
-            }
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->active.push_back(j);
         }
 
-        // 2. the two-in-flight rule: finish the older step before reusing its buffer
-        if (inflight.size() == 2) {
+        // 2. completions: the oldest step when two are in flight (its table
+        //    buffer is needed next), or any step whose finalize already ended
+        while (!inflight.empty() &&
+               (inflight.size() == 2 || hipEventQuery(s->ev_fin[inflight.front().buf]) == hipSuccess)) {
             if (!complete_step(s, inflight.front(), err)) return die(err);
             inflight.pop_front();
         }
 
-        // 3. build the next step from the admitted jobs, oldest first
+        // 3. build the next step from the resident jobs
         Step st;
         st.index = step_index;
         st.buf = static_cast<int>(step_index & 1);
@@ -418,7 +518,7 @@ void scheduler(tdc_server *s)
         double work = 0.0;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            // processor sharing: every admitted job with tiles left gets an equal
+            // processor sharing: every resident job with tiles left gets an equal
             // share of the step (at least one tile), oldest first; what the
             // shares leave is then filled in arrival order
             int nwant = 0;
@@ -443,7 +543,10 @@ void scheduler(tdc_server *s)
                     took = true;
                 }
                 if (took) {
-                    if (j->t_first < 0) j->t_first = s->now();
+                    if (j->t_first < 0) {
+                        j->t_first = s->now();
+                        if (j->needs_stage) st.stage.push_back(j);
+                    }
                     if (std::find(st.touched.begin(), st.touched.end(), j) == st.touched.end()) {
                         ++j->steps;
                         st.touched.push_back(j);
@@ -487,14 +590,26 @@ void scheduler(tdc_server *s)
             }
         }
         if (!ensure_graph(s, bucket, st.buf, err)) return die(err);
-        const size_t bytes = offsetof(tdc::StepTable, tiles) + sizeof(tdc::ServeTile) * nt;
-        hipError_t e = hipMemcpyAsync(s->d_table[st.buf], tab, bytes, hipMemcpyHostToDevice, s->stream);
+        const int p = st.buf;
+        hipStream_t ss = s->step_stream[p];
+        hipError_t e = hipSuccess;
+        // jobs staged by the previous step (the other stream) must be staged
+        // before this step reads them
+        if (step_index > 0) e = hipStreamWaitEvent(ss, s->ev_staged[1 - p], 0);
+        for (Job *j : st.stage)
+            if (e == hipSuccess) e = stage_job(*j, ss);
+        if (e == hipSuccess) e = hipEventRecord(s->ev_staged[p], ss);
+        if (e == hipSuccess) e = hipEventRecord(s->ev_start[p], ss);
+        if (e == hipSuccess) e = hipGraphLaunch(s->graph[bucket][p], ss);
+        if (e == hipSuccess) e = hipEventRecord(s->ev_end[p], ss);
+        // finalize on the same stream, right behind this step, once the previous
+        // step (the other stream) has ended too: the two cover every earlier
+        // step of the finishing jobs
+        if (e == hipSuccess && step_index > 0 && nf > 0)
+            e = hipStreamWaitEvent(ss, s->ev_end[1 - p], 0);
         if (e == hipSuccess && nf > 0)
-            e = hipMemcpyAsync(&s->d_table[st.buf]->fin[0], &tab->fin[0], sizeof(tdc::FinTile) * nf,
-                               hipMemcpyHostToDevice, s->stream);
-        if (e == hipSuccess) e = hipEventRecord(s->ev_start[st.buf], s->stream);
-        if (e == hipSuccess) e = hipGraphLaunch(s->graph[bucket][st.buf], s->stream);
-        if (e == hipSuccess) e = hipEventRecord(s->ev_end[st.buf], s->stream);
+            e = tdc::launch_serve_finalize(s->dev_jobs, s->dev_table[p], kFinWorkgroups, ss);
+        if (e == hipSuccess) e = hipEventRecord(s->ev_fin[p], ss);
         if (e != hipSuccess) return die(std::string("step launch failed: ") + hipGetErrorString(e));
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -528,18 +643,29 @@ int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, t
     std::string err;
     auto init = [&]() -> bool {
         SRV_HIP(hipSetDevice(gpu_num));
-        SRV_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-        SRV_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+        SRV_HIP(hipStreamCreateWithFlags(&s->up_stream, hipStreamNonBlocking));
+        SRV_HIP(hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking));
+        SRV_HIP(hipEventCreate(&s->ev_origin));
         for (int i = 0; i < 2; ++i) {
+            SRV_HIP(hipStreamCreateWithFlags(&s->step_stream[i], hipStreamNonBlocking));
             SRV_HIP(hipEventCreate(&s->ev_start[i]));
             SRV_HIP(hipEventCreate(&s->ev_end[i]));
-            SRV_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_table[i]), sizeof(tdc::StepTable)));
+            SRV_HIP(hipEventCreateWithFlags(&s->ev_fin[i], hipEventDisableTiming));
+            SRV_HIP(hipEventCreateWithFlags(&s->ev_staged[i], hipEventDisableTiming));
+            // step tables and the job table are rewritten by the host between
+            // launches: coherent memory, so no GPU cache keeps a stale copy
             SRV_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_table[i]), sizeof(tdc::StepTable),
-                                  hipHostMallocDefault));
+                                  hipHostMallocCoherent | hipHostMallocMapped));
+            SRV_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->dev_table[i]),
+                                            s->h_table[i], 0));
         }
-        SRV_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_jobs), sizeof(tdc::ServeJob) * tdc::kMaxJobSlots));
-        SRV_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_counter), 256));
-        SRV_HIP(hipEventCreateWithFlags(&s->ev_upload, hipEventDisableTiming));
+        SRV_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_jobs),
+                              sizeof(tdc::ServeJob) * tdc::kMaxJobSlots,
+                              hipHostMallocCoherent | hipHostMallocMapped));
+        SRV_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->dev_jobs), s->h_jobs, 0));
+        SRV_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_counter), 512));
+        SRV_HIP(hipEventRecord(s->ev_origin, s->step_stream[0]));
+        SRV_HIP(hipEventSynchronize(s->ev_origin));
         return true;
     };
     if (!init()) {
@@ -549,6 +675,7 @@ int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, t
     for (int i = tdc::kMaxJobSlots - 1; i >= 0; --i) s->free_slots.push_back(i);
     tdc_server *raw = s.release();
     raw->worker = std::thread(scheduler, raw);
+    raw->uploader = std::thread(uploader, raw);
     *out = raw;
     tdc::clear_last_error();
     return TDC_OK;
@@ -659,22 +786,30 @@ int tdc_server_destroy(tdc_server *s)
         s->cv_done.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop_uploader = true;
+        s->cv_up.notify_all();
+    }
+    if (s->uploader.joinable()) s->uploader.join();
     (void) hipSetDevice(s->device);
-    if (s->stream) (void) hipStreamSynchronize(s->stream);
+    (void) hipDeviceSynchronize();
     for (auto &row : s->graph)
         for (auto &g : row)
             if (g) (void) hipGraphExecDestroy(g);
     for (int i = 0; i < 2; ++i) {
         if (s->ev_start[i]) (void) hipEventDestroy(s->ev_start[i]);
         if (s->ev_end[i]) (void) hipEventDestroy(s->ev_end[i]);
-        if (s->d_table[i]) (void) hipFree(s->d_table[i]);
+        if (s->ev_fin[i]) (void) hipEventDestroy(s->ev_fin[i]);
+        if (s->ev_staged[i]) (void) hipEventDestroy(s->ev_staged[i]);
+        if (s->step_stream[i]) (void) hipStreamDestroy(s->step_stream[i]);
         if (s->h_table[i]) (void) hipHostFree(s->h_table[i]);
     }
-    if (s->ev_upload) (void) hipEventDestroy(s->ev_upload);
-    if (s->d_jobs) (void) hipFree(s->d_jobs);
+    if (s->ev_origin) (void) hipEventDestroy(s->ev_origin);
+    if (s->h_jobs) (void) hipHostFree(s->h_jobs);
     if (s->d_counter) (void) hipFree(s->d_counter);
-    if (s->copy_stream) (void) hipStreamDestroy(s->copy_stream);
-    if (s->stream) (void) hipStreamDestroy(s->stream);
+    if (s->free_stream) (void) hipStreamDestroy(s->free_stream);
+    if (s->up_stream) (void) hipStreamDestroy(s->up_stream);
     delete s;
     return TDC_OK;
 }
