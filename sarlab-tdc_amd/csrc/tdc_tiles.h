@@ -30,6 +30,42 @@ template <class T> __device__ __forceinline__ cptr<T> as_const(const T *p)
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
 
+// |d| from |d|^2 in FP64: FP32 reciprocal-square-root seed and one FP64
+// Newton step (relative error ~1e-13, i.e. ~1e-10 cycles of phase at metre
+// ranges); zero stays zero.
+__device__ __forceinline__ double range_from_sq(double r2)
+{
+    const double y0 = static_cast<double>(__builtin_amdgcn_rsqf(static_cast<float>(r2)));
+    const double y1 = y0 * fma(-0.5 * r2 * y0, y0, 1.5);
+    return r2 > 0.0 ? r2 * y1 : 0.0;
+}
+
+// sin and cos of 2 pi x for x in [0, 1): reduction to the nearest quarter
+// turn, |2 pi f| <= pi/4, then the Taylor series through a^9 (sin) and a^10
+// (cos), whose truncation error there is below 2e-9.
+__device__ __forceinline__ void sincos_rev(float x, float &s, float &c)
+{
+    const float q = __builtin_rintf(4.0F * x);
+    const float f = fmaf(-0.25F, q, x);
+    const float a = 6.28318530717958647692F * f;
+    const float a2 = a * a;
+    float ps = fmaf(a2, 2.75573192e-6F, -1.98412698e-4F);
+    ps = fmaf(a2, ps, 8.33333333e-3F);
+    ps = fmaf(a2, ps, -1.66666667e-1F);
+    const float sa = fmaf(a * a2, ps, a);
+    float pc = fmaf(a2, -2.75573192e-7F, 2.48015873e-5F);
+    pc = fmaf(a2, pc, -1.38888889e-3F);
+    pc = fmaf(a2, pc, 4.16666667e-2F);
+    pc = fmaf(a2, pc, -0.5F);
+    const float ca = fmaf(a2, pc, 1.0F);
+    const int qi = static_cast<int>(q) & 3;
+    // rotate (cos a, sin a) by qi quarter turns
+    s = (qi & 1) ? ca : sa;
+    c = (qi & 1) ? sa : ca;
+    if (qi == 1 || qi == 2) c = -c;
+    if (qi >= 2) s = -s;
+}
+
 // Error-free accumulation: acc + comp carries the running sum.
 __device__ __forceinline__ void two_sum_acc(float &acc, float &comp, float x)
 {
@@ -157,10 +193,10 @@ __device__ __forceinline__ void precision_tile(
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             const double dx = c.x - px[j], dy = c.y - py[j], dz = c.z - pz[j];
-            rng[j] = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
+            rng[j] = range_from_sq(fma(dz, dz, fma(dy, dy, dx * dx)));
             double rot = dk * rng[j];
             rot -= floor(rot);
-            sincospif(2.0F * static_cast<float>(rot), &wi[j], &wr[j]);
+            sincos_rev(static_cast<float>(rot), wi[j], wr[j]);
         }
         const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
         float chr[P], chi[P];
