@@ -45,6 +45,14 @@ extern "C" {
 #define TDC_FORM_PLAIN 0     /* direct per-term FP32 reference phase: the drop-in */
 #define TDC_FORM_PRECISION 1 /* FP64 range/anchor + FP32 rotation recurrence */
 
+/* Engines of the precision form (additive). Both compute the same sums to
+ * FP32-level accuracy; the matrix engine forms the 8-sample group sums of
+ * every 64-sample block with FP16 hi/lo MFMA (32x32x16) and the vector engine
+ * runs a one-sample rotation recurrence on the VALU. */
+#define TDC_ENGINE_AUTO 0   /* matrix, unless pixels_per_lane > 1 is forced */
+#define TDC_ENGINE_VECTOR 1
+#define TDC_ENGINE_MATRIX 2
+
 /* Raw sample storage on the device for the precision form (additive). */
 #define TDC_RAW_C64 0  /* complex64 */
 #define TDC_RAW_F16 1  /* two IEEE half values per sample */
@@ -71,6 +79,8 @@ typedef struct tdc_stats {
     int32_t workgroups;
     int64_t skipped_chunks;   /* (pixel, chunk) pairs skipped; only counted when doppler_active */
     int64_t processed_chunks; /* (pixel, chunk) pairs processed */
+    int32_t engine;           /* TDC_ENGINE_* actually run (precision form), else 0 */
+    int32_t reserved;
 } tdc_stats;
 
 /* Message for the last failing call on this thread ("" when none). */
@@ -129,6 +139,8 @@ int tdc_set_range_window(tdc_processor *h, int kind, double p0, double p1);
 /* Form and device raw storage used by the next start / prepare. */
 int tdc_set_form(tdc_processor *h, int form);
 int tdc_set_raw_storage(tdc_processor *h, int storage);
+/* Engine of the precision form used by the next start / prepare. */
+int tdc_set_engine(tdc_processor *h, int engine);
 
 /* FP64 inputs for the precision form: raw_iq complex64 [n_pri][n_samples],
  * sample_times [n_samples] (uniformly spaced), chirp_position [n_pri][3]

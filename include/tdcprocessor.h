@@ -91,6 +91,7 @@ public:
     void setFocusGridF64(double const *focusGrid, int nRows, int nCols);
     void setForm(int form);             /* TDC_FORM_* */
     void setRawStorage(int storage);    /* TDC_RAW_* */
+    void setEngine(int engine);         /* TDC_ENGINE_* (precision form) */
     void setRangeWindow(int kind, double p0 = 0.0, double p1 = 0.0); /* TDC_WIN_* */
     void setLaunch(int pixelsPerLane, int chirpSplits);
     void prepare(float dopplerWinCenter, float dopplerBandwidth, bool dopCentroidWin,

@@ -52,6 +52,18 @@ int parse_storage(const py::object &o)
     return o.cast<int>();
 }
 
+int parse_engine(const py::object &o)
+{
+    if (py::isinstance<py::str>(o)) {
+        const auto s = o.cast<std::string>();
+        if (s == "auto") return TDC_ENGINE_AUTO;
+        if (s == "vector") return TDC_ENGINE_VECTOR;
+        if (s == "matrix") return TDC_ENGINE_MATRIX;
+        throw std::invalid_argument("engine must be 'auto', 'vector' or 'matrix'");
+    }
+    return o.cast<int>();
+}
+
 int parse_window(const py::object &o)
 {
     if (py::isinstance<py::str>(o)) {
@@ -178,6 +190,7 @@ public:
 
     void setForm(const py::object &form) { proc_->setForm(parse_form(form)); }
     void setRawStorage(const py::object &s) { proc_->setRawStorage(parse_storage(s)); }
+    void setEngine(const py::object &e) { proc_->setEngine(parse_engine(e)); }
     void setRangeWindow(const py::object &kind, double p0, double p1)
     {
         int k = parse_window(kind);
@@ -222,6 +235,7 @@ public:
         d["workgroups"] = st.workgroups;
         d["skipped_chunks"] = st.skipped_chunks;
         d["processed_chunks"] = st.processed_chunks;
+        d["engine"] = st.engine == TDC_ENGINE_MATRIX ? "matrix" : (st.engine == TDC_ENGINE_VECTOR ? "vector" : "none");
         return d;
     }
 
@@ -381,6 +395,7 @@ PYBIND11_MODULE(_sarlabtdc, m)
         .def("setFocusGridF64", &PyTdcProcessor::setFocusGridF64)
         .def("setForm", &PyTdcProcessor::setForm, py::arg("form"))
         .def("setRawStorage", &PyTdcProcessor::setRawStorage, py::arg("storage"))
+        .def("setEngine", &PyTdcProcessor::setEngine, py::arg("engine"))
         .def("setRangeWindow", &PyTdcProcessor::setRangeWindow, py::arg("kind"),
              py::arg("p0") = 0.0, py::arg("p1") = 0.0)
         .def("setLaunch", &PyTdcProcessor::setLaunch, py::arg("pixelsPerLane"),

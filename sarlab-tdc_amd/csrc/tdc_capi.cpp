@@ -180,18 +180,20 @@ struct tdc_processor {
     // options
     int form = TDC_FORM_PLAIN;
     int storage = TDC_RAW_C64;
+    int engine = TDC_ENGINE_AUTO;
     int win_kind = TDC_WIN_HAMMING;
     double win_p0 = 25.0 / 46.0, win_p1 = 0.0; // tuning.h:31
     int force_ppl = 0, force_split = 0;
 
     // device buffers
     DevBuf d_raw, d_pos, d_vel, d_att, d_kt, d_wr, d_rec, d_grid, d_part, d_img, d_cpos,
-        d_kanchor, d_counters;
+        d_kanchor, d_counters, d_atab, d_ascale;
 
     // prepared state
     bool prepared = false;
     int run_form = TDC_FORM_PLAIN;
     int run_storage = TDC_RAW_C64;
+    int run_engine = 0;
     bool doppler = false;
     tdc::Geom geom{};
     tdc::DopParams dp{};
@@ -357,6 +359,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     TDC_HIP(h->d_img.ensure(sizeof(float2) * static_cast<size_t>(npx)));
     if (nsplit > 1) TDC_HIP(h->d_part.ensure(sizeof(float2) * static_cast<size_t>(npx) * nsplit));
     h->run_form = TDC_FORM_PLAIN;
+    h->run_engine = 0;
     h->run_storage = TDC_RAW_C64;
     return TDC_OK;
 }
@@ -434,7 +437,21 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     TDC_HIP(tdc::launch_stage_precision(h->storage, h->d_raw.as<float>(), h->d_wr.as<float>(), npri,
                                         ns, h->d_rec.p, s));
     h->doppler = false;
-    h->ppl = h->force_ppl ? h->force_ppl : 2;
+    int engine = h->engine;
+    if (engine == TDC_ENGINE_AUTO) engine = h->force_ppl > 1 ? TDC_ENGINE_VECTOR : TDC_ENGINE_MATRIX;
+    if (engine == TDC_ENGINE_MATRIX) {
+        if (h->force_ppl > 1)
+            return fail(TDC_ERR_INVALID_ARGUMENT, "the matrix engine runs one pixel per lane");
+        TDC_HIP(h->d_atab.ensure(tdc::matrix_table_bytes(npri, ns, h->storage)));
+        TDC_HIP(h->d_ascale.ensure(sizeof(float) * static_cast<size_t>(npri)));
+        TDC_HIP(tdc::launch_stage_matrix(h->storage, h->d_rec.p, npri, ns,
+                                         tdc::matrix_parts(h->storage), h->d_atab.p,
+                                         h->d_ascale.as<float>(), s));
+        h->ppl = 1;
+    } else {
+        h->ppl = h->force_ppl ? h->force_ppl : 2;
+    }
+    h->run_engine = engine;
     const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
     const int nsplit = choose_split(ntiles, npri, h->force_split);
     make_geom(h->geom, npx, npri, ns, h->ppl, nsplit);
@@ -473,10 +490,15 @@ bool precision_tables(const double *t, int ns, double f0, double krate,
             return false;
         }
     }
-    const int nblk = (ns + kAnchorBlock - 1) / kAnchorBlock;
+    // one entry per anchor block, padded to whole block pairs; a padding
+    // block holds no samples and takes its time from the uniform spacing
+    const int nblk = 2 * tdc::matrix_pairs(ns);
     kanchor.resize(nblk);
-    for (int b = 0; b < nblk; ++b)
-        kanchor[b] = 2.0 * (f0 + krate * t[b * kAnchorBlock]) / kSpeedOfLight;
+    for (int b = 0; b < nblk; ++b) {
+        const int m = b * kAnchorBlock;
+        const double tm = m < ns ? t[m] : t[0] + m * dt;
+        kanchor[b] = 2.0 * (f0 + krate * tm) / kSpeedOfLight;
+    }
     dk = 2.0 * krate * dt / kSpeedOfLight;
     return true;
 }
@@ -487,7 +509,7 @@ extern "C" {
 
 const char *tdc_last_error(void) { return t_err.c_str(); }
 
-int tdc_abi_version(void) { return 10000; }
+int tdc_abi_version(void) { return 10100; }
 
 int tdc_device_count(void)
 {
@@ -621,6 +643,16 @@ int tdc_set_raw_storage(tdc_processor *h, int storage)
     return ok();
 }
 
+int tdc_set_engine(tdc_processor *h, int engine)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (engine != TDC_ENGINE_AUTO && engine != TDC_ENGINE_VECTOR && engine != TDC_ENGINE_MATRIX)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "unknown engine");
+    h->engine = engine;
+    h->prepared = false;
+    return ok();
+}
+
 int tdc_set_raw_data_f64(tdc_processor *h, const float *raw_iq, const double *pri_times,
                          const double *sample_times, const double *chirp_position, int n_pri,
                          int n_samples, double mod_rate, double start_freq)
@@ -726,10 +758,17 @@ int tdc_run(tdc_processor *h, int iterations)
                 TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
                                                  h->d_img.as<float2>(), s));
         } else {
-            TDC_HIP(tdc::launch_precision(h->ppl, h->run_storage, h->d_rec.p,
-                                          h->d_cpos.as<tdc::ChirpPos>(), h->d_grid.as<double>(),
-                                          h->d_kanchor.as<double>(), h->dk, g,
-                                          h->d_part.as<double2>(), s, &wgs));
+            if (h->run_engine == TDC_ENGINE_MATRIX) {
+                TDC_HIP(tdc::launch_precision_matrix(
+                    tdc::matrix_parts(h->run_storage), h->d_atab.p, h->d_ascale.as<float>(),
+                    h->d_cpos.as<tdc::ChirpPos>(), h->d_grid.as<double>(),
+                    h->d_kanchor.as<double>(), h->dk, g, h->d_part.as<double2>(), s, &wgs));
+            } else {
+                TDC_HIP(tdc::launch_precision(h->ppl, h->run_storage, h->d_rec.p,
+                                              h->d_cpos.as<tdc::ChirpPos>(),
+                                              h->d_grid.as<double>(), h->d_kanchor.as<double>(),
+                                              h->dk, g, h->d_part.as<double2>(), s, &wgs));
+            }
             TDC_HIP(hipEventRecord(h->ev_b, s));
             TDC_HIP(tdc::launch_finalize_f64(h->d_part.as<double2>(), g.nsplit, g.npx,
                                              h->d_img.as<float2>(), s));
@@ -753,6 +792,7 @@ int tdc_run(tdc_processor *h, int iterations)
     st.chirp_splits = g.nsplit;
     st.pixels_per_lane = h->ppl;
     st.workgroups = wgs;
+    st.engine = h->run_form == TDC_FORM_PRECISION ? h->run_engine : 0;
     if (h->doppler && g.npx > 0) {
         unsigned long long c[2] = {0, 0};
         TDC_HIP(hipMemcpy(c, h->d_counters.p, sizeof c, hipMemcpyDeviceToHost));

@@ -16,6 +16,14 @@ constexpr int kChunk = 128;
 constexpr int kBlock = 256;
 // Samples per FP64 re-anchor block of the precision form.
 constexpr int kAnchorBlock = 64;
+// Matrix engine of the precision form: samples per 8-sample group (one
+// complex k-slice of the MFMA) and per block pair (one 32x32x16 A tile: two
+// anchor blocks of eight groups, real and imaginary parts).
+constexpr int kGroup = 8;
+constexpr int kPairSamples = 2 * kAnchorBlock;
+// Bytes of one A tile: 64 lanes x 8 FP16 values.
+constexpr int kTileBytes = 64 * 16;
+
 
 // One (chirp, sample) term of the plain form, staged once per job.
 // Every field is wave-uniform in the correlate-sum kernel, so a record is
@@ -85,6 +93,21 @@ hipError_t launch_stage_precision(int storage, const float *raw, const float *wr
 hipError_t launch_precision(int ppl, int storage, const void *raw, const ChirpPos *cpos,
                             const double *grid, const double *kanchor, double dk, const Geom &g,
                             double2 *out, hipStream_t s, int *workgroups);
+// Matrix engine: A tiles of every (chirp, block pair) from the staged raw
+// samples (in `storage` format), `parts` = 2 (FP16 hi + lo) for c64 raw and
+// 1 for raw that FP16 holds exactly (f16, bf16); one scale per chirp.
+hipError_t launch_stage_matrix(int storage, const void *staged, int npri, int ns, int parts,
+                               void *atab, float *inv_scale, hipStream_t s);
+hipError_t launch_precision_matrix(int parts, const void *atab, const float *inv_scale,
+                                   const ChirpPos *cpos, const double *grid,
+                                   const double *kanchor, double dk, const Geom &g,
+                                   double2 *out, hipStream_t s, int *workgroups);
+__host__ __device__ inline int matrix_pairs(int ns) { return (ns + kPairSamples - 1) / kPairSamples; }
+inline int matrix_parts(int storage) { return storage == 0 ? 2 : 1; }
+inline size_t matrix_table_bytes(int npri, int ns, int storage)
+{
+    return static_cast<size_t>(npri) * matrix_pairs(ns) * matrix_parts(storage) * kTileBytes;
+}
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
 

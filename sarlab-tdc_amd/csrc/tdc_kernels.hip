@@ -353,6 +353,110 @@ __global__ __launch_bounds__(kBlock) void precision_kernel(
                            out + static_cast<size_t>(split) * g.npx);
 }
 
+// Matrix engine staging: one workgroup per chirp. The chirp's samples (in
+// their storage format) are scaled by a power of two that puts the largest
+// component in [2^13, 2^14) and laid out as the A operand of
+// v_mfma_f32_32x32x16_f16 per 128-sample block pair: lane L holds row
+// m = L % 32 and contraction slice hk = L / 32; row m is (block 2p + (m/4)%2,
+// group a = 2 (m/8) + (m%4)/2, real part for even m); contraction index
+// (hk, c) pairs with Re w^c (hk = 0) or Im w^c (hk = 1), so a real row holds
+// (Re x, -Im x) and an imaginary row (Im x, Re x) for x = raw[64 block + 8a + c].
+template <class RAW>
+__device__ __forceinline__ void load_raw(const typename RAW::T *p, size_t e, float &re, float &im);
+template <>
+__device__ __forceinline__ void load_raw<RawC64>(const float2 *p, size_t e, float &re, float &im)
+{
+    const float2 v = p[e];
+    re = v.x;
+    im = v.y;
+}
+template <>
+__device__ __forceinline__ void load_raw<RawF16>(const unsigned int *p, size_t e, float &re,
+                                                 float &im)
+{
+    const unsigned int v = p[e];
+    re = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v & 0xffffU)));
+    im = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v >> 16)));
+}
+template <>
+__device__ __forceinline__ void load_raw<RawBF16>(const unsigned int *p, size_t e, float &re,
+                                                  float &im)
+{
+    const unsigned int v = p[e];
+    re = __uint_as_float(v << 16);
+    im = __uint_as_float(v & 0xffff0000U);
+}
+
+template <class RAW, int PARTS>
+__global__ __launch_bounds__(kBlock) void stage_matrix_kernel(const typename RAW::T *__restrict__ raw,
+                                                              int npri, int ns,
+                                                              uintx4_t *__restrict__ atab,
+                                                              float *__restrict__ inv_scale)
+{
+    __shared__ float s_max[kBlock / 64];
+    const int nbp = matrix_pairs(ns);
+    for (int n = static_cast<int>(blockIdx.x); n < npri; n += static_cast<int>(gridDim.x)) {
+        const size_t row = static_cast<size_t>(n) * ns;
+        float mx = 0.0F;
+        for (int m = static_cast<int>(threadIdx.x); m < ns; m += kBlock) {
+            float re, im;
+            load_raw<RAW>(raw, row + m, re, im);
+            mx = fmaxf(mx, fmaxf(fabsf(re), fabsf(im)));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        mx = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+        __syncthreads();
+        int sh = 0;
+        if (mx > 0.0F) {
+            int e;
+            frexpf(mx, &e);
+            sh = 14 - max(e, -100);
+        }
+        if (threadIdx.x == 0) inv_scale[n] = ldexpf(1.0F, -sh);
+        for (int it = static_cast<int>(threadIdx.x); it < nbp * 64; it += kBlock) {
+            const int bp = it >> 6, L = it & 63;
+            const int m = L & 31, hk = L >> 5;
+            const int blk = 2 * bp + ((m >> 2) & 1);
+            const int a = 2 * (m >> 3) + ((m & 3) >> 1);
+            const bool imag_row = (m & 1) != 0;
+            half8_t hi, lo;
+#pragma unroll
+            for (int c = 0; c < kGroup; ++c) {
+                const int i = blk * kAnchorBlock + kGroup * a + c;
+                float re = 0.0F, im = 0.0F;
+                if (i < ns) load_raw<RAW>(raw, row + i, re, im);
+                re = ldexpf(re, sh);
+                im = ldexpf(im, sh);
+                const float v = imag_row ? (hk ? re : im) : (hk ? -im : re);
+                const _Float16 h = static_cast<_Float16>(v);
+                hi[c] = h;
+                lo[c] = static_cast<_Float16>(v - static_cast<float>(h));
+            }
+            uintx4_t *dst = atab + (static_cast<size_t>(n) * nbp + bp) * PARTS * 64 + L;
+            dst[0] = __builtin_bit_cast(uintx4_t, hi);
+            if (PARTS == 2) dst[64] = __builtin_bit_cast(uintx4_t, lo);
+        }
+    }
+}
+
+template <int PARTS>
+__global__ __launch_bounds__(kBlock) void precision_matrix_kernel(
+    const uintx4_t *__restrict__ atab, const float *__restrict__ inv_scale,
+    const ChirpPos *__restrict__ cpos, const double *__restrict__ grid,
+    const double *__restrict__ kanchor, double dk, Geom g, double2 *__restrict__ out)
+{
+    int tile, split;
+    map_block(g, tile, split);
+    if (tile >= g.ntiles) return;
+    const int n0 = split * g.cps;
+    const int n1 = min(g.npri, n0 + g.cps);
+    matrix_tile<PARTS>(atab, inv_scale, cpos, grid, kanchor, dk, g.npx, matrix_pairs(g.ns), n0, n1,
+                       tile * kBlock, out + static_cast<size_t>(split) * g.npx);
+}
+
 __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const double2 *__restrict__ partial,
                                                               int nsplit, int npx,
                                                               float2 *__restrict__ img)
@@ -508,6 +612,51 @@ hipError_t launch_precision(int ppl, int storage, const void *raw, const ChirpPo
     default:
         return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_stage_matrix(int storage, const void *staged, int npri, int ns, int parts,
+                               void *atab, float *inv_scale, hipStream_t s)
+{
+    if (parts != matrix_parts(storage)) return hipErrorInvalidValue;
+    const int grid = npri < 65535 ? (npri > 0 ? npri : 1) : 65535;
+    auto *t = static_cast<uintx4_t *>(atab);
+    switch (storage) {
+    case 0:
+        hipLaunchKernelGGL((stage_matrix_kernel<RawC64, 2>), dim3(grid), dim3(kBlock), 0, s,
+                           static_cast<const float2 *>(staged), npri, ns, t, inv_scale);
+        break;
+    case 1:
+        hipLaunchKernelGGL((stage_matrix_kernel<RawF16, 1>), dim3(grid), dim3(kBlock), 0, s,
+                           static_cast<const unsigned int *>(staged), npri, ns, t, inv_scale);
+        break;
+    case 2:
+        hipLaunchKernelGGL((stage_matrix_kernel<RawBF16, 1>), dim3(grid), dim3(kBlock), 0, s,
+                           static_cast<const unsigned int *>(staged), npri, ns, t, inv_scale);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_precision_matrix(int parts, const void *atab, const float *inv_scale,
+                                   const ChirpPos *cpos, const double *grid,
+                                   const double *kanchor, double dk, const Geom &g,
+                                   double2 *out, hipStream_t s, int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    const auto *t = static_cast<const uintx4_t *>(atab);
+    if (parts == 2) {
+        hipLaunchKernelGGL(precision_matrix_kernel<2>, dim3(nb), dim3(kBlock), 0, s, t, inv_scale,
+                           cpos, grid, kanchor, dk, g, out);
+    } else if (parts == 1) {
+        hipLaunchKernelGGL(precision_matrix_kernel<1>, dim3(nb), dim3(kBlock), 0, s, t, inv_scale,
+                           cpos, grid, kanchor, dk, g, out);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,

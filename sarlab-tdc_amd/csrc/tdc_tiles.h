@@ -269,4 +269,210 @@ __device__ __forceinline__ void precision_tile(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix engine of the precision form.
+//
+// With i = 8a + c inside a 64-sample block b, w = exp(j 2 pi dk R) and u = w^8,
+//   S_b = sum_i raw[64b + i] w^i = sum_a u^a T_{b,a},
+//   T_{b,a} = sum_c raw[64b + 8a + c] w^c,
+// and T is a matrix product: rows (block, a, re/im) from the chirp's raw
+// samples (the A operand, the same for every pixel), columns pixels, and the
+// contraction over (c, re/im of w^c) (the B operand, per pixel). One
+// v_mfma_f32_32x32x16_f16 computes T for two blocks and 32 pixels. FP16
+// operands are split hi + lo (each value = hi + lo to ~2^-22), so the product
+// keeps FP32-level accuracy: hi*hi + hi*lo (+ lo*hi for c64 raw). The VALU
+// then runs the 8-step nested form over a with u, and rotates each block sum
+// by its FP64 anchor phase, as the one-sample form does.
+//
+// Lane layout (wave64; l = lane, g = l / 32): lane l owns pixel wave_base + l
+// and computes its w^0..w^7; the B operand of pixel half h (pixels
+// wave_base + 32h + (l % 32)) holds Re w^c in lanes g = 0 and Im w^c in lanes
+// g = 1. Accumulator register r of lane l holds T_{2p+g, r/2}, real part for
+// even r, of pixel half h. So lane group g sums the even (g = 0) or odd
+// (g = 1) blocks of both pixel halves; the two groups are added at the end.
+// ---------------------------------------------------------------------------
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int uintx4_t __attribute__((ext_vector_type(4)));
+
+// Lanes 0..31 of the first result hold x of lanes 0..31, lanes 32..63 hold y
+// of lanes 0..31; the second result holds x then y of lanes 32..63.
+__device__ __forceinline__ void swap_halves(unsigned int x, unsigned int y, unsigned int &lo,
+                                            unsigned int &hi)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    lo = r[0];
+    hi = r[1];
+}
+__device__ __forceinline__ void swap_halves(float x, float y, float &lo, float &hi)
+{
+    unsigned int a, b;
+    swap_halves(__float_as_uint(x), __float_as_uint(y), a, b);
+    lo = __uint_as_float(a);
+    hi = __uint_as_float(b);
+}
+__device__ __forceinline__ void swap_halves(double x, double y, double &lo, double &hi)
+{
+    const uint2 xa = __builtin_bit_cast(uint2, x), ya = __builtin_bit_cast(uint2, y);
+    uint2 l, h;
+    swap_halves(xa.x, ya.x, l.x, h.x);
+    swap_halves(xa.y, ya.y, l.y, h.y);
+    lo = __builtin_bit_cast(double, l);
+    hi = __builtin_bit_cast(double, h);
+}
+__device__ __forceinline__ void swap_halves(half8_t x, half8_t y, half8_t &lo, half8_t &hi)
+{
+    const uintx4_t xa = __builtin_bit_cast(uintx4_t, x), ya = __builtin_bit_cast(uintx4_t, y);
+    uintx4_t l, h;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        unsigned int a, b;
+        swap_halves(xa[k], ya[k], a, b);
+        l[k] = a;
+        h[k] = b;
+    }
+    lo = __builtin_bit_cast(half8_t, l);
+    hi = __builtin_bit_cast(half8_t, h);
+}
+
+__device__ __forceinline__ void cmul(float ar, float ai, float br, float bi, float &cr, float &ci)
+{
+    cr = fmaf(ar, br, -(ai * bi));
+    ci = fmaf(ar, bi, ai * br);
+}
+
+// Accumulate the 8 group sums of one block with the nested form in u and
+// rotate the block sum by the anchor exp(j 2 pi theta).
+__device__ __forceinline__ void matrix_block(const floatx16_t &d, float ur, float ui, float theta,
+                                             float &chr, float &chi)
+{
+    float sr = d[14], si = d[15];
+#pragma unroll
+    for (int a = 6; a >= 0; --a) {
+        const float nr = fmaf(sr, ur, fmaf(-si, ui, d[2 * a]));
+        const float ni = fmaf(sr, ui, fmaf(si, ur, d[2 * a + 1]));
+        sr = nr;
+        si = ni;
+    }
+    const float zr = cos_rev(theta), zi = sin_rev(theta);
+    chr += fmaf(zr, sr, -(zi * si));
+    chi += fmaf(zr, si, zi * sr);
+}
+
+// One tile: kBlock consecutive pixels from pix0 (one per thread) times the
+// chirp range [n0, n1). atab holds PARTS A tiles per (chirp, block pair),
+// each 64 lanes x 16 bytes; inv_scale undoes the per-chirp power-of-two scale
+// of the A tiles; kanchor has 2 entries per block pair.
+template <int PARTS>
+__device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
+                                            const float *__restrict__ inv_scale,
+                                            const ChirpPos *__restrict__ cpos,
+                                            const double *__restrict__ grid,
+                                            const double *__restrict__ kanchor, double dk, int npx,
+                                            int nbp, int n0, int n1, int pix0,
+                                            double2 *__restrict__ out)
+{
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const bool upper = lane >= 32;
+    const int pix = pix0 + static_cast<int>(threadIdx.x);
+    const int q = min(pix, npx - 1);
+    const double px = grid[3 * q], py = grid[3 * q + 1], pz = grid[3 * q + 2];
+    const double dk8 = 8.0 * dk;
+    float accr[2] = {0.0F, 0.0F}, acci[2] = {0.0F, 0.0F};
+    float cmpr[2] = {0.0F, 0.0F}, cmpi[2] = {0.0F, 0.0F};
+    const cptr<double> ka = as_const(kanchor);
+
+    for (int n = n0; n < n1; ++n) {
+        const ChirpPos c = as_const(cpos)[n];
+        const double dx = c.x - px, dy = c.y - py, dz = c.z - pz;
+        const double rng = range_from_sq(fma(dz, dz, fma(dy, dy, dx * dx)));
+        double t = dk * rng;
+        t -= floor(t);
+        float wr, wi;
+        sincos_rev(static_cast<float>(t), wi, wr);
+        t = dk8 * rng;
+        t -= floor(t);
+        float ur0, ui0;
+        sincos_rev(static_cast<float>(t), ui0, ur0);
+
+        // w^c, c = 0..7, split into FP16 hi + lo
+        float pr[kGroup], pi[kGroup];
+        pr[0] = 1.0F;
+        pi[0] = 0.0F;
+        pr[1] = wr;
+        pi[1] = wi;
+        cmul(wr, wi, wr, wi, pr[2], pi[2]);
+        cmul(pr[2], pi[2], wr, wi, pr[3], pi[3]);
+        cmul(pr[2], pi[2], pr[2], pi[2], pr[4], pi[4]);
+        cmul(pr[4], pi[4], wr, wi, pr[5], pi[5]);
+        cmul(pr[3], pi[3], pr[3], pi[3], pr[6], pi[6]);
+        cmul(pr[6], pi[6], wr, wi, pr[7], pi[7]);
+        half8_t rh, ih, rl, il;
+#pragma unroll
+        for (int k = 0; k < kGroup; ++k) {
+            const _Float16 a = static_cast<_Float16>(pr[k]);
+            const _Float16 b = static_cast<_Float16>(pi[k]);
+            rh[k] = a;
+            ih[k] = b;
+            rl[k] = static_cast<_Float16>(pr[k] - static_cast<float>(a));
+            il[k] = static_cast<_Float16>(pi[k] - static_cast<float>(b));
+        }
+        half8_t b0h, b1h, b0l, b1l;
+        swap_halves(rh, ih, b0h, b1h);
+        swap_halves(rl, il, b0l, b1l);
+        float ur[2], ui[2];
+        double rh2[2];
+        swap_halves(ur0, ur0, ur[0], ur[1]);
+        swap_halves(ui0, ui0, ui[0], ui[1]);
+        swap_halves(rng, rng, rh2[0], rh2[1]);
+
+        float chr[2] = {0.0F, 0.0F}, chi[2] = {0.0F, 0.0F};
+        const uintx4_t *tp = atab + static_cast<size_t>(n) * nbp * PARTS * 64 + lane;
+        for (int p = 0; p < nbp; ++p) {
+            const half8_t ah = __builtin_bit_cast(half8_t, tp[0]);
+            half8_t al;
+            if (PARTS == 2) al = __builtin_bit_cast(half8_t, tp[64]);
+            tp += PARTS * 64;
+            floatx16_t d0 = {}, d1 = {};
+            d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b0h, d0, 0, 0, 0);
+            d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b1h, d1, 0, 0, 0);
+            d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b0l, d0, 0, 0, 0);
+            d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b1l, d1, 0, 0, 0);
+            if (PARTS == 2) {
+                d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b0h, d0, 0, 0, 0);
+                d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b1h, d1, 0, 0, 0);
+            }
+            // anchor wavenumber of this lane group's block
+            const double ke = ka[2 * p], ko = ka[2 * p + 1];
+            const double kb = upper ? ko : ke;
+            double th0 = kb * rh2[0], th1 = kb * rh2[1];
+            th0 -= floor(th0);
+            th1 -= floor(th1);
+            matrix_block(d0, ur[0], ui[0], static_cast<float>(th0), chr[0], chi[0]);
+            matrix_block(d1, ur[1], ui[1], static_cast<float>(th1), chr[1], chi[1]);
+        }
+        const float is = as_const(inv_scale)[n];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            two_sum_acc(accr[h], cmpr[h], chr[h] * is);
+            two_sum_acc(acci[h], cmpi[h], chi[h] * is);
+        }
+    }
+
+    // even-block and odd-block sums of the lane's own pixel
+    float ar0, ar1, ai0, ai1, cr0, cr1, ci0, ci1;
+    swap_halves(accr[0], accr[1], ar0, ar1);
+    swap_halves(acci[0], acci[1], ai0, ai1);
+    swap_halves(cmpr[0], cmpr[1], cr0, cr1);
+    swap_halves(cmpi[0], cmpi[1], ci0, ci1);
+    if (pix < npx) {
+        const double re = (static_cast<double>(ar0) + static_cast<double>(cr0)) +
+                          (static_cast<double>(ar1) + static_cast<double>(cr1));
+        const double im = (static_cast<double>(ai0) + static_cast<double>(ci0)) +
+                          (static_cast<double>(ai1) + static_cast<double>(ci1));
+        out[pix] = make_double2(re, im);
+    }
+}
+
 } // namespace tdc

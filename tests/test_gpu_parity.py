@@ -22,7 +22,7 @@ PREC_TOL = 1e-4
 
 
 def form(tdc, d=None, f64=None, center=0.0, bw=1e30, centroid=False, rw=False, taper=False,
-         kind="plain", storage="c64", ppl=0, splits=0):
+         kind="plain", storage="c64", ppl=0, splits=0, engine="auto"):
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
     if f64 is None:
@@ -36,6 +36,7 @@ def form(tdc, d=None, f64=None, center=0.0, bw=1e30, centroid=False, rw=False, t
     p.setForm(kind)
     p.setRawStorage(storage)
     p.setLaunch(ppl, splits)
+    p.setEngine(engine)
     p.start(center, bw, centroid, rw, taper)
     return p.getFocusedImage(), p.stats()
 
@@ -239,6 +240,47 @@ def test_precision_ragged(tdc, orc, ppl, splits):
     s = ragged_scene()
     img, _ = form(tdc, f64=s.f64(), kind="precision", ppl=ppl, splits=splits)
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
+@pytest.mark.parametrize("engine", ["vector", "matrix"])
+@pytest.mark.parametrize("storage,tol", [("c64", PREC_TOL), ("f16", 3e-3), ("bf16", 2e-2)])
+def test_precision_engines_ragged(tdc, orc, engine, storage, tol):
+    s = ragged_scene()
+    img, st = form(tdc, f64=s.f64(), kind="precision", storage=storage, engine=engine)
+    assert st["engine"] == engine and st["raw_storage"] == storage
+    check(orc, img, oracle_f64(orc, s), tol)
+
+
+@pytest.mark.parametrize("ns", [1, 7, 64, 65, 127, 128, 129, 300, 512])
+def test_matrix_engine_sample_counts(tdc, orc, ns):
+    """Block-pair padding: 1..4 pairs, empty odd blocks, one-sample chirps."""
+    s = ragged_scene(npri=40, ns=ns, rows=9, cols=11, seed=ns)
+    img, st = form(tdc, f64=s.f64(), kind="precision", engine="matrix")
+    assert st["engine"] == "matrix"
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
+@pytest.mark.parametrize("gain_exp", [-30, 0, 30])
+def test_matrix_engine_per_chirp_scale(tdc, orc, gain_exp):
+    """Chirp amplitudes spread over 1e-2..1e2 around 10**gain_exp: each chirp
+    gets its own power-of-two FP16 scale."""
+    s = ragged_scene(npri=61, ns=200, rows=7, cols=9, seed=11)
+    rng = np.random.default_rng(1)
+    gain = 10.0 ** (gain_exp + rng.uniform(-2, 2, s.n_chirps))
+    s.raw = (s.raw * gain[:, None]).astype(np.complex64)
+    img, _ = form(tdc, f64=s.f64(), kind="precision", engine="matrix")
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
+def test_engines_agree(tdc, orc):
+    from sarlabtdc import scene
+
+    s = scene.line_jitter()
+    a, _ = form(tdc, f64=s.f64(), kind="precision", engine="vector")
+    b, _ = form(tdc, f64=s.f64(), kind="precision", engine="matrix")
+    e = orc.rel_l2(b, a)
+    print(f"matrix vs vector rel-L2 {e:.3e}")
+    assert e < 1e-5
 
 
 def test_precision_range_window(tdc, orc):
