@@ -492,7 +492,7 @@ bool precision_tables(const double *t, int ns, double f0, double krate,
     }
     // one entry per anchor block, padded to whole block pairs; a padding
     // block holds no samples and takes its time from the uniform spacing
-    const int nblk = 2 * tdc::matrix_pairs(ns);
+    const int nblk = tdc::anchor_entries(ns);
     kanchor.resize(nblk);
     for (int b = 0; b < nblk; ++b) {
         const int m = b * kAnchorBlock;

@@ -16,11 +16,14 @@ constexpr int kChunk = 128;
 constexpr int kBlock = 256;
 // Samples per FP64 re-anchor block of the precision form.
 constexpr int kAnchorBlock = 64;
-// Matrix engine of the precision form: samples per 8-sample group (one
-// complex k-slice of the MFMA) and per block pair (one 32x32x16 A tile: two
-// anchor blocks of eight groups, real and imaginary parts).
-constexpr int kGroup = 8;
-constexpr int kPairSamples = 2 * kAnchorBlock;
+// Matrix engine of the precision form: complex samples per contraction group
+// (two k-slices of the 32x32x16 FP16 MFMA, 8 complex samples each), samples
+// per anchor block, and samples per set of A tiles (two anchor blocks: the
+// rows of one MFMA output).
+constexpr int kGroup = 16;
+constexpr int kSlice = 8;
+constexpr int kMatrixBlock = 128;
+constexpr int kQuadSamples = 2 * kMatrixBlock;
 // Bytes of one A tile: 64 lanes x 8 FP16 values.
 constexpr int kTileBytes = 64 * 16;
 
@@ -102,12 +105,15 @@ hipError_t launch_precision_matrix(int parts, const void *atab, const float *inv
                                    const ChirpPos *cpos, const double *grid,
                                    const double *kanchor, double dk, const Geom &g,
                                    double2 *out, hipStream_t s, int *workgroups);
-__host__ __device__ inline int matrix_pairs(int ns) { return (ns + kPairSamples - 1) / kPairSamples; }
+__host__ __device__ inline int matrix_quads(int ns) { return (ns + kQuadSamples - 1) / kQuadSamples; }
 inline int matrix_parts(int storage) { return storage == 0 ? 2 : 1; }
 inline size_t matrix_table_bytes(int npri, int ns, int storage)
 {
-    return static_cast<size_t>(npri) * matrix_pairs(ns) * matrix_parts(storage) * kTileBytes;
+    return static_cast<size_t>(npri) * matrix_quads(ns) * 2 * matrix_parts(storage) * kTileBytes;
 }
+// Entries of the anchor table (one per 64 samples, padded to whole A tile
+// sets so that the matrix engine's last anchor exists).
+inline int anchor_entries(int ns) { return 4 * matrix_quads(ns); }
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
 

@@ -356,11 +356,12 @@ __global__ __launch_bounds__(kBlock) void precision_kernel(
 // Matrix engine staging: one workgroup per chirp. The chirp's samples (in
 // their storage format) are scaled by a power of two that puts the largest
 // component in [2^13, 2^14) and laid out as the A operand of
-// v_mfma_f32_32x32x16_f16 per 128-sample block pair: lane L holds row
-// m = L % 32 and contraction slice hk = L / 32; row m is (block 2p + (m/4)%2,
-// group a = 2 (m/8) + (m%4)/2, real part for even m); contraction index
-// (hk, c) pairs with Re w^c (hk = 0) or Im w^c (hk = 1), so a real row holds
-// (Re x, -Im x) and an imaginary row (Im x, Re x) for x = raw[64 block + 8a + c].
+// v_mfma_f32_32x32x16_f16 per 256-sample tile set q and k-slice s: lane L
+// holds row m = L % 32 and contraction half hk = L / 32; row m is (block
+// 2q + (m/4)%2 of 128 samples, group a = 2 (m/8) + (m%4)/2 of 16 samples,
+// real part for even m); element j of the lane pairs with w^c, c = 8s + j,
+// Re w^c for hk = 0 and Im w^c for hk = 1, so a real row holds (Re x, -Im x)
+// and an imaginary row (Im x, Re x) for x = raw[128 block + 16a + c].
 template <class RAW>
 __device__ __forceinline__ void load_raw(const typename RAW::T *p, size_t e, float &re, float &im);
 template <>
@@ -394,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void stage_matrix_kernel(const typename RAW
                                                               float *__restrict__ inv_scale)
 {
     __shared__ float s_max[kBlock / 64];
-    const int nbp = matrix_pairs(ns);
+    const int nq = matrix_quads(ns);
     for (int n = static_cast<int>(blockIdx.x); n < npri; n += static_cast<int>(gridDim.x)) {
         const size_t row = static_cast<size_t>(n) * ns;
         float mx = 0.0F;
@@ -416,26 +417,28 @@ __global__ __launch_bounds__(kBlock) void stage_matrix_kernel(const typename RAW
             sh = 14 - max(e, -100);
         }
         if (threadIdx.x == 0) inv_scale[n] = ldexpf(1.0F, -sh);
-        for (int it = static_cast<int>(threadIdx.x); it < nbp * 64; it += kBlock) {
-            const int bp = it >> 6, L = it & 63;
+        // one item = (tile set, k-slice, lane)
+        for (int it = static_cast<int>(threadIdx.x); it < nq * 2 * 64; it += kBlock) {
+            const int qs = it >> 6, L = it & 63;
+            const int q = qs >> 1, sl = qs & 1;
             const int m = L & 31, hk = L >> 5;
-            const int blk = 2 * bp + ((m >> 2) & 1);
+            const int blk = 2 * q + ((m >> 2) & 1);
             const int a = 2 * (m >> 3) + ((m & 3) >> 1);
             const bool imag_row = (m & 1) != 0;
             half8_t hi, lo;
 #pragma unroll
-            for (int c = 0; c < kGroup; ++c) {
-                const int i = blk * kAnchorBlock + kGroup * a + c;
+            for (int j = 0; j < kSlice; ++j) {
+                const int i = blk * kMatrixBlock + kGroup * a + kSlice * sl + j;
                 float re = 0.0F, im = 0.0F;
                 if (i < ns) load_raw<RAW>(raw, row + i, re, im);
                 re = ldexpf(re, sh);
                 im = ldexpf(im, sh);
                 const float v = imag_row ? (hk ? re : im) : (hk ? -im : re);
                 const _Float16 h = static_cast<_Float16>(v);
-                hi[c] = h;
-                lo[c] = static_cast<_Float16>(v - static_cast<float>(h));
+                hi[j] = h;
+                lo[j] = static_cast<_Float16>(v - static_cast<float>(h));
             }
-            uintx4_t *dst = atab + (static_cast<size_t>(n) * nbp + bp) * PARTS * 64 + L;
+            uintx4_t *dst = atab + ((static_cast<size_t>(n) * nq + q) * 2 * PARTS + sl * PARTS) * 64 + L;
             dst[0] = __builtin_bit_cast(uintx4_t, hi);
             if (PARTS == 2) dst[64] = __builtin_bit_cast(uintx4_t, lo);
         }
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void precision_matrix_kernel(
     if (tile >= g.ntiles) return;
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
-    matrix_tile<PARTS>(atab, inv_scale, cpos, grid, kanchor, dk, g.npx, matrix_pairs(g.ns), n0, n1,
+    matrix_tile<PARTS>(atab, inv_scale, cpos, grid, kanchor, dk, g.npx, matrix_quads(g.ns), n0, n1,
                        tile * kBlock, out + static_cast<size_t>(split) * g.npx);
 }
 

@@ -205,7 +205,7 @@ void plan(Job &j)
     j.ntiles = static_cast<int64_t>(j.npt) * j.nsplit;
     j.nfin = (j.npx + tdc::kBlock - 1) / tdc::kBlock;
     j.terms = static_cast<double>(npx) * d.n_pri * d.n_samples;
-    const int nblk = 2 * tdc::matrix_pairs(d.n_samples); // anchor table entries
+    const int nblk = tdc::anchor_entries(d.n_samples);
     size_t o = align256(8 * nterm); // raw samples at offset 0
     j.off_cpos = o;
     o += align256(sizeof(tdc::ChirpPos) * d.n_pri);

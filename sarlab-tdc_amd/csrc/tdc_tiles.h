@@ -272,24 +272,27 @@ __device__ __forceinline__ void precision_tile(
 // ---------------------------------------------------------------------------
 // Matrix engine of the precision form.
 //
-// With i = 8a + c inside a 64-sample block b, w = exp(j 2 pi dk R) and u = w^8,
-//   S_b = sum_i raw[64b + i] w^i = sum_a u^a T_{b,a},
-//   T_{b,a} = sum_c raw[64b + 8a + c] w^c,
+// With i = 16a + c inside a 128-sample block b, w = exp(j 2 pi dk R) and
+// u = w^16,
+//   S_b = sum_i raw[128b + i] w^i = sum_a u^a T_{b,a},
+//   T_{b,a} = sum_{c<16} raw[128b + 16a + c] w^c,
 // and T is a matrix product: rows (block, a, re/im) from the chirp's raw
 // samples (the A operand, the same for every pixel), columns pixels, and the
-// contraction over (c, re/im of w^c) (the B operand, per pixel). One
-// v_mfma_f32_32x32x16_f16 computes T for two blocks and 32 pixels. FP16
-// operands are split hi + lo (each value = hi + lo to ~2^-22), so the product
-// keeps FP32-level accuracy: hi*hi + hi*lo (+ lo*hi for c64 raw). The VALU
-// then runs the 8-step nested form over a with u, and rotates each block sum
-// by its FP64 anchor phase, as the one-sample form does.
+// contraction over (c, re/im of w^c) (the B operand, per pixel). Two chained
+// v_mfma_f32_32x32x16_f16 (k-slices c < 8 and c >= 8) compute T for two
+// blocks (256 samples) and 32 pixels. FP16 operands are split hi + lo (each
+// value = hi + lo to ~2^-22), so the product keeps FP32-level accuracy:
+// hi*hi + hi*lo (+ lo*hi for c64 raw). The VALU then runs the 8-step nested
+// form over a with u, and rotates each block sum by its FP64 anchor phase, as
+// the one-sample form does every 64 samples.
 //
 // Lane layout (wave64; l = lane, g = l / 32): lane l owns pixel wave_base + l
-// and computes its w^0..w^7; the B operand of pixel half h (pixels
-// wave_base + 32h + (l % 32)) holds Re w^c in lanes g = 0 and Im w^c in lanes
-// g = 1. Accumulator register r of lane l holds T_{2p+g, r/2}, real part for
-// even r, of pixel half h. So lane group g sums the even (g = 0) or odd
-// (g = 1) blocks of both pixel halves; the two groups are added at the end.
+// and computes its w^0..w^15; the B operand of pixel half h (pixels
+// wave_base + 32h + (l % 32)) and k-slice s holds Re w^(8s+j) in lanes g = 0
+// and Im w^(8s+j) in lanes g = 1. Accumulator register r of lane l holds
+// T_{2q+g, r/2} of pixel half h, real part for even r, q the tile set. So lane
+// group g sums the even (g = 0) or odd (g = 1) blocks of both pixel halves;
+// the two groups are added at the end.
 // ---------------------------------------------------------------------------
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
@@ -336,6 +339,21 @@ __device__ __forceinline__ void swap_halves(half8_t x, half8_t y, half8_t &lo, h
     hi = __builtin_bit_cast(half8_t, h);
 }
 
+// FP16 hi and lo parts of two FP32 values, each packed as (first, second):
+// hi = f16(v), lo = f16(v - hi), both rounded to nearest even; the
+// subtraction is exact in FP32 and v_fma_mix rounds it to FP16 once.
+__device__ __forceinline__ void split_f16x2(float v0, float v1, unsigned int &hi, unsigned int &lo)
+{
+    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(hi) : "v"(v0), "v"(v1));
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=&v"(lo) : "v"(hi), "v"(v0));
+    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "+v"(lo)
+        : "v"(hi), "v"(v1));
+}
+
+// x - floor(x) in FP64 (one instruction)
+__device__ __forceinline__ double fract64(double x) { return __builtin_amdgcn_fract(x); }
+
 __device__ __forceinline__ void cmul(float ar, float ai, float br, float bi, float &cr, float &ci)
 {
     cr = fmaf(ar, br, -(ai * bi));
@@ -356,71 +374,89 @@ __device__ __forceinline__ void matrix_block(const floatx16_t &d, float ur, floa
         si = ni;
     }
     const float zr = cos_rev(theta), zi = sin_rev(theta);
-    chr += fmaf(zr, sr, -(zi * si));
-    chi += fmaf(zr, si, zi * sr);
+    chr = fmaf(zr, sr, fmaf(-zi, si, chr));
+    chi = fmaf(zr, si, fmaf(zi, sr, chi));
+}
+
+__device__ __forceinline__ floatx16_t mfma16(const uintx4_t &a, const half8_t &b,
+                                             const floatx16_t &c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8_t, a), b, c, 0, 0, 0);
 }
 
 // One tile: kBlock consecutive pixels from pix0 (one per thread) times the
-// chirp range [n0, n1). atab holds PARTS A tiles per (chirp, block pair),
-// each 64 lanes x 16 bytes; inv_scale undoes the per-chirp power-of-two scale
-// of the A tiles; kanchor has 2 entries per block pair.
+// chirp range [n0, n1). atab holds, per (chirp, tile set), the A tiles of
+// k-slices 0 and 1, each PARTS parts of 64 lanes x 16 bytes; inv_scale undoes
+// the per-chirp power-of-two scale of the A tiles; kanchor holds one
+// wavenumber per 64 samples.
 template <int PARTS>
 __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
                                             const float *__restrict__ inv_scale,
                                             const ChirpPos *__restrict__ cpos,
                                             const double *__restrict__ grid,
                                             const double *__restrict__ kanchor, double dk, int npx,
-                                            int nbp, int n0, int n1, int pix0,
+                                            int nq, int n0, int n1, int pix0,
                                             double2 *__restrict__ out)
 {
+    constexpr int kTiles = 2 * PARTS; // A tiles per tile set
     const int lane = static_cast<int>(threadIdx.x) & 63;
     const bool upper = lane >= 32;
     const int pix = pix0 + static_cast<int>(threadIdx.x);
     const int q = min(pix, npx - 1);
     const double px = grid[3 * q], py = grid[3 * q + 1], pz = grid[3 * q + 2];
-    const double dk8 = 8.0 * dk;
-    float accr[2] = {0.0F, 0.0F}, acci[2] = {0.0F, 0.0F};
-    float cmpr[2] = {0.0F, 0.0F}, cmpi[2] = {0.0F, 0.0F};
+    const double dk16 = 16.0 * dk;
+    double accr[2] = {0.0, 0.0}, acci[2] = {0.0, 0.0};
     const cptr<double> ka = as_const(kanchor);
+    const uintx4_t *tp = atab + static_cast<size_t>(max(n0, 0)) * nq * kTiles * 64 + lane;
+    uintx4_t nxt[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) nxt[t] = uintx4_t{0U, 0U, 0U, 0U};
+    if (n0 < n1) {
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) nxt[t] = tp[64 * t];
+    }
 
     for (int n = n0; n < n1; ++n) {
         const ChirpPos c = as_const(cpos)[n];
         const double dx = c.x - px, dy = c.y - py, dz = c.z - pz;
         const double rng = range_from_sq(fma(dz, dz, fma(dy, dy, dx * dx)));
-        double t = dk * rng;
-        t -= floor(t);
         float wr, wi;
-        sincos_rev(static_cast<float>(t), wi, wr);
-        t = dk8 * rng;
-        t -= floor(t);
+        sincos_rev(static_cast<float>(fract64(dk * rng)), wi, wr);
         float ur0, ui0;
-        sincos_rev(static_cast<float>(t), ui0, ur0);
+        sincos_rev(static_cast<float>(fract64(dk16 * rng)), ui0, ur0);
 
-        // w^c, c = 0..7, split into FP16 hi + lo
+        // w^c, c = 0..15, split into FP16 hi + lo
         float pr[kGroup], pi[kGroup];
         pr[0] = 1.0F;
         pi[0] = 0.0F;
         pr[1] = wr;
         pi[1] = wi;
-        cmul(wr, wi, wr, wi, pr[2], pi[2]);
-        cmul(pr[2], pi[2], wr, wi, pr[3], pi[3]);
-        cmul(pr[2], pi[2], pr[2], pi[2], pr[4], pi[4]);
-        cmul(pr[4], pi[4], wr, wi, pr[5], pi[5]);
-        cmul(pr[3], pi[3], pr[3], pi[3], pr[6], pi[6]);
-        cmul(pr[6], pi[6], wr, wi, pr[7], pi[7]);
-        half8_t rh, ih, rl, il;
 #pragma unroll
-        for (int k = 0; k < kGroup; ++k) {
-            const _Float16 a = static_cast<_Float16>(pr[k]);
-            const _Float16 b = static_cast<_Float16>(pi[k]);
-            rh[k] = a;
-            ih[k] = b;
-            rl[k] = static_cast<_Float16>(pr[k] - static_cast<float>(a));
-            il[k] = static_cast<_Float16>(pi[k] - static_cast<float>(b));
+        for (int k = 2; k < kGroup; ++k) {
+            // w^k from two lower powers, each of at most k/2 + 1 factors
+            const int a = k / 2, b = k - k / 2;
+            cmul(pr[a], pi[a], pr[b], pi[b], pr[k], pi[k]);
         }
-        half8_t b0h, b1h, b0l, b1l;
-        swap_halves(rh, ih, b0h, b1h);
-        swap_halves(rl, il, b0l, b1l);
+        half8_t bh[2][2], bl[2][2]; // [k-slice][pixel half]
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            uintx4_t rh, ih, rl, il;
+#pragma unroll
+            for (int k = 0; k < kSlice / 2; ++k) {
+                const int e = kSlice * s + 2 * k;
+                unsigned int h, l;
+                split_f16x2(pr[e], pr[e + 1], h, l);
+                rh[k] = h;
+                rl[k] = l;
+                split_f16x2(pi[e], pi[e + 1], h, l);
+                ih[k] = h;
+                il[k] = l;
+            }
+            swap_halves(__builtin_bit_cast(half8_t, rh), __builtin_bit_cast(half8_t, ih),
+                        bh[s][0], bh[s][1]);
+            swap_halves(__builtin_bit_cast(half8_t, rl), __builtin_bit_cast(half8_t, il),
+                        bl[s][0], bl[s][1]);
+        }
         float ur[2], ui[2];
         double rh2[2];
         swap_halves(ur0, ur0, ur[0], ur[1]);
@@ -428,51 +464,49 @@ __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
         swap_halves(rng, rng, rh2[0], rh2[1]);
 
         float chr[2] = {0.0F, 0.0F}, chi[2] = {0.0F, 0.0F};
-        const uintx4_t *tp = atab + static_cast<size_t>(n) * nbp * PARTS * 64 + lane;
-        for (int p = 0; p < nbp; ++p) {
-            const half8_t ah = __builtin_bit_cast(half8_t, tp[0]);
-            half8_t al;
-            if (PARTS == 2) al = __builtin_bit_cast(half8_t, tp[64]);
-            tp += PARTS * 64;
+        for (int p = 0; p < nq; ++p) {
+            uintx4_t cur[kTiles];
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) cur[t] = nxt[t];
+            // prefetch the next (chirp, tile set) of the tile; the tables are
+            // contiguous over chirps, the last one reloads itself
+            if (n + 1 < n1 || p + 1 < nq) tp += kTiles * 64;
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) nxt[t] = tp[64 * t];
             floatx16_t d0 = {}, d1 = {};
-            d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b0h, d0, 0, 0, 0);
-            d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b1h, d1, 0, 0, 0);
-            d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b0l, d0, 0, 0, 0);
-            d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b1l, d1, 0, 0, 0);
-            if (PARTS == 2) {
-                d0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b0h, d0, 0, 0, 0);
-                d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b1h, d1, 0, 0, 0);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const uintx4_t &ah = cur[s * PARTS];
+                d0 = mfma16(ah, bh[s][0], d0);
+                d1 = mfma16(ah, bh[s][1], d1);
+                d0 = mfma16(ah, bl[s][0], d0);
+                d1 = mfma16(ah, bl[s][1], d1);
+                if (PARTS == 2) {
+                    const uintx4_t &al = cur[s * PARTS + 1];
+                    d0 = mfma16(al, bh[s][0], d0);
+                    d1 = mfma16(al, bh[s][1], d1);
+                }
             }
-            // anchor wavenumber of this lane group's block
-            const double ke = ka[2 * p], ko = ka[2 * p + 1];
+            // anchor wavenumber of this lane group's block (128 samples = two
+            // entries of the 64-sample table)
+            const double ke = ka[4 * p], ko = ka[4 * p + 2];
             const double kb = upper ? ko : ke;
-            double th0 = kb * rh2[0], th1 = kb * rh2[1];
-            th0 -= floor(th0);
-            th1 -= floor(th1);
-            matrix_block(d0, ur[0], ui[0], static_cast<float>(th0), chr[0], chi[0]);
-            matrix_block(d1, ur[1], ui[1], static_cast<float>(th1), chr[1], chi[1]);
+            matrix_block(d0, ur[0], ui[0], static_cast<float>(fract64(kb * rh2[0])), chr[0], chi[0]);
+            matrix_block(d1, ur[1], ui[1], static_cast<float>(fract64(kb * rh2[1])), chr[1], chi[1]);
         }
         const float is = as_const(inv_scale)[n];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            two_sum_acc(accr[h], cmpr[h], chr[h] * is);
-            two_sum_acc(acci[h], cmpi[h], chi[h] * is);
+            accr[h] += static_cast<double>(chr[h] * is);
+            acci[h] += static_cast<double>(chi[h] * is);
         }
     }
 
     // even-block and odd-block sums of the lane's own pixel
-    float ar0, ar1, ai0, ai1, cr0, cr1, ci0, ci1;
+    double ar0, ar1, ai0, ai1;
     swap_halves(accr[0], accr[1], ar0, ar1);
     swap_halves(acci[0], acci[1], ai0, ai1);
-    swap_halves(cmpr[0], cmpr[1], cr0, cr1);
-    swap_halves(cmpi[0], cmpi[1], ci0, ci1);
-    if (pix < npx) {
-        const double re = (static_cast<double>(ar0) + static_cast<double>(cr0)) +
-                          (static_cast<double>(ar1) + static_cast<double>(cr1));
-        const double im = (static_cast<double>(ai0) + static_cast<double>(ci0)) +
-                          (static_cast<double>(ai1) + static_cast<double>(ci1));
-        out[pix] = make_double2(re, im);
-    }
+    if (pix < npx) out[pix] = make_double2(ar0 + ar1, ai0 + ai1);
 }
 
 } // namespace tdc
