@@ -40,14 +40,20 @@ METRIC = ("G pixel·chirp·samples/s at rel-L2≤1e-4 vs FP64 (plain vs fast); "
 UNIT = "G pixel·chirp·samples/s"
 PEAK_FP32_TFLOPS = 157.3        # MI355X published FP32 vector peak (packed)
 PEAK_FP32_NONPACKED = 78.6      # without packed FP32 issue
-FLOP_PER_TERM = {"plain": 20.0, "precision": 8.0}
+PEAK_FP16_MFMA_TFLOPS = 2500.0  # MI355X published dense FP16 matrix peak
+# algorithmic FP32 flops per term: plain = 3 sub + 5 norm^2 + sqrt + scale +
+# sin + cos + 8 (complex MAC); precision = 6 (complex rotation) + 8 (MAC)
+FLOP_PER_TERM = {"plain": 20.0, "precision": 14.0}
+# FP16 MFMA flops the matrix engine executes per term: 8 per product (one
+# complex MAC as a real 2x2 block) times hi*hi + hi*lo (+ lo*hi for c64 raw)
+MFMA_FLOP_PER_TERM = {"c64": 24.0, "f16": 16.0, "bf16": 16.0}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_proc(tdc, s, kind, storage="c64"):
+def make_proc(tdc, s, kind, storage="c64", engine="auto"):
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
     if kind == "precision":
@@ -56,6 +62,7 @@ def make_proc(tdc, s, kind, storage="c64"):
                         f["modRate"], f["startFreq"])
         p.setFocusGridF64(f["focusGrid"])
         p.setRawStorage(storage)
+        p.setEngine(engine)
         keep = f
     else:
         d = s.dropin()
@@ -66,8 +73,8 @@ def make_proc(tdc, s, kind, storage="c64"):
     return p, keep
 
 
-def time_variant(tdc, s, kind, storage, steps, warmup):
-    p, keep = make_proc(tdc, s, kind, storage)
+def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto"):
+    p, keep = make_proc(tdc, s, kind, storage, engine)
     # end-to-end start(): host arrays -> H2D -> compute -> D2H (PCIe inclusive)
     p.start(0.0, 1e30, False, False, False)
     e2e_ms = p.stats()["end_to_end_ms"]
@@ -83,7 +90,7 @@ def time_variant(tdc, s, kind, storage, steps, warmup):
     ms_step = (t1 - t0) * 1e3 / steps
     return dict(kind=kind, storage=storage, ms_per_step=ms_step, main_kernel_ms=st["main_kernel_ms"],
                 kernel_ms=st["kernel_ms"], e2e_ms=e2e_ms, terms=st["terms"],
-                splits=st["chirp_splits"], ppl=st["pixels_per_lane"],
+                splits=st["chirp_splits"], ppl=st["pixels_per_lane"], engine=st["engine"],
                 workgroups=st["workgroups"], image=img, keep=keep)
 
 
@@ -184,8 +191,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="arc", choices=["line_jitter", "arc", "helix"])
-    ap.add_argument("--variants", default="c64,f16,bf16,plain",
-                    help="comma list of c64, f16, bf16 (precision form) and plain")
+    ap.add_argument("--variants", default="c64,f16,bf16,vector,plain",
+                    help="comma list of c64, f16, bf16 (precision form, matrix engine), vector "
+                         "(precision form, vector engine, c64) and plain")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -229,15 +237,16 @@ def main():
     results = {}
     for v in [x.strip() for x in args.variants.split(",") if x.strip()]:
         kind = "plain" if v == "plain" else "precision"
-        storage = "c64" if v == "plain" else v
-        r = time_variant(tdc, s, kind, storage, args.steps, args.warmup)
+        storage = "c64" if v in ("plain", "vector") else v
+        engine = "vector" if v == "vector" else "auto"
+        r = time_variant(tdc, s, kind, storage, args.steps, args.warmup, engine)
         r["gterms"] = r["terms"] / (r["ms_per_step"] * 1e-3) / 1e9
         r["gterms_kernel"] = r["terms"] / (r["main_kernel_ms"] * 1e-3) / 1e9
         r["gterms_e2e"] = r["terms"] / (r["e2e_ms"] * 1e-3) / 1e9
         results[v] = r
         log(f"{v:5s}: {r['ms_per_step']:.3f} ms/step, kernel {r['main_kernel_ms']:.3f} ms, "
             f"{r['gterms']:.1f} G terms/s (e2e {r['e2e_ms']:.1f} ms), splits {r['splits']}, "
-            f"ppl {r['ppl']}, wgs {r['workgroups']}")
+            f"engine {r['engine']}, wgs {r['workgroups']}")
 
     # ---- accuracy against the FP64 oracle (checker only) ----
     acc = {}
@@ -302,17 +311,31 @@ def main():
     head = results.get("c64") or next(iter(results.values()))
     kind = head["kind"]
     flop = FLOP_PER_TERM[kind]
-    achieved = head["terms"] * flop / (head["main_kernel_ms"] * 1e-3) / 1e12
+    terms_per_s = head["terms"] / (head["main_kernel_ms"] * 1e-3)
+    algorithmic = terms_per_s * flop / 1e12
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{kind}_{args.workload}.json")
+    kname = {"matrix": "precision_matrix", "vector": "precision"}.get(head["engine"], kind)
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             tj = json.load(fh)
         traffic = float(tj["traffic_bytes_per_launch"])
-    roof = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=traffic,
-                frac_nonpacked=round(achieved / PEAK_FP32_NONPACKED, 4),
-                flop_per_term=flop, kernel=f"{kind}_kernel ({head['storage']})")
+    if head["engine"] == "matrix":
+        # FP16 products on the matrix cores; the VALU part (phasors, nested
+        # sums, anchors) issues on the same SIMDs in between
+        mflop = MFMA_FLOP_PER_TERM[head["storage"]]
+        achieved = terms_per_s * mflop / 1e12
+        roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP16_MFMA_TFLOPS,
+                    unit="TFLOP/s", frac=round(achieved / PEAK_FP16_MFMA_TFLOPS, 4),
+                    traffic=traffic, mfma_flop_per_term=mflop,
+                    kernel=f"{kname}_kernel ({head['storage']})",
+                    algorithmic_fp32_tflops=round(algorithmic, 3), algorithmic_flop_per_term=flop,
+                    algorithmic_frac_of_fp32_peak=round(algorithmic / PEAK_FP32_TFLOPS, 4))
+    else:
+        roof = dict(bound="valu", achieved=round(algorithmic, 3), peak=PEAK_FP32_TFLOPS,
+                    unit="TFLOP/s", frac=round(algorithmic / PEAK_FP32_TFLOPS, 4),
+                    traffic=traffic, frac_nonpacked=round(algorithmic / PEAK_FP32_NONPACKED, 4),
+                    flop_per_term=flop, kernel=f"{kname}_kernel ({head['storage']})")
     out = {
         "metric": METRIC,
         "value": round(head["gterms"], 3),
@@ -341,7 +364,7 @@ def main():
                     ms_per_step=round(r["ms_per_step"], 4),
                     main_kernel_ms=round(r["main_kernel_ms"], 4),
                     e2e_ms=round(r["e2e_ms"], 3), gterms_e2e_pcie=round(r["gterms_e2e"], 3),
-                    chirp_splits=r["splits"], pixels_per_lane=r["ppl"],
+                    chirp_splits=r["splits"], engine=r["engine"],
                     **({k: float(f"{x:.4g}") for k, x in acc.get(v, {}).items()}))
             for v, r in results.items()
         },
@@ -353,12 +376,15 @@ def main():
         pr = serving["precision"]
         out["p50_ms"] = round(pr["p50_ms"], 3)
         out["p99_ms"] = round(pr["p99_ms"], 3)
+        # algorithmic FP32 flops of the served terms over the FP32 vector peak
         out["serving"]["precision"]["fp32_peak_frac"] = round(
             pr["gterms"] * 1e9 * FLOP_PER_TERM["precision"] / (PEAK_FP32_TFLOPS * 1e12), 4)
         out["serving"]["precision"]["p99_target_ms"] = args.serve_p99_target_ms
         out["serving"]["precision"]["p99_target_met"] = bool(pr["p99_ms"] <= args.serve_p99_target_ms)
     if "plain" in results and "c64" in results:
         out["precision_vs_plain"] = round(results["c64"]["gterms"] / results["plain"]["gterms"], 3)
+    if "vector" in results and "c64" in results:
+        out["matrix_vs_vector"] = round(results["c64"]["gterms"] / results["vector"]["gterms"], 3)
     print(json.dumps(out), flush=True)
 
 

@@ -7,10 +7,9 @@
 
 namespace tdc {
 
-// Pixels per lane of the serving tiles (the one-job kernels' measured best):
-// a precision tile covers 512 pixels, a plain tile 256.
-constexpr int kServePixelsPrecision = 2;
-constexpr int kServePixelsPlain = 1;
+// Pixels of one serving tile: one per thread of the workgroup (the matrix
+// engine's tile for the precision form, one pixel per lane for the plain form).
+constexpr int kServeTilePixels = kBlock;
 // Capacity of one step.
 constexpr int kMaxStepTiles = 32768;
 constexpr int kMaxStepFin = 32768;
@@ -18,7 +17,8 @@ constexpr int kMaxJobSlots = 8192;
 
 // One admitted job as the kernels see it.
 struct alignas(16) ServeJob {
-    const void *raw;          // precision: staged raw samples; plain: PlainRec records
+    const void *raw;          // precision: matrix engine A tiles; plain: PlainRec records
+    const float *inv_scale;   // precision: per-chirp scale of the A tiles
     const ChirpPos *cpos;     // precision: FP64 phase centre per chirp
     const double *grid64;     // precision: FP64 pixel coordinates
     const float *grid32;      // plain: FP32 pixel coordinates

@@ -25,7 +25,6 @@ namespace {
 
 __global__ void serve_reset_kernel(unsigned int *counter) { *counter = 0U; }
 
-template <int PP, int PL>
 __global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restrict__ jobs,
                                                        const StepTable *__restrict__ table,
                                                        unsigned int *__restrict__ counter)
@@ -44,22 +43,19 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restric
         const int n0 = tl.split * j.cps;
         const int n1 = min(j.npri, n0 + j.cps);
         if (j.form == 0) {
-            plain_tile<PL>(static_cast<const PlainRec *>(j.raw), j.grid32, j.npx, j.ns, n0, n1,
-                           tl.pix0,
-                           static_cast<float2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx);
+            plain_tile<1>(static_cast<const PlainRec *>(j.raw), j.grid32, j.npx, j.ns, n0, n1,
+                          tl.pix0,
+                          static_cast<float2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx);
         } else {
             double2 *out = static_cast<double2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx;
+            const auto *atab = static_cast<const uintx4_t *>(j.raw);
+            const int nq = matrix_quads(j.ns);
             if (j.storage == 0) {
-                precision_tile<PP, RawC64>(static_cast<const float2 *>(j.raw), j.cpos, j.grid64,
-                                           j.kanchor, j.dk, j.npx, j.ns, n0, n1, tl.pix0, out);
-            } else if (j.storage == 1) {
-                precision_tile<PP, RawF16>(static_cast<const unsigned int *>(j.raw), j.cpos,
-                                           j.grid64, j.kanchor, j.dk, j.npx, j.ns, n0, n1, tl.pix0,
-                                           out);
+                matrix_tile<2>(atab, j.inv_scale, j.cpos, j.grid64, j.kanchor, j.dk, j.npx, nq, n0,
+                               n1, tl.pix0, out);
             } else {
-                precision_tile<PP, RawBF16>(static_cast<const unsigned int *>(j.raw), j.cpos,
-                                            j.grid64, j.kanchor, j.dk, j.npx, j.ns, n0, n1,
-                                            tl.pix0, out);
+                matrix_tile<1>(atab, j.inv_scale, j.cpos, j.grid64, j.kanchor, j.dk, j.npx, nq, n0,
+                               n1, tl.pix0, out);
             }
         }
     }
@@ -128,8 +124,7 @@ hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsig
                              int workgroups, hipStream_t s)
 {
     hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL((serve_kernel<kServePixelsPrecision, kServePixelsPlain>), dim3(workgroups),
-                       dim3(kBlock), 0, s, jobs, table, counter);
+    hipLaunchKernelGGL(serve_kernel, dim3(workgroups), dim3(kBlock), 0, s, jobs, table, counter);
     return hipGetLastError();
 }
 

@@ -118,7 +118,7 @@ struct Job {
     double terms = 0.0;
     // device arena layout: [upload region | staged records or raw | partials]
     size_t off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
-           off_part = 0, up_bytes = 0, bytes = 0;
+           off_atab = 0, off_ascale = 0, off_part = 0, up_bytes = 0, bytes = 0;
     bool needs_stage = false;
     char *arena = nullptr;
     int slot = -1;
@@ -196,7 +196,7 @@ void plan(Job &j)
     const size_t npx = static_cast<size_t>(d.n_rows) * d.n_cols;
     j.npx = static_cast<int>(npx);
     const bool prec = d.form == TDC_FORM_PRECISION;
-    j.tile_px = tdc::kBlock * (prec ? tdc::kServePixelsPrecision : tdc::kServePixelsPlain);
+    j.tile_px = tdc::kServeTilePixels;
     j.npt = (j.npx + j.tile_px - 1) / j.tile_px;
     const double per_chirp = static_cast<double>(std::min(j.npx, j.tile_px)) * d.n_samples;
     j.cps = std::max(1, static_cast<int>(kTileTerms / std::max(per_chirp, 1.0)));
@@ -225,10 +225,16 @@ void plan(Job &j)
         o += align256(12 * npx);
     }
     j.up_bytes = o;
-    j.needs_stage = !prec || d.raw_storage != TDC_RAW_C64;
+    j.needs_stage = true;
     j.off_stage = o;
     if (!prec) o += align256(sizeof(tdc::PlainRec) * nterm);
     else if (d.raw_storage != TDC_RAW_C64) o += align256(4 * nterm);
+    if (prec) {
+        j.off_atab = o;
+        o += align256(tdc::matrix_table_bytes(d.n_pri, d.n_samples, d.raw_storage));
+        j.off_ascale = o;
+        o += align256(sizeof(float) * d.n_pri);
+    }
     j.off_part = o;
     o += align256((prec ? sizeof(double2) : sizeof(float2)) * npx * j.nsplit);
     j.bytes = o;
@@ -326,8 +332,8 @@ bool upload_job(tdc_server *s, Job &j, std::string &err)
     sj.storage = d.raw_storage;
     sj.partial = j.arena + j.off_part;
     if (prec) {
-        sj.raw = d.raw_storage == TDC_RAW_C64 ? static_cast<const void *>(j.arena)
-                                              : static_cast<const void *>(j.arena + j.off_stage);
+        sj.raw = j.arena + j.off_atab;
+        sj.inv_scale = reinterpret_cast<const float *>(j.arena + j.off_ascale);
         sj.grid64 = reinterpret_cast<const double *>(j.arena + j.off_grid);
         sj.kanchor = reinterpret_cast<const double *>(j.arena + j.off_ka);
         sj.dk = dk;
@@ -346,9 +352,20 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
 {
     const tdc_job_desc &d = j.d;
     if (d.form == TDC_FORM_PRECISION) {
-        return tdc::launch_stage_precision(d.raw_storage, reinterpret_cast<const float *>(j.arena),
-                                           reinterpret_cast<const float *>(j.arena + j.off_ones),
-                                           d.n_pri, d.n_samples, j.arena + j.off_stage, ss);
+        // raw samples in their storage format (complex64 as uploaded), then
+        // the matrix engine's A tiles
+        const void *staged = j.arena;
+        if (d.raw_storage != TDC_RAW_C64) {
+            hipError_t e = tdc::launch_stage_precision(
+                d.raw_storage, reinterpret_cast<const float *>(j.arena),
+                reinterpret_cast<const float *>(j.arena + j.off_ones), d.n_pri, d.n_samples,
+                j.arena + j.off_stage, ss);
+            if (e != hipSuccess) return e;
+            staged = j.arena + j.off_stage;
+        }
+        return tdc::launch_stage_matrix(d.raw_storage, staged, d.n_pri, d.n_samples,
+                                        tdc::matrix_parts(d.raw_storage), j.arena + j.off_atab,
+                                        reinterpret_cast<float *>(j.arena + j.off_ascale), ss);
     }
     return tdc::launch_stage_plain_chirp(
         reinterpret_cast<const float *>(j.arena),
