@@ -198,7 +198,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--serve-jobs", type=int, default=1000, help="serving-stream jobs (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=1400.0, help="offered jobs/s")
+    ap.add_argument("--serve-rate", type=float, default=2400.0, help="offered jobs/s")
     ap.add_argument("--serve-p99-target-ms", type=float, default=60.0,
                     help="stated p99 latency target of the serving stream")
     ap.add_argument("--serve-plain-jobs", type=int, default=200,

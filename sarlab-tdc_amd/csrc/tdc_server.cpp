@@ -3,10 +3,10 @@
 // No counterpart in the reference, whose TdcProcessor forms one image per
 // blocking start() call (tdc-lib/src/tdcprocessor.cpp:59-167). Two host
 // threads drive one GPU:
-//   uploader   takes admitted jobs in arrival order, packs each job's inputs
-//              into pinned staging memory and moves them with one large
-//              host-to-device copy (an SDMA transfer, no kernel), then marks
-//              the job resident;
+//   uploaders  (kUploaders threads) take admitted jobs in arrival order, pack
+//              each job's inputs into pinned staging memory and move them with
+//              one large host-to-device copy (an SDMA transfer, no kernel) on
+//              their own stream, then mark the job resident;
 //   scheduler  admits queued jobs in arrival order while their HBM footprint
 //              fits the budget, builds steps and completes them.
 // A step is a table of tiles (job, pixel tile, chirp range) taken from the
@@ -53,6 +53,13 @@ constexpr double kTileTerms = 8388608.0; // ~2^23 terms per tile
 constexpr int kFinWorkgroups = 256;
 constexpr int kBuckets[] = {256, 512, 1024, 2048};
 constexpr int kNumBuckets = 4;
+// Upload threads: packing a job into pinned memory runs at host memcpy speed,
+// so several jobs are packed and copied at once.
+constexpr int kUploaders = 4;
+// Steps in flight: each has its own tile table, counter and events, so the
+// host builds and launches the next steps while earlier ones run and their
+// completions wait; steps alternate between two streams.
+constexpr int kDepth = 4;
 
 int sfail(int code, const std::string &msg) { return tdc::set_last_error(code, msg); }
 
@@ -122,9 +129,8 @@ struct Job {
     bool needs_stage = false;
     char *arena = nullptr;
     int slot = -1;
-    float *h_image = nullptr; // pinned, written by the finalize kernel
+    float *h_image = nullptr; // pinned, written by the finalize kernel, read by wait
     // host
-    std::vector<float> image;
     double t_submit = 0, t_admit = 0, t_first = -1, t_done = 0;
     int steps = 0;
 };
@@ -144,17 +150,20 @@ struct tdc_server {
     double budget = 0.0;
     double step_terms = kDefaultStepTerms;
     hipStream_t step_stream[2] = {nullptr, nullptr}; // steps alternate between these
-    hipStream_t up_stream = nullptr;                 // uploads (uploader thread)
+    hipStream_t up_stream[kUploaders] = {};          // uploads, one per uploader thread
     hipStream_t free_stream = nullptr;               // stream-ordered frees
-    hipEvent_t ev_start[2] = {nullptr, nullptr}, ev_end[2] = {nullptr, nullptr};
-    hipEvent_t ev_fin[2] = {nullptr, nullptr}, ev_staged[2] = {nullptr, nullptr};
+    // job arenas come from a pool of the server's own that keeps freed memory
+    // (up to the budget) instead of returning it to the driver at each sync
+    hipMemPool_t pool = nullptr;
+    hipEvent_t ev_start[kDepth] = {}, ev_end[kDepth] = {};
+    hipEvent_t ev_fin[kDepth] = {}, ev_staged[kDepth] = {};
     hipEvent_t ev_origin = nullptr;
     tdc::ServeJob *h_jobs = nullptr;       // pinned job table
     tdc::ServeJob *dev_jobs = nullptr;     // its device address
-    tdc::StepTable *h_table[2] = {nullptr, nullptr};   // pinned step tables
-    tdc::StepTable *dev_table[2] = {nullptr, nullptr}; // their device addresses
-    unsigned int *d_counter = nullptr; // two counters, 256 bytes apart
-    hipGraphExec_t graph[kNumBuckets][2] = {};
+    tdc::StepTable *h_table[kDepth] = {};   // pinned step tables
+    tdc::StepTable *dev_table[kDepth] = {}; // their device addresses
+    unsigned int *d_counter = nullptr; // kDepth counters, 256 bytes apart
+    hipGraphExec_t graph[kNumBuckets][kDepth] = {};
     int ngraphs = 0;
     // staging for SDMA uploads, and coherent (uncached by the GPU) memory for
     // the images the finalize writes
@@ -165,7 +174,8 @@ struct tdc_server {
     std::mutex mu;
     std::condition_variable cv_sched, cv_done, cv_up;
     std::deque<Job *> queued;    // submitted, not yet admitted
-    std::deque<Job *> uploading; // admitted, waiting for the uploader
+    std::deque<Job *> uploading; // admitted, waiting for an uploader
+    int uploads_in_progress = 0;
     std::vector<Job *> active;   // resident, tiles outstanding or finishing
     std::map<int64_t, std::unique_ptr<Job>> jobs;
     std::vector<int> free_slots;
@@ -173,7 +183,7 @@ struct tdc_server {
     bool stop = false;
     bool stop_uploader = false;
     std::string fatal;
-    std::thread worker, uploader;
+    std::thread worker, uploader[kUploaders];
 
     // stats (guarded by mu)
     int64_t submitted = 0, completed = 0, failed = 0, nsteps = 0;
@@ -268,7 +278,7 @@ int validate(const tdc_job_desc *d)
     } while (0)
 
 // Pack, upload and describe one admitted job (uploader thread).
-bool upload_job(tdc_server *s, Job &j, std::string &err)
+bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
 {
     const tdc_job_desc &d = j.d;
     const size_t nterm = static_cast<size_t>(d.n_pri) * d.n_samples;
@@ -303,12 +313,12 @@ bool upload_job(tdc_server *s, Job &j, std::string &err)
         for (size_t i = 0; i < 3 * static_cast<size_t>(j.npx); ++i) g[i] = static_cast<float>(d.grid[i]);
     }
     void *arena = nullptr;
-    hipError_t e = hipMallocAsync(&arena, j.bytes, s->up_stream);
+    hipError_t e = hipMallocFromPoolAsync(&arena, j.bytes, s->pool, us);
     if (e == hipSuccess) {
         j.arena = static_cast<char *>(arena);
-        e = hipMemcpyAsync(arena, stage_mem, j.up_bytes, hipMemcpyHostToDevice, s->up_stream);
+        e = hipMemcpyAsync(arena, stage_mem, j.up_bytes, hipMemcpyHostToDevice, us);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(s->up_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(us);
     s->pinned.put(stage_mem, j.up_bytes);
     if (e != hipSuccess) {
         err = std::string("job upload failed: ") + hipGetErrorString(e);
@@ -415,9 +425,6 @@ bool complete_step(tdc_server *s, Step &st, std::string &err)
     SRV_HIP(hipEventElapsedTime(&t0, s->ev_origin, s->ev_start[st.buf]));
     SRV_HIP(hipEventElapsedTime(&t1, s->ev_origin, s->ev_end[st.buf]));
     for (Job *j : st.finishing) {
-        j->image.assign(j->h_image, j->h_image + 2 * static_cast<size_t>(j->npx));
-        s->coherent.put(j->h_image, sizeof(float2) * j->npx);
-        j->h_image = nullptr;
         SRV_HIP(hipFreeAsync(j->arena, s->free_stream));
         j->arena = nullptr;
     }
@@ -441,8 +448,9 @@ bool complete_step(tdc_server *s, Step &st, std::string &err)
     return true;
 }
 
-// Uploads admitted jobs in arrival order (uploader thread).
-void uploader(tdc_server *s)
+// Uploads admitted jobs, taken in arrival order (uploader thread `k`); a job
+// becomes resident when its own upload has ended.
+void uploader(tdc_server *s, int k)
 {
     if (hipSetDevice(s->device) != hipSuccess) return;
     for (;;) {
@@ -452,17 +460,19 @@ void uploader(tdc_server *s)
             s->cv_up.wait(lk, [&] { return !s->uploading.empty() || s->stop_uploader; });
             if (s->uploading.empty()) return;
             j = s->uploading.front();
+            s->uploading.pop_front();
+            ++s->uploads_in_progress;
         }
         std::string why;
-        const bool okay = upload_job(s, *j, why);
+        const bool okay = upload_job(s, *j, s->up_stream[k], why);
         std::lock_guard<std::mutex> lk(s->mu);
-        s->uploading.pop_front();
+        --s->uploads_in_progress;
         if (okay) {
             s->active.push_back(j);
         } else {
             s->in_use -= j->bytes;
             s->free_slots.push_back(j->slot);
-            if (j->arena) (void) hipFreeAsync(j->arena, s->up_stream);
+            if (j->arena) (void) hipFreeAsync(j->arena, s->up_stream[k]);
             j->arena = nullptr;
             if (j->h_image) s->coherent.put(j->h_image, sizeof(float2) * j->npx);
             j->h_image = nullptr;
@@ -498,7 +508,8 @@ void scheduler(tdc_server *s)
         // 1. admission in arrival order: charge the budget, hand to the uploader
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->queued.empty() && s->uploading.empty() && s->active.empty() && inflight.empty()) {
+            if (s->queued.empty() && s->uploading.empty() && s->uploads_in_progress == 0 &&
+                s->active.empty() && inflight.empty()) {
                 if (s->stop) break;
                 s->cv_sched.wait_for(lk, std::chrono::milliseconds(20));
                 continue;
@@ -521,7 +532,8 @@ void scheduler(tdc_server *s)
         // 2. completions: the oldest step when two are in flight (its table
         //    buffer is needed next), or any step whose finalize already ended
         while (!inflight.empty() &&
-               (inflight.size() == 2 || hipEventQuery(s->ev_fin[inflight.front().buf]) == hipSuccess)) {
+               (inflight.size() == static_cast<size_t>(kDepth) ||
+                hipEventQuery(s->ev_fin[inflight.front().buf]) == hipSuccess)) {
             if (!complete_step(s, inflight.front(), err)) return die(err);
             inflight.pop_front();
         }
@@ -529,7 +541,7 @@ void scheduler(tdc_server *s)
         // 3. build the next step from the resident jobs
         Step st;
         st.index = step_index;
-        st.buf = static_cast<int>(step_index & 1);
+        st.buf = static_cast<int>(step_index % kDepth);
         tdc::StepTable *tab = s->h_table[st.buf];
         int nt = 0, nf = 0;
         double work = 0.0;
@@ -608,11 +620,12 @@ void scheduler(tdc_server *s)
         }
         if (!ensure_graph(s, bucket, st.buf, err)) return die(err);
         const int p = st.buf;
-        hipStream_t ss = s->step_stream[p];
+        const int prev = (p + kDepth - 1) % kDepth; // the step before, on the other stream
+        hipStream_t ss = s->step_stream[p & 1];
         hipError_t e = hipSuccess;
         // jobs staged by the previous step (the other stream) must be staged
         // before this step reads them
-        if (step_index > 0) e = hipStreamWaitEvent(ss, s->ev_staged[1 - p], 0);
+        if (step_index > 0) e = hipStreamWaitEvent(ss, s->ev_staged[prev], 0);
         for (Job *j : st.stage)
             if (e == hipSuccess) e = stage_job(*j, ss);
         if (e == hipSuccess) e = hipEventRecord(s->ev_staged[p], ss);
@@ -623,7 +636,7 @@ void scheduler(tdc_server *s)
         // step (the other stream) has ended too: the two cover every earlier
         // step of the finishing jobs
         if (e == hipSuccess && step_index > 0 && nf > 0)
-            e = hipStreamWaitEvent(ss, s->ev_end[1 - p], 0);
+            e = hipStreamWaitEvent(ss, s->ev_end[prev], 0);
         if (e == hipSuccess && nf > 0)
             e = tdc::launch_serve_finalize(s->dev_jobs, s->dev_table[p], kFinWorkgroups, ss);
         if (e == hipSuccess) e = hipEventRecord(s->ev_fin[p], ss);
@@ -660,11 +673,18 @@ int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, t
     std::string err;
     auto init = [&]() -> bool {
         SRV_HIP(hipSetDevice(gpu_num));
-        SRV_HIP(hipStreamCreateWithFlags(&s->up_stream, hipStreamNonBlocking));
+        for (auto &us : s->up_stream) SRV_HIP(hipStreamCreateWithFlags(&us, hipStreamNonBlocking));
         SRV_HIP(hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking));
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = gpu_num;
+        SRV_HIP(hipMemPoolCreate(&s->pool, &props));
+        uint64_t keep = static_cast<uint64_t>(std::min(s->budget, 1.8e19));
+        SRV_HIP(hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep));
         SRV_HIP(hipEventCreate(&s->ev_origin));
-        for (int i = 0; i < 2; ++i) {
-            SRV_HIP(hipStreamCreateWithFlags(&s->step_stream[i], hipStreamNonBlocking));
+        for (auto &ss : s->step_stream) SRV_HIP(hipStreamCreateWithFlags(&ss, hipStreamNonBlocking));
+        for (int i = 0; i < kDepth; ++i) {
             SRV_HIP(hipEventCreate(&s->ev_start[i]));
             SRV_HIP(hipEventCreate(&s->ev_end[i]));
             SRV_HIP(hipEventCreateWithFlags(&s->ev_fin[i], hipEventDisableTiming));
@@ -680,7 +700,7 @@ int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, t
                               sizeof(tdc::ServeJob) * tdc::kMaxJobSlots,
                               hipHostMallocCoherent | hipHostMallocMapped));
         SRV_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->dev_jobs), s->h_jobs, 0));
-        SRV_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_counter), 512));
+        SRV_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_counter), 256 * kDepth));
         SRV_HIP(hipEventRecord(s->ev_origin, s->step_stream[0]));
         SRV_HIP(hipEventSynchronize(s->ev_origin));
         return true;
@@ -692,7 +712,7 @@ int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, t
     for (int i = tdc::kMaxJobSlots - 1; i >= 0; --i) s->free_slots.push_back(i);
     tdc_server *raw = s.release();
     raw->worker = std::thread(scheduler, raw);
-    raw->uploader = std::thread(uploader, raw);
+    for (int k = 0; k < kUploaders; ++k) raw->uploader[k] = std::thread(uploader, raw, k);
     *out = raw;
     tdc::clear_last_error();
     return TDC_OK;
@@ -761,7 +781,9 @@ int tdc_server_wait(tdc_server *s, int64_t job_id, float *image_iq, tdc_job_info
         info->steps = j->steps;
     }
     if (j->state == TDC_JOB_FAILED) return sfail(j->status, j->error);
-    if (image_iq) std::memcpy(image_iq, j->image.data(), sizeof(float) * j->image.size());
+    if (image_iq) std::memcpy(image_iq, j->h_image, sizeof(float2) * static_cast<size_t>(j->npx));
+    s->coherent.put(j->h_image, sizeof(float2) * j->npx);
+    j->h_image = nullptr;
     tdc::clear_last_error();
     return TDC_OK;
 }
@@ -808,25 +830,29 @@ int tdc_server_destroy(tdc_server *s)
         s->stop_uploader = true;
         s->cv_up.notify_all();
     }
-    if (s->uploader.joinable()) s->uploader.join();
+    for (auto &t : s->uploader)
+        if (t.joinable()) t.join();
     (void) hipSetDevice(s->device);
     (void) hipDeviceSynchronize();
     for (auto &row : s->graph)
         for (auto &g : row)
             if (g) (void) hipGraphExecDestroy(g);
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < kDepth; ++i) {
         if (s->ev_start[i]) (void) hipEventDestroy(s->ev_start[i]);
         if (s->ev_end[i]) (void) hipEventDestroy(s->ev_end[i]);
         if (s->ev_fin[i]) (void) hipEventDestroy(s->ev_fin[i]);
         if (s->ev_staged[i]) (void) hipEventDestroy(s->ev_staged[i]);
-        if (s->step_stream[i]) (void) hipStreamDestroy(s->step_stream[i]);
         if (s->h_table[i]) (void) hipHostFree(s->h_table[i]);
     }
+    for (auto &ss : s->step_stream)
+        if (ss) (void) hipStreamDestroy(ss);
     if (s->ev_origin) (void) hipEventDestroy(s->ev_origin);
     if (s->h_jobs) (void) hipHostFree(s->h_jobs);
     if (s->d_counter) (void) hipFree(s->d_counter);
     if (s->free_stream) (void) hipStreamDestroy(s->free_stream);
-    if (s->up_stream) (void) hipStreamDestroy(s->up_stream);
+    if (s->pool) (void) hipMemPoolDestroy(s->pool);
+    for (auto &us : s->up_stream)
+        if (us) (void) hipStreamDestroy(us);
     delete s;
     return TDC_OK;
 }

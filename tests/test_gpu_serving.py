@@ -108,11 +108,14 @@ def test_admission_respects_the_budget(tdc, orc):
 
 def test_small_jobs_overtake_a_large_one(tdc):
     """Jobs submitted while a large one runs join it on the device and finish first."""
-    big = job_scene(30, 2048, 512, 512)
+    big = job_scene(30, 2048, 512, 768)
     small = [job_scene(31 + k, 64, 128, 32) for k in range(5)]
     srv = tdc.Server(0, 64e9)
+    # warm every upload thread, its stream and the pinned pools
+    for i in [submit(srv, s) for s in small + small]:
+        srv.wait(i)
     bid = submit(srv, big)
-    time.sleep(0.005)
+    time.sleep(0.003)
     sids = [submit(srv, s) for s in small]
     res_small = [srv.wait(i) for i in sids]
     _, binfo = srv.wait(bid)
