@@ -272,6 +272,38 @@ def test_matrix_engine_per_chirp_scale(tdc, orc, gain_exp):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
+@pytest.mark.parametrize("engine", ["vector", "matrix"])
+def test_precision_down_chirp(tdc, orc, engine):
+    """Negative chirp rate: the rotation step dk and the phase fractions are negative."""
+    from sarlabtdc import scene
+
+    rng = np.random.default_rng(21)
+    npri, ns = 90, 300
+    track = np.zeros((npri, 3))
+    track[:, 0] = np.linspace(-0.2, 0.2, npri)
+    grid = scene.plane_grid(11, 13, 4e-3, (0.0, 0.6, 0.0))
+    scat = scene._uniform_on_grid(rng, grid, 5)
+    s = scene.synthesize("down", track, ns, grid, scat, scene._cn(rng, 5, 1.0), 30.0, rng,
+                         f0=81e9, mod_rate=-scene.MOD_RATE)
+    img, st = form(tdc, f64=s.f64(), kind="precision", engine=engine)
+    assert st["engine"] == engine
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
+@pytest.mark.parametrize("engine", ["vector", "matrix"])
+def test_precision_far_from_origin(tdc, orc, engine):
+    """Coordinates 1e5 m from the origin (a projected map frame): FP64 ranges
+    keep the phase; the image equals the one formed near the origin."""
+    s = ragged_scene(npri=70, ns=200, rows=9, cols=12, seed=8)
+    near, _ = form(tdc, f64=s.f64(), kind="precision", engine=engine)
+    off = np.array([1.5e5, -2.3e5, 310.0])
+    s.track = s.track + off
+    s.grid = s.grid + off
+    img, _ = form(tdc, f64=s.f64(), kind="precision", engine=engine)
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+    assert orc.rel_l2(img, near) < 1e-5
+
+
 def test_engines_agree(tdc, orc):
     from sarlabtdc import scene
 
