@@ -350,7 +350,9 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
                                         h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
                                         h->d_rec.as<tdc::PlainRec>(), s));
     }
-    h->ppl = h->force_ppl ? h->force_ppl : 1;
+    // measured best: two pixels per lane for the LDS-staged plain records, one
+    // for the Doppler kernel
+    h->ppl = h->force_ppl ? h->force_ppl : (h->doppler ? 1 : 2);
     if (h->ppl > 4)
         return fail(TDC_ERR_INVALID_ARGUMENT, "the plain form runs 1, 2 or 4 pixels per lane");
     const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);

@@ -4,8 +4,11 @@
 //
 // A tile is up to kBlock * P consecutive pixels (lane l of the workgroup owns
 // pixels pix0 + j * kBlock + l, j < P) times a chirp range [n0, n1). Every
-// (chirp, sample) input is the same for all lanes of a wavefront and is read
-// with scalar loads; pixel coordinates and accumulators live in VGPRs.
+// (chirp, sample) input is the same for all lanes of a wavefront: the plain
+// form stages it in LDS and reads it into VGPRs, the vector precision form
+// reads it with scalar loads, the matrix form with vector loads of
+// pre-arranged MFMA operands. Pixel coordinates and accumulators live in
+// VGPRs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -111,13 +114,19 @@ struct RawBF16 {
 
 // Plain form over one tile, Doppler window identically one: per term the
 // direct phase theta = fract(k_m |pos - p|) in cycles and a complex MAC of
-// raw * exp(+j 2 pi theta) (reference tdckernels.cu:206-215, :270-272).
+// raw * exp(+j 2 pi theta) (reference tdckernels.cu:206-215, :270-272). The
+// workgroup stages the tile's term records in LDS, kStageRecords at a time,
+// and every lane reads each record (a broadcast ds_read): the records then
+// reach the VALU as VGPR operands, which issue at twice the rate of SGPR
+// operands on gfx950.
+constexpr int kStageRecords = 256;
+
 template <int P>
 __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
                                            const float *__restrict__ grid, int npx, int ns,
                                            int n0, int n1, int pix0, float2 *__restrict__ out)
 {
-
+    __shared__ float4 s_rec[2 * kStageRecords];
     float px[P], py[P], pz[P], ar[P], ai[P];
     int pix[P];
 #pragma unroll
@@ -130,28 +139,34 @@ __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
         ar[j] = 0.0F;
         ai[j] = 0.0F;
     }
-
-    const cptr<PlainRec> r = as_const(rec + static_cast<size_t>(max(n0, 0)) * ns);
+    const float4 *src = reinterpret_cast<const float4 *>(rec + static_cast<size_t>(max(n0, 0)) * ns);
     const int nterms = (n1 > n0) ? (n1 - n0) * ns : 0;
+    for (int e0 = 0; e0 < nterms; e0 += kStageRecords) {
+        const int cnt = min(kStageRecords, nterms - e0);
+        __syncthreads();
+        for (int i = static_cast<int>(threadIdx.x); i < 2 * cnt; i += kBlock)
+            s_rec[i] = src[2 * static_cast<size_t>(e0) + i];
+        __syncthreads();
 #pragma unroll 4
-    for (int e = 0; e < nterms; ++e) {
-        const PlainRec t = r[e];
+        for (int e = 0; e < cnt; ++e) {
+            // record fields: a = (rr, ri, x, y), b = (z, k, pad, pad)
+            const float4 a = s_rec[2 * e], b = s_rec[2 * e + 1];
 #pragma unroll
-        for (int j = 0; j < P; ++j) {
-            const float dx = px[j] - t.x;
-            const float dy = py[j] - t.y;
-            const float dz = pz[j] - t.z;
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float th = __builtin_amdgcn_fractf(__builtin_amdgcn_sqrtf(r2) * t.k);
-            const float sn = sin_rev(th);
-            const float cs = cos_rev(th);
-            ar[j] = fmaf(t.rr, cs, ar[j]);
-            ar[j] = fmaf(-t.ri, sn, ar[j]);
-            ai[j] = fmaf(t.rr, sn, ai[j]);
-            ai[j] = fmaf(t.ri, cs, ai[j]);
+            for (int j = 0; j < P; ++j) {
+                const float dx = px[j] - a.z;
+                const float dy = py[j] - a.w;
+                const float dz = pz[j] - b.x;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float th = __builtin_amdgcn_fractf(__builtin_amdgcn_sqrtf(r2) * b.y);
+                const float sn = sin_rev(th);
+                const float cs = cos_rev(th);
+                ar[j] = fmaf(a.x, cs, ar[j]);
+                ar[j] = fmaf(-a.y, sn, ar[j]);
+                ai[j] = fmaf(a.x, sn, ai[j]);
+                ai[j] = fmaf(a.y, cs, ai[j]);
+            }
         }
     }
-
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         if (pix[j] < npx) out[pix[j]] = make_float2(ar[j], ai[j]);
