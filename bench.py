@@ -73,15 +73,23 @@ def make_proc(tdc, s, kind, storage="c64", engine="auto"):
     return p, keep
 
 
-def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto"):
+# The drop-in with the Doppler window on: centroid-centred band of 4 kHz with
+# the raised-cosine taper (the platform moves ~10 m/s along the arc, so the
+# Doppler spread at 77 GHz is ~5 kHz and the window varies over the aperture).
+DOPPLER = dict(center=0.0, bandwidth=4000.0, centroid=True, taper=True)
+
+
+def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=None):
     p, keep = make_proc(tdc, s, kind, storage, engine)
+    win = ((doppler["center"], doppler["bandwidth"], doppler["centroid"], False,
+            doppler["taper"]) if doppler else (0.0, 1e30, False, False, False))
     # end-to-end start(): host arrays -> H2D -> compute -> D2H (PCIe inclusive); the
     # first call also allocates the processor's device buffers, the second reuses them
-    p.start(0.0, 1e30, False, False, False)
+    p.start(*win)
     e2e_first_ms = p.stats()["end_to_end_ms"]
-    p.start(0.0, 1e30, False, False, False)
+    p.start(*win)
     e2e_ms = p.stats()["end_to_end_ms"]
-    p.prepare(0.0, 1e30, False, False, False)
+    p.prepare(*win)
     if warmup > 0:
         p.run(warmup)
     t0 = time.perf_counter()
@@ -95,6 +103,8 @@ def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto"):
                 kernel_ms=st["kernel_ms"], e2e_ms=e2e_ms, e2e_first_ms=e2e_first_ms,
                 terms=st["terms"],
                 splits=st["chirp_splits"], ppl=st["pixels_per_lane"], engine=st["engine"],
+                doppler_active=st["doppler_active"], skipped_chunks=st["skipped_chunks"],
+                processed_chunks=st["processed_chunks"],
                 workgroups=st["workgroups"], image=img, keep=keep)
 
 
@@ -195,9 +205,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="arc", choices=["line_jitter", "arc", "helix"])
-    ap.add_argument("--variants", default="c64,f16,bf16,vector,plain",
+    ap.add_argument("--variants", default="c64,f16,bf16,vector,plain,doppler",
                     help="comma list of c64, f16, bf16 (precision form, matrix engine), vector "
-                         "(precision form, vector engine, c64) and plain")
+                         "(precision form, vector engine, c64), plain and doppler (the plain "
+                         "form with the Doppler window on)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -247,10 +258,11 @@ def main():
 
     results = {}
     for v in [x.strip() for x in args.variants.split(",") if x.strip()]:
-        kind = "plain" if v == "plain" else "precision"
-        storage = "c64" if v in ("plain", "vector") else v
+        kind = "plain" if v in ("plain", "doppler") else "precision"
+        storage = "c64" if v in ("plain", "vector", "doppler") else v
         engine = "vector" if v == "vector" else "auto"
-        r = time_variant(tdc, s, kind, storage, args.steps, args.warmup, engine)
+        r = time_variant(tdc, s, kind, storage, args.steps, args.warmup, engine,
+                         DOPPLER if v == "doppler" else None)
         r["gterms"] = r["terms"] / (r["ms_per_step"] * 1e-3) / 1e9
         r["gterms_kernel"] = r["terms"] / (r["main_kernel_ms"] * 1e-3) / 1e9
         r["gterms_e2e"] = r["terms"] / (r["e2e_ms"] * 1e-3) / 1e9
@@ -274,7 +286,16 @@ def main():
         o64 = None
         o32 = None
         for v, r in results.items():
-            if v == "plain":
+            if v == "doppler":
+                d = r["keep"]
+                dop = dict(velocity=d["velocity"], attitude=d["attitude"],
+                           boresight=d["bodyBoresight"], center=DOPPLER["center"],
+                           bandwidth=DOPPLER["bandwidth"], centroid=DOPPLER["centroid"],
+                           taper=DOPPLER["taper"])
+                ref = oracle.image_f64(d["rawData"], d["sampleTimes"], d["position"],
+                                       d["focusGrid"], float(d["modRate"]),
+                                       float(d["startFreq"]), doppler=dop, pixels=px)
+            elif v == "plain":
                 if o32 is None:
                     d = r["keep"]
                     o32 = oracle.image_f64(d["rawData"], d["sampleTimes"], d["position"],
@@ -377,6 +398,8 @@ def main():
                     e2e_ms=round(r["e2e_ms"], 3), e2e_first_ms=round(r["e2e_first_ms"], 3),
                     gterms_e2e_pcie=round(r["gterms_e2e"], 3),
                     chirp_splits=r["splits"], engine=r["engine"],
+                    **({"skipped_chunks": r["skipped_chunks"],
+                        "processed_chunks": r["processed_chunks"]} if r["doppler_active"] else {}),
                     **({k: float(f"{x:.4g}") for k, x in acc.get(v, {}).items()}))
             for v, r in results.items()
         },
