@@ -711,6 +711,7 @@ int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwi
                 int dop_centroid_win, int apply_range_win, int doppler_win_taper)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    tdc::Range range("tdc_prepare");
     if (!h->have_raw32 && !h->have_raw64)
         return fail(TDC_ERR_STATE, "setRawData must be called before start()");
     int rc = ensure_device(h);
@@ -733,6 +734,7 @@ int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwi
 int tdc_run(tdc_processor *h, int iterations)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    tdc::Range range("tdc_run");
     if (!h->prepared) return fail(TDC_ERR_STATE, "tdc_prepare must succeed before tdc_run");
     if (iterations < 1) return fail(TDC_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
     int rc = ensure_device(h);
@@ -810,6 +812,7 @@ int tdc_run(tdc_processor *h, int iterations)
 int tdc_fetch_image(tdc_processor *h)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    tdc::Range range("tdc_fetch_image");
     if (!h->prepared) return fail(TDC_ERR_STATE, "nothing has been run");
     int rc = ensure_device(h);
     if (rc != TDC_OK) return rc;
@@ -828,6 +831,7 @@ int tdc_start(tdc_processor *h, float doppler_win_center, float doppler_bandwidt
               int dop_centroid_win, int apply_range_win, int doppler_win_taper)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    tdc::Range range("tdc_start");
     const auto t0 = std::chrono::steady_clock::now();
     int rc = tdc_prepare(h, doppler_win_center, doppler_bandwidth, dop_centroid_win,
                          apply_range_win, doppler_win_taper);

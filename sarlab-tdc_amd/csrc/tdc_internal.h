@@ -7,6 +7,7 @@
 #include <vector>
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 namespace tdc {
 
@@ -116,6 +117,17 @@ inline size_t matrix_table_bytes(int npri, int ns, int storage)
 inline int anchor_entries(int ns) { return 4 * matrix_quads(ns); }
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
+
+// A named roctx range over a scope: host phases (prepare, run, fetch, job
+// uploads, serving steps) appear beside the kernels in a profiler's marker
+// trace; without a profiler attached the calls return at once.
+class Range {
+  public:
+    explicit Range(const char *name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range &) = delete;
+    Range &operator=(const Range &) = delete;
+};
 
 // Host helpers shared by the processor and the server (tdc_capi.cpp).
 // The thread-local message returned by tdc_last_error().

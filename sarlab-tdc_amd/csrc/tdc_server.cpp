@@ -280,6 +280,7 @@ int validate(const tdc_job_desc *d)
 // Pack, upload and describe one admitted job (uploader thread).
 bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
 {
+    tdc::Range range("tdc_server_upload");
     const tdc_job_desc &d = j.d;
     const size_t nterm = static_cast<size_t>(d.n_pri) * d.n_samples;
     const bool prec = d.form == TDC_FORM_PRECISION;
@@ -420,6 +421,7 @@ void fail_job_locked(tdc_server *s, Job *j, int code, const std::string &msg)
 // Finish the jobs of a step whose finalize has ended (scheduler thread).
 bool complete_step(tdc_server *s, Step &st, std::string &err)
 {
+    tdc::Range range("tdc_server_complete_step");
     SRV_HIP(hipEventSynchronize(s->ev_fin[st.buf]));
     float t0 = 0.0F, t1 = 0.0F;
     SRV_HIP(hipEventElapsedTime(&t0, s->ev_origin, s->ev_start[st.buf]));
@@ -619,6 +621,7 @@ void scheduler(tdc_server *s)
             }
         }
         if (!ensure_graph(s, bucket, st.buf, err)) return die(err);
+        tdc::Range range("tdc_server_launch_step");
         const int p = st.buf;
         const int prev = (p + kDepth - 1) % kDepth; // the step before, on the other stream
         hipStream_t ss = s->step_stream[p & 1];
