@@ -1,0 +1,119 @@
+"""GPU parity at the benchmark sizes: the arc workload (1024 chirps x 512
+samples, 512 x 512 pixels) and the helix workload (2048 x 512, 768 x 768).
+
+The FP64 oracle is evaluated on seeded pixel subsets (plus the brightest
+pixels); properties that hold at any size are checked on the whole image:
+linearity in the raw data, agreement of the two precision engines, the FP16
+storage bound, and independence from the chirp-split count.
+"""
+import numpy as np
+import pytest
+
+from conftest import subset
+
+pytestmark = pytest.mark.gpu
+
+PREC_TOL = 1e-4
+PLAIN_TOL = 2e-3
+
+
+@pytest.fixture(scope="module")
+def arc():
+    from sarlabtdc import scene
+
+    return scene.make("arc")
+
+
+def precision(tdc, s, raw=None, storage="c64", engine="auto", splits=0):
+    f = s.f64()
+    if raw is not None:
+        f["rawData"] = np.ascontiguousarray(raw, dtype=np.complex64)
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"], f["modRate"],
+                    f["startFreq"])
+    p.setFocusGridF64(f["focusGrid"])
+    p.setRawStorage(storage)
+    p.setEngine(engine)
+    p.setLaunch(0, splits)
+    p.start(0.0, 1e30, False, False, False)
+    return p.getFocusedImage().reshape(-1), p.stats()
+
+
+def oracle_subset(orc, s, img, n=256, seed=11):
+    px = subset(img.size, n, seed=seed, extra=np.argsort(np.abs(img))[-16:])
+    ref = orc.image_f64(s.raw, s.sample_times, s.track, s.grid, s.mod_rate, s.f0, pixels=px)
+    return px, ref
+
+
+def test_arc_matrix_engine_vs_oracle(tdc, orc, arc):
+    img, st = precision(tdc, arc)
+    assert st["engine"] == "matrix" and st["terms"] == arc.terms
+    px, ref = oracle_subset(orc, arc, img)
+    e = orc.rel_l2(img[px], ref)
+    ph = orc.peak_phase_error(img[px], ref)
+    print(f"arc matrix: rel-L2 {e:.3e}, peak phase {ph:.3e} rad")
+    assert e <= PREC_TOL and ph <= PREC_TOL
+
+
+def test_arc_linearity(tdc, arc):
+    """I(raw1 + 2j raw2) = I(raw1) + 2j I(raw2) over all 262 144 pixels."""
+    rng = np.random.default_rng(4)
+    raw2 = (rng.normal(size=arc.raw.shape) + 1j * rng.normal(size=arc.raw.shape)).astype(np.complex64)
+    a, _ = precision(tdc, arc)
+    b, _ = precision(tdc, arc, raw=raw2)
+    c, _ = precision(tdc, arc, raw=arc.raw + 2j * raw2)
+    want = a.astype(np.complex128) + 2j * b.astype(np.complex128)
+    e = np.linalg.norm(c - want) / np.linalg.norm(want)
+    print(f"linearity residual {e:.3e}")
+    assert e < 1e-5
+
+
+def test_arc_engines_and_storage(tdc, arc):
+    m, _ = precision(tdc, arc, engine="matrix")
+    v, _ = precision(tdc, arc, engine="vector")
+    h, _ = precision(tdc, arc, storage="f16")
+    ev = np.linalg.norm(m - v) / np.linalg.norm(v)
+    eh = np.linalg.norm(h - m) / np.linalg.norm(m)
+    print(f"matrix vs vector {ev:.3e}; f16 storage vs complex64 {eh:.3e}")
+    assert ev < 2e-6
+    assert eh < 3e-3
+
+
+def test_arc_split_count_invariance(tdc, arc):
+    """Chirp splits change only the order of FP64 additions of the partials."""
+    a, sa = precision(tdc, arc, splits=4)
+    b, sb = precision(tdc, arc, splits=64)
+    assert sa["chirp_splits"] == 4 and sb["chirp_splits"] == 64
+    e = np.linalg.norm(a - b) / np.linalg.norm(a)
+    print(f"4 vs 64 splits: {e:.3e}")
+    assert e < 1e-6
+
+
+def test_arc_plain_vs_oracle(tdc, orc, arc):
+    d = arc.dropin()
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"], d["velocity"],
+                 d["attitude"], d["modRate"], d["startFreq"], d["bodyBoresight"])
+    p.setFocusGrid(d["focusGrid"])
+    p.start(0.0, 1e30, False, False, False)
+    img = p.getFocusedImage().reshape(-1)
+    px = subset(img.size, 128, seed=12, extra=np.argsort(np.abs(img))[-8:])
+    ref = orc.image_f64(d["rawData"], d["sampleTimes"], d["position"], d["focusGrid"],
+                        float(d["modRate"]), float(d["startFreq"]), pixels=px)
+    e = orc.rel_l2(img[px], ref)
+    print(f"arc plain: rel-L2 {e:.3e}")
+    assert e <= PLAIN_TOL
+
+
+def test_helix_matrix_engine_vs_oracle(tdc, orc):
+    from sarlabtdc import scene
+
+    s = scene.make("helix")
+    img, st = precision(tdc, s)
+    assert st["terms"] == s.terms
+    px, ref = oracle_subset(orc, s, img, n=128, seed=13)
+    e = orc.rel_l2(img[px], ref)
+    print(f"helix matrix: rel-L2 {e:.3e}")
+    assert e <= PREC_TOL
