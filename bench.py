@@ -346,12 +346,23 @@ def main():
     terms_per_s = head["terms"] / (head["main_kernel_ms"] * 1e-3)
     algorithmic = terms_per_s * flop / 1e12
     traffic = None
+    simd = None
     kname = {"matrix": "precision_matrix", "vector": "precision"}.get(head["engine"], kind)
     tfile = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             tj = json.load(fh)
         traffic = float(tj["traffic_bytes_per_launch"])
+        sq = tj.get("sq", {})
+        if sq.get("GRBM_GUI_ACTIVE"):
+            # share of the SIMDs' cycles (1024 SIMDs, GRBM summed over 8 XCDs) spent
+            # issuing MFMA and other VALU work (2 cycles per wave64 VALU instruction),
+            # from the committed rocprofv3 PMC pass of the same kernel and workload
+            cyc = sq["GRBM_GUI_ACTIVE"] / 8 * 1024
+            mfma = sq.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / cyc
+            valu = 2.0 * (sq["SQ_INSTS_VALU"] - sq.get("SQ_INSTS_MFMA", 0.0)) / cyc
+            simd = dict(mfma_busy_frac=round(mfma, 3), valu_issue_frac=round(valu, 3),
+                        source=os.path.relpath(tfile, ROOT))
     if head["engine"] == "matrix":
         # FP16 products on the matrix cores; the VALU part (phasors, nested
         # sums, anchors) issues on the same SIMDs in between
@@ -362,12 +373,14 @@ def main():
                     traffic=traffic, mfma_flop_per_term=mflop,
                     kernel=f"{kname}_kernel ({head['storage']})",
                     algorithmic_fp32_tflops=round(algorithmic, 3), algorithmic_flop_per_term=flop,
-                    algorithmic_frac_of_fp32_peak=round(algorithmic / PEAK_FP32_TFLOPS, 4))
+                    algorithmic_frac_of_fp32_peak=round(algorithmic / PEAK_FP32_TFLOPS, 4),
+                    simd_cycles=simd)
     else:
         roof = dict(bound="valu", achieved=round(algorithmic, 3), peak=PEAK_FP32_TFLOPS,
                     unit="TFLOP/s", frac=round(algorithmic / PEAK_FP32_TFLOPS, 4),
                     traffic=traffic, frac_nonpacked=round(algorithmic / PEAK_FP32_NONPACKED, 4),
-                    flop_per_term=flop, kernel=f"{kname}_kernel ({head['storage']})")
+                    flop_per_term=flop, kernel=f"{kname}_kernel ({head['storage']})",
+                    simd_cycles=simd)
     out = {
         "metric": METRIC,
         "value": round(head["gterms"], 3),
