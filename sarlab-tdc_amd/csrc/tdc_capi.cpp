@@ -9,6 +9,7 @@
 // reused across calls (the reference reallocates every buffer on every
 // start(), tdcprocessor.cpp:65-66).
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -569,6 +570,8 @@ int tdc_set_raw_data(tdc_processor *h, const float *raw_iq, const float *pri_tim
     // tdcprocessor.cpp:189-195
     if (n_pri <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nPri must be greater than 0");
     if (n_samples <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nSamples must be greater than 0");
+    if (static_cast<int64_t>(n_pri) * n_samples > INT_MAX)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "nPri * nSamples must not exceed 2^31 - 1");
     if (!raw_iq || !sample_times || !position)
         return fail(TDC_ERR_INVALID_ARGUMENT, "rawData, sampleTimes and position must not be NULL");
     h->raw = raw_iq;
@@ -598,7 +601,9 @@ int tdc_set_focus_grid(tdc_processor *h, const float *grid, int n_rows, int n_co
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     if (n_rows < 0 || n_cols < 0)
         return fail(TDC_ERR_INVALID_ARGUMENT, "grid dimensions must not be negative");
-    if (!grid && n_rows * n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
+    if (static_cast<int64_t>(n_rows) * n_cols > INT_MAX)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^31 - 1 pixels");
+    if (!grid && n_rows > 0 && n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
     h->grid = grid;
     h->grid64 = nullptr;
     h->rows = n_rows;
@@ -662,6 +667,8 @@ int tdc_set_raw_data_f64(tdc_processor *h, const float *raw_iq, const double *pr
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     if (n_pri <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nPri must be greater than 0");
     if (n_samples <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nSamples must be greater than 0");
+    if (static_cast<int64_t>(n_pri) * n_samples > INT_MAX)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "nPri * nSamples must not exceed 2^31 - 1");
     if (!raw_iq || !sample_times || !chirp_position)
         return fail(TDC_ERR_INVALID_ARGUMENT, "rawData, sampleTimes and position must not be NULL");
     (void) pri_times;
@@ -684,7 +691,9 @@ int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     if (n_rows < 0 || n_cols < 0)
         return fail(TDC_ERR_INVALID_ARGUMENT, "grid dimensions must not be negative");
-    if (!grid && n_rows * n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
+    if (static_cast<int64_t>(n_rows) * n_cols > INT_MAX)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^31 - 1 pixels");
+    if (!grid && n_rows > 0 && n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
     h->grid64 = grid;
     h->grid = nullptr;
     h->rows = n_rows;

@@ -28,6 +28,7 @@
 // conversion) run on the step stream ahead of the job's first step.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
@@ -257,6 +258,10 @@ int validate(const tdc_job_desc *d)
     if (d->n_samples <= 0) return sfail(TDC_ERR_INVALID_ARGUMENT, "nSamples must be greater than 0");
     if (d->n_rows <= 0 || d->n_cols <= 0)
         return sfail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not be empty");
+    if (static_cast<int64_t>(d->n_pri) * d->n_samples > INT_MAX)
+        return sfail(TDC_ERR_INVALID_ARGUMENT, "nPri * nSamples must not exceed 2^31 - 1");
+    if (static_cast<int64_t>(d->n_rows) * d->n_cols > INT_MAX)
+        return sfail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^31 - 1 pixels");
     if (!d->raw_iq || !d->sample_times || !d->chirp_position || !d->grid)
         return sfail(TDC_ERR_INVALID_ARGUMENT, "job arrays must not be NULL");
     if (d->form != TDC_FORM_PLAIN && d->form != TDC_FORM_PRECISION)

@@ -126,3 +126,27 @@ def test_capi_precision_golden_ragged(lib, orc, engine):
     e = orc.rel_l2(img, z["image_f64"])
     print(f"precision engine {engine} via C ABI: rel-L2 {e:.3e}")
     assert e <= 1e-4
+
+
+def test_capi_rejects_sizes_beyond_int_range(lib):
+    """Pixel and per-pixel term counts are 32-bit: larger inputs fail cleanly
+    at the setter, before anything reads the (here too small) arrays."""
+    codes = status_codes()
+    bad = codes["TDC_ERR_INVALID_ARGUMENT"]
+    buf, p_buf = f32(np.zeros(16))
+    dbuf, p_dbuf = f64(np.zeros(16))
+    h = ctypes.c_void_p()
+    assert lib.tdc_create(0, ctypes.byref(h)) == 0, lib.tdc_last_error()
+    try:
+        assert lib.tdc_set_focus_grid(h, p_buf, 65536, 65536) == bad
+        assert b"2^31" in lib.tdc_last_error()
+        assert lib.tdc_set_focus_grid_f64(h, p_dbuf, 46341, 46341) == bad
+        assert lib.tdc_set_raw_data(h, p_buf, p_buf, p_buf, p_buf, p_buf, p_buf, 65536, 32768,
+                                    1e14, 77e9, p_buf) == bad
+        assert lib.tdc_set_raw_data_f64(h, p_buf, p_dbuf, p_dbuf, p_dbuf, 1 << 20, 4096,
+                                        1e14, 77e9) == bad
+        assert b"2^31" in lib.tdc_last_error()
+        # at the limit the setters accept the sizes (nothing is read before a start)
+        assert lib.tdc_set_focus_grid(h, p_buf, 46340, 46340) == 0
+    finally:
+        lib.tdc_destroy(h)
