@@ -304,6 +304,21 @@ def test_precision_far_from_origin(tdc, orc, engine):
     assert orc.rel_l2(img, near) < 1e-5
 
 
+@pytest.mark.parametrize("kind,engine", [("precision", "matrix"), ("precision", "vector"),
+                                         ("plain", "auto")])
+def test_more_splits_than_chirps(tdc, orc, kind, engine):
+    """64 chirp ranges over 40 chirps: the empty ranges contribute zero partials."""
+    s = ragged_scene(npri=40, ns=96, rows=5, cols=7, seed=17)
+    if kind == "precision":
+        img, st = form(tdc, f64=s.f64(), kind=kind, engine=engine, splits=64)
+        tol = PREC_TOL
+    else:
+        img, st = form(tdc, s.dropin(), kind=kind, splits=64)
+        tol = PLAIN_TOL
+    assert st["chirp_splits"] == 64
+    check(orc, img, oracle_f64(orc, s), tol)
+
+
 def test_engines_agree(tdc, orc):
     from sarlabtdc import scene
 
