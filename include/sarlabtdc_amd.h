@@ -135,6 +135,10 @@ int tdc_image_shape(const tdc_processor *h, int *n_rows, int *n_cols);
 
 /* Range window used when start(..., apply_range_win = 1). */
 int tdc_set_range_window(tdc_processor *h, int kind, double p0, double p1);
+/* The range window of (kind, p0, p1) as the library applies it: evaluated in
+ * FP64, periodic (DFT-even), rounded once to window[n_samples]. Host only; no
+ * device or handle needed. */
+int tdc_range_window(int kind, double p0, double p1, int n_samples, float *window);
 
 /* Form and device raw storage used by the next start / prepare. */
 int tdc_set_form(tdc_processor *h, int form);

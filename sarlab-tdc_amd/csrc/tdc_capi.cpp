@@ -612,9 +612,8 @@ int tdc_set_focus_grid(tdc_processor *h, const float *grid, int n_rows, int n_co
     return ok();
 }
 
-int tdc_set_range_window(tdc_processor *h, int kind, double p0, double p1)
+static int check_window(int kind, double p0, double p1)
 {
-    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     if (kind == TDC_WIN_HAMMING) {
         if (!(p0 >= 0.0 && p0 <= 1.0)) return fail(TDC_ERR_INVALID_ARGUMENT, "Hamming A must be in [0, 1]");
     } else if (kind == TDC_WIN_TAYLOR) {
@@ -623,6 +622,26 @@ int tdc_set_range_window(tdc_processor *h, int kind, double p0, double p1)
     } else if (kind != TDC_WIN_HANN) {
         return fail(TDC_ERR_INVALID_ARGUMENT, "unknown window kind");
     }
+    return TDC_OK;
+}
+
+int tdc_range_window(int kind, double p0, double p1, int n_samples, float *window)
+{
+    if (n_samples <= 0) return fail(TDC_ERR_INVALID_ARGUMENT, "nSamples must be greater than 0");
+    if (!window) return fail(TDC_ERR_INVALID_ARGUMENT, "window is NULL");
+    const int rc = check_window(kind, p0, p1);
+    if (rc != TDC_OK) return rc;
+    std::vector<float> w;
+    range_window(kind, p0, p1, n_samples, true, w);
+    std::copy(w.begin(), w.end(), window);
+    return ok();
+}
+
+int tdc_set_range_window(tdc_processor *h, int kind, double p0, double p1)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    const int rc = check_window(kind, p0, p1);
+    if (rc != TDC_OK) return rc;
     h->win_kind = kind;
     h->win_p0 = p0;
     h->win_p1 = p1;

@@ -119,3 +119,39 @@ int main() {
            "-lsarlabtdc_amd", "-Wl,-rpath," + pkg, "-o", str(tmp_path / "user")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("ns", [1, 7, 64, 100, 512])
+def test_range_window_matches_scipy(lib, ns):
+    """The library's FP64 range windows against scipy, on the host. Periodic =
+    the symmetric (ns+1)-point window without its last point (scipy's sym=False,
+    written out so that ns = 1 gives the reference's A - (1-A) cos 0, not 1)."""
+    import numpy as np
+    from scipy.signal import windows
+
+    def periodic(f, *a, **k):
+        return f(ns + 1, *a, sym=True, **k)[:-1]
+
+    lib.tdc_range_window.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float)]
+    codes = status_codes()
+    cases = [
+        (codes["TDC_WIN_HAMMING"], 25.0 / 46.0, 0.0,
+         periodic(windows.general_hamming, 25.0 / 46.0)),
+        (codes["TDC_WIN_HANN"], 0.0, 0.0, periodic(windows.hann)),
+        (codes["TDC_WIN_TAYLOR"], 4.0, 30.0, periodic(windows.taylor, nbar=4, sll=30)),
+        (codes["TDC_WIN_TAYLOR"], 6.0, 40.0, periodic(windows.taylor, nbar=6, sll=40)),
+    ]
+    for kind, p0, p1, want in cases:
+        out = np.empty(ns, np.float32)
+        rc = lib.tdc_range_window(kind, p0, p1, ns,
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        assert rc == codes["TDC_OK"], lib.tdc_last_error()
+        np.testing.assert_allclose(out, want.astype(np.float32), rtol=0, atol=2e-7)
+    bad = np.empty(ns, np.float32)
+    p = bad.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    assert lib.tdc_range_window(99, 0.0, 0.0, ns, p) == codes["TDC_ERR_INVALID_ARGUMENT"]
+    assert lib.tdc_range_window(codes["TDC_WIN_TAYLOR"], 0.5, 30.0, ns, p) == \
+        codes["TDC_ERR_INVALID_ARGUMENT"]
+    assert lib.tdc_range_window(codes["TDC_WIN_HANN"], 0.0, 0.0, 0, p) == \
+        codes["TDC_ERR_INVALID_ARGUMENT"]
