@@ -80,7 +80,8 @@ typedef struct tdc_stats {
     int64_t skipped_chunks;   /* (pixel, chunk) pairs skipped; only counted when doppler_active */
     int64_t processed_chunks; /* (pixel, chunk) pairs processed */
     int32_t engine;           /* TDC_ENGINE_* actually run (precision form), else 0 */
-    int32_t reserved;
+    int32_t window_per_chirp; /* 1 when the Doppler window ran once per (pixel, chirp): the
+                                 antenna state was constant within every chirp */
 } tdc_stats;
 
 /* Message for the last failing call on this thread ("" when none). */

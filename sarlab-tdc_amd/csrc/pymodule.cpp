@@ -235,6 +235,7 @@ public:
         d["workgroups"] = st.workgroups;
         d["skipped_chunks"] = st.skipped_chunks;
         d["processed_chunks"] = st.processed_chunks;
+        d["window_per_chirp"] = static_cast<bool>(st.window_per_chirp);
         d["engine"] = st.engine == TDC_ENGINE_MATRIX ? "matrix" : (st.engine == TDC_ENGINE_VECTOR ? "vector" : "none");
         return d;
     }

@@ -196,6 +196,7 @@ struct tdc_processor {
     int run_storage = TDC_RAW_C64;
     int run_engine = 0;
     bool doppler = false;
+    bool dop_chirp = false; // Doppler window evaluated per (pixel, chirp)
     tdc::Geom geom{};
     tdc::DopParams dp{};
     int ppl = 1;
@@ -253,6 +254,24 @@ void make_geom(tdc::Geom &g, int npx, int npri, int ns, int ppl, int nsplit)
 // |f_c| <= (2/lambda)|v| |q|^2 |b| (a rotation by a non-unit quaternion
 // scales by |q|^2), and excludes pixels that coincide with an antenna
 // position (0/0 there makes the reference's window NaN, hence 0).
+// True when every chirp's antenna position, velocity and attitude are the same
+// for all of its samples (bit for bit): the Doppler window of a term then
+// depends only on its pixel and chirp.
+bool antenna_constant_per_chirp(const tdc_processor *h)
+{
+    const int ns = h->ns;
+    for (int n = 0; n < h->npri; ++n) {
+        const size_t base = static_cast<size_t>(n) * ns;
+        for (int m = 1; m < ns; ++m) {
+            if (std::memcmp(h->pos + 3 * (base + m), h->pos + 3 * base, 12) != 0 ||
+                std::memcmp(h->vel + 3 * (base + m), h->vel + 3 * base, 12) != 0 ||
+                std::memcmp(h->att + 4 * (base + m), h->att + 4 * base, 16) != 0)
+                return false;
+        }
+    }
+    return true;
+}
+
 bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool centroid,
                            bool taper)
 {
@@ -319,6 +338,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         kt[m] = static_cast<float>(2.0 * f / kSpeedOfLight);
     }
     h->doppler = !doppler_window_is_one(h, dop_center, dop_bw, centroid, taper);
+    h->dop_chirp = false;
     hipStream_t s = h->stream;
     int rc;
     if ((rc = upload(h->d_raw, h->raw, nterm * 2, s)) != TDC_OK) return rc;
@@ -329,6 +349,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     if (h->doppler) {
         if (!h->vel || !h->att)
             return fail(TDC_ERR_INVALID_ARGUMENT, "the Doppler window needs velocity and attitude");
+        h->dop_chirp = antenna_constant_per_chirp(h);
         if ((rc = upload(h->d_vel, h->vel, nterm * 3, s)) != TDC_OK) return rc;
         if ((rc = upload(h->d_att, h->att, nterm * 4, s)) != TDC_OK) return rc;
         TDC_HIP(h->d_rec.ensure(sizeof(tdc::DopRec) * nterm));
@@ -778,9 +799,9 @@ int tdc_run(tdc_processor *h, int iterations)
         if (h->run_form == TDC_FORM_PLAIN) {
             float2 *dst = g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>();
             if (h->doppler) {
-                TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(), h->d_grid.as<float>(),
-                                            g, h->dp, dst, h->d_counters.as<unsigned long long>(), s,
-                                            &wgs));
+                const auto launch = h->dop_chirp ? tdc::launch_doppler_chirp : tdc::launch_doppler;
+                TDC_HIP(launch(h->ppl, h->d_rec.as<tdc::DopRec>(), h->d_grid.as<float>(), g, h->dp,
+                               dst, h->d_counters.as<unsigned long long>(), s, &wgs));
             } else {
                 TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), h->d_grid.as<float>(),
                                           g, dst, s, &wgs));
@@ -825,6 +846,7 @@ int tdc_run(tdc_processor *h, int iterations)
     st.pixels_per_lane = h->ppl;
     st.workgroups = wgs;
     st.engine = h->run_form == TDC_FORM_PRECISION ? h->run_engine : 0;
+    st.window_per_chirp = (h->doppler && h->dop_chirp) ? 1 : 0;
     if (h->doppler && g.npx > 0) {
         unsigned long long c[2] = {0, 0};
         TDC_HIP(hipMemcpy(c, h->d_counters.p, sizeof c, hipMemcpyDeviceToHost));

@@ -89,6 +89,10 @@ hipError_t launch_plain(int ppl, const PlainRec *rec, const float *grid, const G
 hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const Geom &g,
                           const DopParams &dp, float2 *out, unsigned long long *counters,
                           hipStream_t s, int *workgroups);
+// Doppler window evaluated per (pixel, chirp): antenna state constant within each chirp.
+hipError_t launch_doppler_chirp(int ppl, const DopRec *rec, const float *grid, const Geom &g,
+                                const DopParams &dp, float2 *out, unsigned long long *counters,
+                                hipStream_t s, int *workgroups);
 hipError_t launch_finalize_f32(const float2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
 

@@ -290,6 +290,117 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
     if (skipped) atomicAdd(&counters[1], skipped);
 }
 
+// The same sum when the antenna position, velocity and attitude are constant
+// within every chirp (stop-and-hop, checked on the host): every term of a
+// chirp then has the same Doppler window value for a pixel, so the window is
+// evaluated once per (pixel, chirp) from the chirp's first record and scales
+// the chirp's sum, sum_m w raw exp(j theta) = w sum_m raw exp(j theta); a
+// chirp that no pixel of the wave has in band is skipped.
+template <int P>
+__global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
+    const DopRec *__restrict__ rec, const float *__restrict__ grid, Geom g, DopParams dp,
+    float2 *__restrict__ out, unsigned long long *__restrict__ counters)
+{
+    int tile, split;
+    map_block(g, tile, split);
+    if (tile >= g.ntiles) return;
+    const int n0 = split * g.cps;
+    const int n1 = min(g.npri, n0 + g.cps);
+
+    float px[P], py[P], pz[P], ar[P], ai[P];
+    int pix[P];
+    bool live[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
+        live[j] = pix[j] < g.npx;
+        const int q = min(pix[j], g.npx - 1);
+        px[j] = grid[3 * q];
+        py[j] = grid[3 * q + 1];
+        pz[j] = grid[3 * q + 2];
+        ar[j] = 0.0F;
+        ai[j] = 0.0F;
+    }
+    unsigned long long processed = 0, skipped = 0;
+    const cptr<DopRec> recc = as_const(rec);
+
+    int n = n0;
+    while (n < n1) {
+        const int chunk = n / kChunk;
+        const int nend = min(n1, (chunk + 1) * kChunk);
+        bool on[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            on[j] = chunk_processed(rec, g, dp, chunk, px[j], py[j], pz[j]);
+            if (live[j] && n == chunk * kChunk) {
+                if (on[j]) ++processed;
+                else ++skipped;
+            }
+        }
+        for (int c = n; c < nend; ++c) {
+            const cptr<DopRec> r = recc + static_cast<size_t>(c) * g.ns;
+            const DopRec t0 = r[0];
+            float fac[P];
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                // the window of windowing.h:29-58 for this chirp (see doppler_kernel)
+                const float dx = px[j] - t0.x;
+                const float dy = py[j] - t0.y;
+                const float dz = pz[j] - t0.z;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float rs = __builtin_amdgcn_rsqf(r2);
+                const float fd = ((t0.vx * dx) + (t0.vy * dy) + (t0.vz * dz)) * rs;
+                const float delta = fd - t0.fc;
+                const bool inband = on[j] && (fabsf(delta) <= dp.half_bw);
+                const float wa =
+                    dp.taper ? fmaf(1.0F - dp.az_a, cos_rev(delta * dp.inv_bw), dp.az_a) : 1.0F;
+                fac[j] = inband ? wa : 0.0F;
+                any |= inband;
+            }
+            if (__ballot(any) == 0ULL) continue;
+            float tr[P], ti[P];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                tr[j] = 0.0F;
+                ti[j] = 0.0F;
+            }
+#pragma unroll 2
+            for (int m = 0; m < g.ns; ++m) {
+                const DopRec t = r[m];
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const float dx = px[j] - t.x;
+                    const float dy = py[j] - t.y;
+                    const float dz = pz[j] - t.z;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float th = __builtin_amdgcn_fractf(__builtin_amdgcn_sqrtf(r2) * t.k);
+                    const float sn = sin_rev(th);
+                    const float cs = cos_rev(th);
+                    tr[j] = fmaf(t.rr, cs, tr[j]);
+                    tr[j] = fmaf(-t.ri, sn, tr[j]);
+                    ti[j] = fmaf(t.rr, sn, ti[j]);
+                    ti[j] = fmaf(t.ri, cs, ti[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                ar[j] = fmaf(fac[j], tr[j], ar[j]);
+                ai[j] = fmaf(fac[j], ti[j], ai[j]);
+            }
+        }
+        n = nend;
+    }
+
+    float2 *o = out + static_cast<size_t>(split) * g.npx;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (live[j]) o[pix[j]] = make_float2(ar[j], ai[j]);
+    }
+    if (processed) atomicAdd(&counters[0], processed);
+    if (skipped) atomicAdd(&counters[1], skipped);
+}
+
 // Sum of the per-split partial images in split order.
 __global__ __launch_bounds__(kBlock) void finalize_f32_kernel(const float2 *__restrict__ partial,
                                                               int nsplit, int npx,
@@ -546,6 +657,31 @@ hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const G
     case 4:
         hipLaunchKernelGGL(doppler_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp, out,
                            counters);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_doppler_chirp(int ppl, const DopRec *rec, const float *grid, const Geom &g,
+                                const DopParams &dp, float2 *out, unsigned long long *counters,
+                                hipStream_t s, int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    switch (ppl) {
+    case 1:
+        hipLaunchKernelGGL(doppler_chirp_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp,
+                           out, counters);
+        break;
+    case 2:
+        hipLaunchKernelGGL(doppler_chirp_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp,
+                           out, counters);
+        break;
+    case 4:
+        hipLaunchKernelGGL(doppler_chirp_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp,
+                           out, counters);
         break;
     default:
         return hipErrorInvalidValue;

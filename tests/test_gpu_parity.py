@@ -394,16 +394,33 @@ def doppler_dict(d, center, bw, centroid, taper):
                 center=center, bandwidth=bw, centroid=centroid, taper=taper)
 
 
+def moving_within_chirp(d, s):
+    """Per-sample positions: the antenna keeps moving along the track during each
+    chirp (float32, as the reference API takes them)."""
+    d = dict(d)
+    vel = d["velocity"].astype(np.float64)
+    t = s.sample_times[None, :, None]
+    d["position"] = np.ascontiguousarray((d["position"].astype(np.float64) + vel * t)
+                                         .astype(np.float32))
+    return d
+
+
+@pytest.mark.parametrize("path", ["per_chirp", "per_term"])
 @pytest.mark.parametrize("centroid,taper,center,bw", [
     (True, True, 0.0, 8e3),
     (False, False, 2e3, 6e3),
     (True, False, 0.0, 12e3),
 ])
-def test_doppler_window_and_skip(tdc, orc, centroid, taper, center, bw):
+def test_doppler_window_and_skip(tdc, orc, centroid, taper, center, bw, path):
+    """Stop-and-hop data take the per-(pixel, chirp) window; antenna motion within
+    the chirps takes the per-term window."""
     s = doppler_scene()
     d = s.dropin()
+    if path == "per_term":
+        d = moving_within_chirp(d, s)
     img, st = form(tdc, d, center=center, bw=bw, centroid=centroid, taper=taper)
     assert st["doppler_active"]
+    assert st["window_per_chirp"] == (path == "per_chirp")
     dop = doppler_dict(d, center, bw, centroid, taper)
     ref = oracle_dropin(orc, d, doppler=dop)
     # skip counts against the oracle's own skip test
@@ -426,6 +443,23 @@ def test_doppler_matches_reference_method(tdc, orc):
     img, _ = form(tdc, d, center=0.0, bw=8e3, centroid=True, taper=True, rw=True)
     rm = orc.refmethod_f32(d, 0.0, 8e3, True, True, True)
     assert orc.rel_l2(img, rm) < 5e-3
+
+
+def test_doppler_window_paths_agree(tdc, orc):
+    """The per-chirp and per-term windows on the same terms: one non-probe sample of
+    every chirp moved by one float32 ulp sends the data down the per-term path."""
+    s = doppler_scene()
+    d = s.dropin()
+    a, sa = form(tdc, d, center=0.0, bw=8e3, centroid=True, taper=True)
+    d2 = dict(d)
+    d2["position"] = d["position"].copy()
+    d2["position"][:, 1, 0] = np.nextafter(d2["position"][:, 1, 0], np.float32(np.inf))
+    b, sb = form(tdc, d2, center=0.0, bw=8e3, centroid=True, taper=True)
+    assert sa["window_per_chirp"] and not sb["window_per_chirp"]
+    assert sa["skipped_chunks"] == sb["skipped_chunks"] > 0
+    e = orc.rel_l2(a, b)
+    print(f"per-chirp vs per-term window: {e:.3e}")
+    assert e < 1e-4
 
 
 def test_doppler_trivial_window_uses_plain_kernel(tdc, orc):
