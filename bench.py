@@ -108,7 +108,7 @@ def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=No
                 terms=st["terms"],
                 splits=st["chirp_splits"], ppl=st["pixels_per_lane"], engine=st["engine"],
                 doppler_active=st["doppler_active"], skipped_chunks=st["skipped_chunks"],
-                processed_chunks=st["processed_chunks"],
+                processed_chunks=st["processed_chunks"], window_per_chirp=st["window_per_chirp"],
                 workgroups=st["workgroups"], image=img, keep=keep)
 
 
@@ -416,7 +416,9 @@ def main():
                     gterms_e2e_pcie=round(r["gterms_e2e"], 3),
                     chirp_splits=r["splits"], engine=r["engine"],
                     **({"skipped_chunks": r["skipped_chunks"],
-                        "processed_chunks": r["processed_chunks"]} if r["doppler_active"] else {}),
+                        "processed_chunks": r["processed_chunks"],
+                        "window_per_chirp": bool(r["window_per_chirp"])}
+                       if r["doppler_active"] else {}),
                     **({k: float(f"{x:.4g}") for k, x in acc.get(v, {}).items()}))
             for v, r in results.items()
         },

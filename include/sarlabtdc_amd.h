@@ -73,7 +73,7 @@ typedef struct tdc_stats {
     double terms;           /* pixels * chirps * samples of the last run */
     int32_t form;           /* TDC_FORM_* actually run */
     int32_t raw_storage;    /* TDC_RAW_* actually run */
-    int32_t doppler_active; /* 1 when the per-term Doppler window / skip test ran */
+    int32_t doppler_active; /* 1 when the Doppler window / skip test ran */
     int32_t chirp_splits;   /* chirp ranges the pixels were split over */
     int32_t pixels_per_lane;
     int32_t workgroups;
