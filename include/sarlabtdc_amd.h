@@ -66,6 +66,14 @@ extern "C" {
 
 typedef struct tdc_processor tdc_processor;
 
+/* tdc_stats.per_chirp bits. RANGE: the antenna position was the same for all
+ * samples of every chirp, so |pos - pixel| was computed once per (pixel,
+ * chirp); the image is bit-identical to the per-term range. WINDOW: velocity
+ * and attitude were constant too, so the Doppler window was evaluated once
+ * per (pixel, chirp) and scaled the chirp's sum. */
+#define TDC_PER_CHIRP_RANGE 1
+#define TDC_PER_CHIRP_WINDOW 2
+
 typedef struct tdc_stats {
     double kernel_ms;       /* device time of the last run's hot-path kernels (HIP events) */
     double main_kernel_ms;  /* device time of the correlate-sum kernel alone */
@@ -80,8 +88,9 @@ typedef struct tdc_stats {
     int64_t skipped_chunks;   /* (pixel, chunk) pairs skipped; only counted when doppler_active */
     int64_t processed_chunks; /* (pixel, chunk) pairs processed */
     int32_t engine;           /* TDC_ENGINE_* actually run (precision form), else 0 */
-    int32_t window_per_chirp; /* 1 when the Doppler window ran once per (pixel, chirp): the
-                                 antenna state was constant within every chirp */
+    int32_t per_chirp;        /* plain form, TDC_PER_CHIRP_* bits: what ran once per (pixel,
+                                 chirp) because the inputs it depends on were constant
+                                 within every chirp (stop-and-hop) */
 } tdc_stats;
 
 /* Message for the last failing call on this thread ("" when none). */

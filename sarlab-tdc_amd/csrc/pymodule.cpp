@@ -235,7 +235,8 @@ public:
         d["workgroups"] = st.workgroups;
         d["skipped_chunks"] = st.skipped_chunks;
         d["processed_chunks"] = st.processed_chunks;
-        d["window_per_chirp"] = static_cast<bool>(st.window_per_chirp);
+        d["range_per_chirp"] = (st.per_chirp & TDC_PER_CHIRP_RANGE) != 0;
+        d["window_per_chirp"] = (st.per_chirp & TDC_PER_CHIRP_WINDOW) != 0;
         d["engine"] = st.engine == TDC_ENGINE_MATRIX ? "matrix" : (st.engine == TDC_ENGINE_VECTOR ? "vector" : "none");
         return d;
     }

@@ -188,7 +188,7 @@ struct tdc_processor {
 
     // device buffers
     DevBuf d_raw, d_pos, d_vel, d_att, d_kt, d_wr, d_rec, d_grid, d_part, d_img, d_cpos,
-        d_kanchor, d_counters, d_atab, d_ascale;
+        d_kanchor, d_counters, d_atab, d_ascale, d_chirp;
 
     // prepared state
     bool prepared = false;
@@ -196,7 +196,7 @@ struct tdc_processor {
     int run_storage = TDC_RAW_C64;
     int run_engine = 0;
     bool doppler = false;
-    bool dop_chirp = false; // Doppler window evaluated per (pixel, chirp)
+    bool chirp_path = false; // plain form: range (and window) once per (pixel, chirp)
     tdc::Geom geom{};
     tdc::DopParams dp{};
     int ppl = 1;
@@ -248,29 +248,40 @@ void make_geom(tdc::Geom &g, int npx, int npri, int ns, int ppl, int nsplit)
     }
 }
 
+// True when `v` (`width` floats per term) is the same for all samples of
+// every chirp, bit for bit.
+bool constant_per_chirp(const float *v, int width, int npri, int ns)
+{
+    const size_t w = static_cast<size_t>(width);
+    for (int n = 0; n < npri; ++n) {
+        const float *first = v + w * static_cast<size_t>(n) * ns;
+        for (int m = 1; m < ns; ++m) {
+            if (std::memcmp(first + w * m, first, w * sizeof(float)) != 0) return false;
+        }
+    }
+    return true;
+}
+
+// Stop-and-hop input: with the position constant within every chirp the range
+// of a term depends only on its pixel and chirp; with velocity and attitude
+// constant too, so does its Doppler window.
+bool position_constant_per_chirp(const tdc_processor *h)
+{
+    return constant_per_chirp(h->pos, 3, h->npri, h->ns);
+}
+
+bool antenna_constant_per_chirp(const tdc_processor *h)
+{
+    return position_constant_per_chirp(h) && constant_per_chirp(h->vel, 3, h->npri, h->ns) &&
+           constant_per_chirp(h->att, 4, h->npri, h->ns);
+}
+
 // True when every term's Doppler window value is exactly 1 and no (pixel,
 // chunk) can be skipped, so the per-term window and the skip test can be
 // left out without changing any result. Uses |f_D| <= (2/lambda)|v| and
 // |f_c| <= (2/lambda)|v| |q|^2 |b| (a rotation by a non-unit quaternion
 // scales by |q|^2), and excludes pixels that coincide with an antenna
 // position (0/0 there makes the reference's window NaN, hence 0).
-// True when every chirp's antenna position, velocity and attitude are the same
-// for all of its samples (bit for bit): the Doppler window of a term then
-// depends only on its pixel and chirp.
-bool antenna_constant_per_chirp(const tdc_processor *h)
-{
-    const int ns = h->ns;
-    for (int n = 0; n < h->npri; ++n) {
-        const size_t base = static_cast<size_t>(n) * ns;
-        for (int m = 1; m < ns; ++m) {
-            if (std::memcmp(h->pos + 3 * (base + m), h->pos + 3 * base, 12) != 0 ||
-                std::memcmp(h->vel + 3 * (base + m), h->vel + 3 * base, 12) != 0 ||
-                std::memcmp(h->att + 4 * (base + m), h->att + 4 * base, 16) != 0)
-                return false;
-        }
-    }
-    return true;
-}
 
 bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool centroid,
                            bool taper)
@@ -338,7 +349,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         kt[m] = static_cast<float>(2.0 * f / kSpeedOfLight);
     }
     h->doppler = !doppler_window_is_one(h, dop_center, dop_bw, centroid, taper);
-    h->dop_chirp = false;
+    h->chirp_path = false;
     hipStream_t s = h->stream;
     int rc;
     if ((rc = upload(h->d_raw, h->raw, nterm * 2, s)) != TDC_OK) return rc;
@@ -349,16 +360,28 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     if (h->doppler) {
         if (!h->vel || !h->att)
             return fail(TDC_ERR_INVALID_ARGUMENT, "the Doppler window needs velocity and attitude");
-        h->dop_chirp = antenna_constant_per_chirp(h);
+        h->chirp_path = antenna_constant_per_chirp(h);
         if ((rc = upload(h->d_vel, h->vel, nterm * 3, s)) != TDC_OK) return rc;
         if ((rc = upload(h->d_att, h->att, nterm * 4, s)) != TDC_OK) return rc;
-        TDC_HIP(h->d_rec.ensure(sizeof(tdc::DopRec) * nterm));
         const float tol = 2.0F / h->lambda_c; // windowing.h:21, :33
-        TDC_HIP(tdc::launch_stage_doppler(h->d_raw.as<float>(), h->d_pos.as<float>(),
-                                          h->d_vel.as<float>(), h->d_att.as<float>(),
-                                          h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns, tol,
-                                          h->bore, centroid ? 1 : 0, dop_center,
-                                          h->d_rec.as<tdc::DopRec>(), s));
+        if (h->chirp_path) {
+            TDC_HIP(h->d_rec.ensure(sizeof(float4) * nterm));
+            TDC_HIP(h->d_chirp.ensure(sizeof(float4) * 2 * static_cast<size_t>(npri)));
+            TDC_HIP(tdc::launch_stage_terms(h->d_raw.as<float>(), h->d_kt.as<float>(),
+                                            h->d_wr.as<float>(), npri, ns, h->d_rec.as<float4>(),
+                                            s));
+            TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), h->d_vel.as<float>(),
+                                             h->d_att.as<float>(), npri, ns, tol, h->bore,
+                                             centroid ? 1 : 0, dop_center, h->d_chirp.as<float4>(),
+                                             h->d_chirp.as<float4>() + npri, s));
+        } else {
+            TDC_HIP(h->d_rec.ensure(sizeof(tdc::DopRec) * nterm));
+            TDC_HIP(tdc::launch_stage_doppler(h->d_raw.as<float>(), h->d_pos.as<float>(),
+                                              h->d_vel.as<float>(), h->d_att.as<float>(),
+                                              h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
+                                              tol, h->bore, centroid ? 1 : 0, dop_center,
+                                              h->d_rec.as<tdc::DopRec>(), s));
+        }
         h->dp.half_bw = dop_bw / 2.0F;
         h->dp.inv_bw = 1.0F / dop_bw;
         h->dp.az_a = static_cast<float>(25.0 / 46.0); // tuning.h:38
@@ -366,15 +389,23 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         h->dp.centroid = centroid ? 1 : 0;
         h->dp.fc_fixed = dop_center;
         TDC_HIP(h->d_counters.ensure(2 * sizeof(unsigned long long)));
+    } else if ((h->chirp_path = position_constant_per_chirp(h))) {
+        TDC_HIP(h->d_rec.ensure(sizeof(float4) * nterm));
+        TDC_HIP(h->d_chirp.ensure(sizeof(float4) * static_cast<size_t>(npri)));
+        TDC_HIP(tdc::launch_stage_terms(h->d_raw.as<float>(), h->d_kt.as<float>(),
+                                        h->d_wr.as<float>(), npri, ns, h->d_rec.as<float4>(), s));
+        TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), nullptr, nullptr, npri, ns, 0.0F,
+                                         nullptr, 0, 0.0F, h->d_chirp.as<float4>(), nullptr, s));
     } else {
         TDC_HIP(h->d_rec.ensure(sizeof(tdc::PlainRec) * nterm));
         TDC_HIP(tdc::launch_stage_plain(h->d_raw.as<float>(), h->d_pos.as<float>(),
                                         h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
                                         h->d_rec.as<tdc::PlainRec>(), s));
     }
-    // measured best: two pixels per lane for the LDS-staged plain records, one
-    // for the Doppler kernel
-    h->ppl = h->force_ppl ? h->force_ppl : (h->doppler ? 1 : 2);
+    // measured best on 1024 chirps x 512 samples x 512^2 pixels: two pixels per
+    // lane for the per-term plain records, one for the per-chirp range
+    // (30.5 vs 31.4 ms at two) and for the Doppler kernels
+    h->ppl = h->force_ppl ? h->force_ppl : ((h->doppler || h->chirp_path) ? 1 : 2);
     if (h->ppl > 4)
         return fail(TDC_ERR_INVALID_ARGUMENT, "the plain form runs 1, 2 or 4 pixels per lane");
     const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
@@ -798,10 +829,19 @@ int tdc_run(tdc_processor *h, int iterations)
         TDC_HIP(hipEventRecord(h->ev_a, s));
         if (h->run_form == TDC_FORM_PLAIN) {
             float2 *dst = g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>();
-            if (h->doppler) {
-                const auto launch = h->dop_chirp ? tdc::launch_doppler_chirp : tdc::launch_doppler;
-                TDC_HIP(launch(h->ppl, h->d_rec.as<tdc::DopRec>(), h->d_grid.as<float>(), g, h->dp,
-                               dst, h->d_counters.as<unsigned long long>(), s, &wgs));
+            const float4 *cpos = h->d_chirp.as<float4>();
+            if (h->doppler && h->chirp_path) {
+                TDC_HIP(tdc::launch_doppler_chirp(h->ppl, h->d_rec.as<float4>(), cpos,
+                                                  cpos + g.npri, h->d_grid.as<float>(), g, h->dp,
+                                                  dst, h->d_counters.as<unsigned long long>(), s,
+                                                  &wgs));
+            } else if (h->doppler) {
+                TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(),
+                                            h->d_grid.as<float>(), g, h->dp, dst,
+                                            h->d_counters.as<unsigned long long>(), s, &wgs));
+            } else if (h->chirp_path) {
+                TDC_HIP(tdc::launch_range(h->ppl, h->d_rec.as<float4>(), cpos, h->d_grid.as<float>(),
+                                          g, dst, s, &wgs));
             } else {
                 TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), h->d_grid.as<float>(),
                                           g, dst, s, &wgs));
@@ -846,7 +886,9 @@ int tdc_run(tdc_processor *h, int iterations)
     st.pixels_per_lane = h->ppl;
     st.workgroups = wgs;
     st.engine = h->run_form == TDC_FORM_PRECISION ? h->run_engine : 0;
-    st.window_per_chirp = (h->doppler && h->dop_chirp) ? 1 : 0;
+    st.per_chirp = 0;
+    if (h->run_form == TDC_FORM_PLAIN && h->chirp_path)
+        st.per_chirp = TDC_PER_CHIRP_RANGE | (h->doppler ? TDC_PER_CHIRP_WINDOW : 0);
     if (h->doppler && g.npx > 0) {
         unsigned long long c[2] = {0, 0};
         TDC_HIP(hipMemcpy(c, h->d_counters.p, sizeof c, hipMemcpyDeviceToHost));

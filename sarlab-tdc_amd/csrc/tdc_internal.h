@@ -51,6 +51,12 @@ struct alignas(16) DopRec {
     float pad0, pad1;
 };
 
+// Records of the plain form when the antenna position is constant within
+// every chirp (stop-and-hop): one float4 (rr, ri, k, 0) per term, the raw
+// sample times the range window and k_m as in PlainRec, plus one float4 per
+// chirp: its position (x, y, z, 0) and, with the Doppler window on, its
+// velocity pre-scaled by 2 / lambda_c and window centre (vx, vy, vz, fc).
+
 // Chirp phase centre of the precision form (FP64, padded to 32 bytes).
 struct alignas(32) ChirpPos {
     double x, y, z, pad;
@@ -89,8 +95,21 @@ hipError_t launch_plain(int ppl, const PlainRec *rec, const float *grid, const G
 hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const Geom &g,
                           const DopParams &dp, float2 *out, unsigned long long *counters,
                           hipStream_t s, int *workgroups);
-// Doppler window evaluated per (pixel, chirp): antenna state constant within each chirp.
-hipError_t launch_doppler_chirp(int ppl, const DopRec *rec, const float *grid, const Geom &g,
+// Stop-and-hop records: terms (rr, ri, k, 0) and per chirp the position of
+// its first sample; with `vel` non-null also (2/lambda_c) v and the window
+// centre of its first sample (the values stage_doppler writes for it).
+hipError_t launch_stage_terms(const float *raw, const float *kt, const float *wr, int npri, int ns,
+                              float4 *out, hipStream_t s);
+hipError_t launch_stage_chirps(const float *pos, const float *vel, const float *att, int npri,
+                               int ns, float two_over_lambda, const float *bore, int centroid,
+                               float fc_fixed, float4 *cpos, float4 *cvel, hipStream_t s);
+// Range evaluated once per (pixel, chirp): position constant within each chirp.
+hipError_t launch_range(int ppl, const float4 *rec, const float4 *cpos, const float *grid,
+                        const Geom &g, float2 *out, hipStream_t s, int *workgroups);
+// Range and Doppler window evaluated once per (pixel, chirp): position,
+// velocity and attitude constant within each chirp.
+hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
+                                const float4 *cvel, const float *grid, const Geom &g,
                                 const DopParams &dp, float2 *out, unsigned long long *counters,
                                 hipStream_t s, int *workgroups);
 hipError_t launch_finalize_f32(const float2 *partial, int nsplit, int npx, float2 *img,

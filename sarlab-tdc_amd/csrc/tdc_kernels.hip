@@ -76,6 +76,17 @@ __device__ __forceinline__ float3 q_rot(float4 q, float3 v)
     return make_float3(r.y, r.z, r.w);
 }
 
+// Doppler window centre of one term: the centroid (2/lambda_c) v . a with the
+// antenna pointing a = q b q* (windowing.h:19-24, gpumath.h:77-87), or the
+// fixed centre.
+__device__ __forceinline__ float window_centre(float3 vv, float4 att, float3 bore,
+                                               float two_over_lambda, int centroid, float fc_fixed)
+{
+    if (!centroid) return fc_fixed;
+    const float3 ant = q_rot(att, bore);
+    return two_over_lambda * ((vv.x * ant.x) + (vv.y * ant.y) + (vv.z * ant.z));
+}
+
 // ---------------------------------------------------------------------------
 // Staging: raw * range window, positions and k_m packed per term.
 // ---------------------------------------------------------------------------
@@ -124,16 +135,48 @@ __global__ __launch_bounds__(kBlock) void stage_doppler_kernel(
         r.vx = two_over_lambda * vv.x;
         r.vy = two_over_lambda * vv.y;
         r.vz = two_over_lambda * vv.z;
-        if (centroid) {
-            // windowing.h:19-24 with the antenna pointing of gpumath.h:77-87
-            const float3 ant = q_rot(att[e], bore);
-            r.fc = two_over_lambda * ((vv.x * ant.x) + (vv.y * ant.y) + (vv.z * ant.z));
-        } else {
-            r.fc = fc_fixed;
-        }
+        r.fc = window_centre(vv, att[e], bore, two_over_lambda, centroid, fc_fixed);
         r.pad0 = 0.0F;
         r.pad1 = 0.0F;
         out[e] = r;
+    }
+}
+
+// Stop-and-hop terms: (raw * range window, k_m), the same products as
+// stage_plain_kernel.
+__global__ __launch_bounds__(kBlock) void stage_terms_kernel(const float2 *__restrict__ raw,
+                                                             const float *__restrict__ kt,
+                                                             const float *__restrict__ wr,
+                                                             int npri, int ns,
+                                                             float4 *__restrict__ out)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    for (size_t e = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; e < total;
+         e += static_cast<size_t>(gridDim.x) * kBlock) {
+        const int m = static_cast<int>(e % ns);
+        const float2 v = raw[e];
+        out[e] = make_float4(v.x * wr[m], v.y * wr[m], kt[m], 0.0F);
+    }
+}
+
+// Per chirp: the position of its first sample and, when `vel` is given, the
+// velocity and window centre of stage_doppler_kernel for that sample.
+__global__ __launch_bounds__(kBlock) void stage_chirps_kernel(
+    const float *__restrict__ pos, const float *__restrict__ vel, const float4 *__restrict__ att,
+    int npri, int ns, float two_over_lambda, float3 bore, int centroid, float fc_fixed,
+    float4 *__restrict__ cpos, float4 *__restrict__ cvel)
+{
+    for (int n = static_cast<int>(blockIdx.x * kBlock + threadIdx.x); n < npri;
+         n += static_cast<int>(gridDim.x) * kBlock) {
+        const size_t e = static_cast<size_t>(n) * ns;
+        cpos[n] = make_float4(pos[3 * e], pos[3 * e + 1], pos[3 * e + 2], 0.0F);
+        if (vel) {
+            const float3 vv = make_float3(vel[3 * e], vel[3 * e + 1], vel[3 * e + 2]);
+            cvel[n] = make_float4(two_over_lambda * vv.x, two_over_lambda * vv.y,
+                                  two_over_lambda * vv.z,
+                                  window_centre(vv, att[e], bore, two_over_lambda, centroid,
+                                                fc_fixed));
+        }
     }
 }
 
@@ -290,17 +333,69 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
     if (skipped) atomicAdd(&counters[1], skipped);
 }
 
-// The same sum when the antenna position, velocity and attitude are constant
-// within every chirp (stop-and-hop, checked on the host): every term of a
-// chirp then has the same Doppler window value for a pixel, so the window is
-// evaluated once per (pixel, chirp) from the chirp's first record and scales
-// the chirp's sum, sum_m w raw exp(j theta) = w sum_m raw exp(j theta); a
-// chirp that no pixel of the wave has in band is skipped.
+// Plain form on stop-and-hop input (position constant within every chirp):
+// range_tile, bit-identical to plain_kernel on the same input.
+template <int P>
+__global__ __launch_bounds__(kBlock) void range_kernel(const float4 *__restrict__ rec,
+                                                       const float4 *__restrict__ cpos,
+                                                       const float *__restrict__ grid, Geom g,
+                                                       float2 *__restrict__ out)
+{
+    int tile, split;
+    map_block(g, tile, split);
+    if (tile >= g.ntiles) return;
+    const int n0 = split * g.cps;
+    const int n1 = min(g.npri, n0 + g.cps);
+    range_tile<P>(rec, cpos, grid, g.npx, g.ns, n0, n1, tile * P * kBlock,
+                  out + static_cast<size_t>(split) * g.npx);
+}
+
+// chunk_processed on stop-and-hop input: the three probe samples of a row
+// hold the same position, velocity and centre, so one evaluation per row
+// gives the same minimum (a NaN in the first row still makes it NaN).
+__device__ __forceinline__ bool chunk_processed_chirp(const float4 *__restrict__ cpos,
+                                                      const float4 *__restrict__ cvel,
+                                                      const Geom &g, const DopParams &dp,
+                                                      int chunk, float px, float py, float pz)
+{
+    const int valid = min(kChunk, g.npri - chunk * kChunk);
+    float mn = 0.0F;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int row = max(i * (kChunk / 2) - 1, 0);
+        int pri = -1;
+        if (row < valid) pri = chunk * kChunk + row;
+        else if (chunk > 0) pri = (chunk - 1) * kChunk + row;
+        float4 c = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+        float4 v = make_float4(0.0F, 0.0F, 0.0F, dp.centroid ? 0.0F : dp.fc_fixed);
+        if (pri >= 0) {
+            c = cpos[pri];
+            v = cvel[pri];
+        }
+        const float dx = px - c.x, dy = py - c.y, dz = pz - c.z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float fd = ((v.x * dx) + (v.y * dy) + (v.z * dz)) / __builtin_amdgcn_sqrtf(r2);
+        const float mag = fabsf(fd - v.w);
+        if (i == 0) mn = mag;
+        else if (mag < mn) mn = mag;
+    }
+    return mn <= dp.half_bw;
+}
+
+// Doppler window on stop-and-hop input (position, velocity and attitude
+// constant within every chirp, checked on the host): every term of a chirp
+// then has the same window value for a pixel, so the window is evaluated once
+// per (pixel, chirp) and scales the chirp's sum,
+// sum_m w raw exp(j theta) = w sum_m raw exp(j theta), with the range of
+// range_tile. A chirp that no pixel of the workgroup has in band is skipped
+// (neither staged nor summed); a wave with none in band only stages.
 template <int P>
 __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
-    const DopRec *__restrict__ rec, const float *__restrict__ grid, Geom g, DopParams dp,
+    const float4 *__restrict__ rec, const float4 *__restrict__ cpos,
+    const float4 *__restrict__ cvel, const float *__restrict__ grid, Geom g, DopParams dp,
     float2 *__restrict__ out, unsigned long long *__restrict__ counters)
 {
+    __shared__ float4 s_term[kTermStage];
     int tile, split;
     map_block(g, tile, split);
     if (tile >= g.ntiles) return;
@@ -322,7 +417,6 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
         ai[j] = 0.0F;
     }
     unsigned long long processed = 0, skipped = 0;
-    const cptr<DopRec> recc = as_const(rec);
 
     int n = n0;
     while (n < n1) {
@@ -331,58 +425,44 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
         bool on[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            on[j] = chunk_processed(rec, g, dp, chunk, px[j], py[j], pz[j]);
+            on[j] = chunk_processed_chirp(cpos, cvel, g, dp, chunk, px[j], py[j], pz[j]);
             if (live[j] && n == chunk * kChunk) {
                 if (on[j]) ++processed;
                 else ++skipped;
             }
         }
         for (int c = n; c < nend; ++c) {
-            const cptr<DopRec> r = recc + static_cast<size_t>(c) * g.ns;
-            const DopRec t0 = r[0];
-            float fac[P];
+            const float4 cp = as_const(cpos)[c];
+            const float4 cv = as_const(cvel)[c];
+            float rng[P], fac[P];
             bool any = false;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 // the window of windowing.h:29-58 for this chirp (see doppler_kernel)
-                const float dx = px[j] - t0.x;
-                const float dy = py[j] - t0.y;
-                const float dz = pz[j] - t0.z;
+                const float dx = px[j] - cp.x;
+                const float dy = py[j] - cp.y;
+                const float dz = pz[j] - cp.z;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                 const float rs = __builtin_amdgcn_rsqf(r2);
-                const float fd = ((t0.vx * dx) + (t0.vy * dy) + (t0.vz * dz)) * rs;
-                const float delta = fd - t0.fc;
+                const float fd = ((cv.x * dx) + (cv.y * dy) + (cv.z * dz)) * rs;
+                const float delta = fd - cv.w;
                 const bool inband = on[j] && (fabsf(delta) <= dp.half_bw);
                 const float wa =
                     dp.taper ? fmaf(1.0F - dp.az_a, cos_rev(delta * dp.inv_bw), dp.az_a) : 1.0F;
                 fac[j] = inband ? wa : 0.0F;
+                rng[j] = __builtin_amdgcn_sqrtf(r2);
                 any |= inband;
             }
-            if (__ballot(any) == 0ULL) continue;
+            const bool run = __ballot(any) != 0ULL;
+            if (!__syncthreads_or(run ? 1 : 0)) continue;
             float tr[P], ti[P];
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 tr[j] = 0.0F;
                 ti[j] = 0.0F;
             }
-#pragma unroll 2
-            for (int m = 0; m < g.ns; ++m) {
-                const DopRec t = r[m];
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const float dx = px[j] - t.x;
-                    const float dy = py[j] - t.y;
-                    const float dz = pz[j] - t.z;
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float th = __builtin_amdgcn_fractf(__builtin_amdgcn_sqrtf(r2) * t.k);
-                    const float sn = sin_rev(th);
-                    const float cs = cos_rev(th);
-                    tr[j] = fmaf(t.rr, cs, tr[j]);
-                    tr[j] = fmaf(-t.ri, sn, tr[j]);
-                    ti[j] = fmaf(t.rr, sn, ti[j]);
-                    ti[j] = fmaf(t.ri, cs, ti[j]);
-                }
-            }
+            chirp_terms<P>(rec + static_cast<size_t>(c) * g.ns, g.ns, s_term, rng, tr, ti, run);
+            if (!run) continue;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 ar[j] = fmaf(fac[j], tr[j], ar[j]);
@@ -664,7 +744,49 @@ hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const G
     return hipGetLastError();
 }
 
-hipError_t launch_doppler_chirp(int ppl, const DopRec *rec, const float *grid, const Geom &g,
+hipError_t launch_stage_terms(const float *raw, const float *kt, const float *wr, int npri, int ns,
+                              float4 *out, hipStream_t s)
+{
+    const size_t total = static_cast<size_t>(npri) * ns;
+    hipLaunchKernelGGL(stage_terms_kernel, dim3(stage_grid(total)), dim3(kBlock), 0, s,
+                       reinterpret_cast<const float2 *>(raw), kt, wr, npri, ns, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_chirps(const float *pos, const float *vel, const float *att, int npri,
+                               int ns, float two_over_lambda, const float *bore, int centroid,
+                               float fc_fixed, float4 *cpos, float4 *cvel, hipStream_t s)
+{
+    const float3 b = bore ? make_float3(bore[0], bore[1], bore[2]) : make_float3(0.0F, 0.0F, 0.0F);
+    hipLaunchKernelGGL(stage_chirps_kernel, dim3(stage_grid(static_cast<size_t>(npri))),
+                       dim3(kBlock), 0, s, pos, vel, reinterpret_cast<const float4 *>(att), npri,
+                       ns, two_over_lambda, b, centroid, fc_fixed, cpos, cvel);
+    return hipGetLastError();
+}
+
+hipError_t launch_range(int ppl, const float4 *rec, const float4 *cpos, const float *grid,
+                        const Geom &g, float2 *out, hipStream_t s, int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    switch (ppl) {
+    case 1:
+        hipLaunchKernelGGL(range_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, g, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL(range_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, g, out);
+        break;
+    case 4:
+        hipLaunchKernelGGL(range_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, g, out);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
+                                const float4 *cvel, const float *grid, const Geom &g,
                                 const DopParams &dp, float2 *out, unsigned long long *counters,
                                 hipStream_t s, int *workgroups)
 {
@@ -672,16 +794,16 @@ hipError_t launch_doppler_chirp(int ppl, const DopRec *rec, const float *grid, c
     *workgroups = nb;
     switch (ppl) {
     case 1:
-        hipLaunchKernelGGL(doppler_chirp_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp,
-                           out, counters);
+        hipLaunchKernelGGL(doppler_chirp_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, cvel,
+                           grid, g, dp, out, counters);
         break;
     case 2:
-        hipLaunchKernelGGL(doppler_chirp_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp,
-                           out, counters);
+        hipLaunchKernelGGL(doppler_chirp_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, cvel,
+                           grid, g, dp, out, counters);
         break;
     case 4:
-        hipLaunchKernelGGL(doppler_chirp_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp,
-                           out, counters);
+        hipLaunchKernelGGL(doppler_chirp_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, cvel,
+                           grid, g, dp, out, counters);
         break;
     default:
         return hipErrorInvalidValue;

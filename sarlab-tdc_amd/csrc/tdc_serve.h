@@ -8,7 +8,8 @@
 namespace tdc {
 
 // Pixels of one serving tile: one per thread of the workgroup (the matrix
-// engine's tile for the precision form, one pixel per lane for the plain form).
+// engine's tile for the precision form, one pixel per lane for the plain form's
+// per-chirp range tile).
 constexpr int kServeTilePixels = kBlock;
 // Capacity of one step.
 constexpr int kMaxStepTiles = 32768;
@@ -17,9 +18,10 @@ constexpr int kMaxJobSlots = 8192;
 
 // One admitted job as the kernels see it.
 struct alignas(16) ServeJob {
-    const void *raw;          // precision: matrix engine A tiles; plain: PlainRec records
+    const void *raw;          // precision: matrix engine A tiles; plain: (rr, ri, k, 0) terms
     const float *inv_scale;   // precision: per-chirp scale of the A tiles
     const ChirpPos *cpos;     // precision: FP64 phase centre per chirp
+    const float4 *cpos32;     // plain: FP32 phase centre per chirp (x, y, z, 0)
     const double *grid64;     // precision: FP64 pixel coordinates
     const float *grid32;      // plain: FP32 pixel coordinates
     const double *kanchor;    // precision: 2 (f0 + K t_a) / c per 64-sample block
@@ -54,6 +56,7 @@ hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsig
 hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,
                                  hipStream_t s);
 hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,
-                                    int npri, int ns, PlainRec *out, hipStream_t s);
+                                    int npri, int ns, float4 *terms, float4 *cpos32,
+                                    hipStream_t s);
 
 } // namespace tdc

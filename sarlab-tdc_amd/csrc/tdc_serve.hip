@@ -43,8 +43,8 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restric
         const int n0 = tl.split * j.cps;
         const int n1 = min(j.npri, n0 + j.cps);
         if (j.form == 0) {
-            plain_tile<1>(static_cast<const PlainRec *>(j.raw), j.grid32, j.npx, j.ns, n0, n1,
-                          tl.pix0,
+            range_tile<1>(static_cast<const float4 *>(j.raw), j.cpos32, j.grid32, j.npx, j.ns, n0,
+                          n1, tl.pix0,
                           static_cast<float2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx);
         } else {
             double2 *out = static_cast<double2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx;
@@ -93,28 +93,24 @@ __global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const ServeJob *
     }
 }
 
-// Plain-form records from per-chirp positions (stop-and-hop), for served jobs.
+// Plain-form records of a served job (stop-and-hop: one phase centre per
+// chirp): terms (rr, ri, k_m, 0) and the FP32 phase centre of every chirp.
 __global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(
     const float2 *__restrict__ raw, const ChirpPos *__restrict__ cpos, const float *__restrict__ kt,
-    int npri, int ns, PlainRec *__restrict__ out)
+    int npri, int ns, float4 *__restrict__ terms, float4 *__restrict__ cpos32)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
-    for (size_t e = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; e < total;
-         e += static_cast<size_t>(gridDim.x) * kBlock) {
-        const int n = static_cast<int>(e / ns);
+    const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
+    const size_t first = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x;
+    for (size_t e = first; e < total; e += stride) {
         const int m = static_cast<int>(e % ns);
         const float2 v = raw[e];
+        terms[e] = make_float4(v.x, v.y, kt[m], 0.0F);
+    }
+    for (size_t n = first; n < static_cast<size_t>(npri); n += stride) {
         const ChirpPos c = cpos[n];
-        PlainRec r;
-        r.rr = v.x;
-        r.ri = v.y;
-        r.x = static_cast<float>(c.x);
-        r.y = static_cast<float>(c.y);
-        r.z = static_cast<float>(c.z);
-        r.k = kt[m];
-        r.pad0 = 0.0F;
-        r.pad1 = 0.0F;
-        out[e] = r;
+        cpos32[n] = make_float4(static_cast<float>(c.x), static_cast<float>(c.y),
+                                static_cast<float>(c.z), 0.0F);
     }
 }
 
@@ -136,13 +132,14 @@ hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, i
 }
 
 hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,
-                                    int npri, int ns, PlainRec *out, hipStream_t s)
+                                    int npri, int ns, float4 *terms, float4 *cpos32,
+                                    hipStream_t s)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
     const size_t want = (total + kBlock - 1) / kBlock;
     const int grid = static_cast<int>(want < 4096 ? (want > 0 ? want : 1) : 4096);
     hipLaunchKernelGGL(stage_plain_chirp_kernel, dim3(grid), dim3(kBlock), 0, s,
-                       reinterpret_cast<const float2 *>(raw), cpos, kt, npri, ns, out);
+                       reinterpret_cast<const float2 *>(raw), cpos, kt, npri, ns, terms, cpos32);
     return hipGetLastError();
 }
 

@@ -126,6 +126,7 @@ struct Job {
     double terms = 0.0;
     // device arena layout: [upload region | staged records or raw | partials]
     size_t off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
+           off_cpos32 = 0,
            off_atab = 0, off_ascale = 0, off_part = 0, up_bytes = 0, bytes = 0;
     bool needs_stage = false;
     char *arena = nullptr;
@@ -238,8 +239,13 @@ void plan(Job &j)
     j.up_bytes = o;
     j.needs_stage = true;
     j.off_stage = o;
-    if (!prec) o += align256(sizeof(tdc::PlainRec) * nterm);
-    else if (d.raw_storage != TDC_RAW_C64) o += align256(4 * nterm);
+    if (!prec) {
+        o += align256(sizeof(float4) * nterm);
+        j.off_cpos32 = o;
+        o += align256(sizeof(float4) * static_cast<size_t>(d.n_pri));
+    } else if (d.raw_storage != TDC_RAW_C64) {
+        o += align256(4 * nterm);
+    }
     if (prec) {
         j.off_atab = o;
         o += align256(tdc::matrix_table_bytes(d.n_pri, d.n_samples, d.raw_storage));
@@ -355,6 +361,7 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
         sj.dk = dk;
     } else {
         sj.raw = j.arena + j.off_stage;
+        sj.cpos32 = reinterpret_cast<const float4 *>(j.arena + j.off_cpos32);
         sj.grid32 = reinterpret_cast<const float *>(j.arena + j.off_grid);
     }
     // the kernels read the descriptor from pinned memory; it is written before
@@ -387,7 +394,8 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
         reinterpret_cast<const float *>(j.arena),
         reinterpret_cast<const tdc::ChirpPos *>(j.arena + j.off_cpos),
         reinterpret_cast<const float *>(j.arena + j.off_kt), d.n_pri, d.n_samples,
-        reinterpret_cast<tdc::PlainRec *>(j.arena + j.off_stage), ss);
+        reinterpret_cast<float4 *>(j.arena + j.off_stage),
+        reinterpret_cast<float4 *>(j.arena + j.off_cpos32), ss);
 }
 
 bool ensure_graph(tdc_server *s, int bucket, int buf, std::string &err)

@@ -173,6 +173,87 @@ __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
     }
 }
 
+// Stop-and-hop input: when the antenna position is the same for every sample
+// of a chirp, the range |pos - p| of plain_tile is one FP32 value per
+// (pixel, chirp). It is computed once with the same operations and each term
+// keeps the direct phase fract(k_m * range), so the image is bit-identical to
+// plain_tile's while the three subtractions, the squared norm and v_sqrt_f32
+// leave the per-term work. Records: `rec` [npri][ns] float4 (rr, ri, k, 0),
+// `cpos` [npri] float4 (x, y, z, 0).
+constexpr int kTermStage = 512;
+
+__device__ __forceinline__ float range32(float px, float py, float pz, float4 c)
+{
+    const float dx = px - c.x;
+    const float dy = py - c.y;
+    const float dz = pz - c.z;
+    return __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+}
+
+// Adds sum_m (rr + j ri) exp(+j 2 pi fract(rng * k_m)) over one chirp's
+// records `row` to (sr, si), term by term in sample order. Every wave of the
+// workgroup takes part in the staging; a wave with run == false (the same for
+// all its lanes) only stages.
+template <int P>
+__device__ __forceinline__ void chirp_terms(const float4 *__restrict__ row, int ns,
+                                            float4 *__restrict__ s_term, const float (&rng)[P],
+                                            float (&sr)[P], float (&si)[P], bool run)
+{
+    for (int m0 = 0; m0 < ns; m0 += kTermStage) {
+        const int cnt = min(kTermStage, ns - m0);
+        __syncthreads();
+        for (int i = static_cast<int>(threadIdx.x); i < cnt; i += kBlock) s_term[i] = row[m0 + i];
+        __syncthreads();
+        if (!run) continue;
+#pragma unroll 4
+        for (int e = 0; e < cnt; ++e) {
+            const float4 a = s_term[e];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const float th = __builtin_amdgcn_fractf(rng[j] * a.z);
+                const float sn = sin_rev(th);
+                const float cs = cos_rev(th);
+                sr[j] = fmaf(a.x, cs, sr[j]);
+                sr[j] = fmaf(-a.y, sn, sr[j]);
+                si[j] = fmaf(a.x, sn, si[j]);
+                si[j] = fmaf(a.y, cs, si[j]);
+            }
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void range_tile(const float4 *__restrict__ rec,
+                                           const float4 *__restrict__ cpos,
+                                           const float *__restrict__ grid, int npx, int ns,
+                                           int n0, int n1, int pix0, float2 *__restrict__ out)
+{
+    __shared__ float4 s_term[kTermStage];
+    float px[P], py[P], pz[P], ar[P], ai[P];
+    int pix[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        pix[j] = pix0 + j * kBlock + static_cast<int>(threadIdx.x);
+        const int q = min(pix[j], npx - 1);
+        px[j] = grid[3 * q];
+        py[j] = grid[3 * q + 1];
+        pz[j] = grid[3 * q + 2];
+        ar[j] = 0.0F;
+        ai[j] = 0.0F;
+    }
+    for (int n = n0; n < n1; ++n) {
+        const float4 c = as_const(cpos)[n];
+        float rng[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) rng[j] = range32(px[j], py[j], pz[j], c);
+        chirp_terms<P>(rec + static_cast<size_t>(n) * ns, ns, s_term, rng, ar, ai, true);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        if (pix[j] < npx) out[pix[j]] = make_float2(ar[j], ai[j]);
+    }
+}
+
 // Precision form over one tile: FP64 range per (pixel, chirp), FP64 anchor
 // phase per 64-sample block, FP32 rotation recurrence in nested form inside
 // the block, block sums accumulated per chirp, chirps accumulated with a

@@ -25,7 +25,7 @@ class Stats(ctypes.Structure):
                 ("doppler_active", ctypes.c_int32), ("chirp_splits", ctypes.c_int32),
                 ("pixels_per_lane", ctypes.c_int32), ("workgroups", ctypes.c_int32),
                 ("skipped_chunks", ctypes.c_int64), ("processed_chunks", ctypes.c_int64),
-                ("engine", ctypes.c_int32), ("window_per_chirp", ctypes.c_int32)]
+                ("engine", ctypes.c_int32), ("per_chirp", ctypes.c_int32)]
 
 
 @pytest.fixture(scope="module")

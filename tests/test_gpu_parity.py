@@ -74,6 +74,17 @@ def ragged_scene(npri=130, ns=100, rows=13, cols=21, seed=3):
     return scene.synthesize("ragged", track, ns, grid, scat, amp, 30.0, rng, speed=10.0)
 
 
+def moving_within_chirp(d, s):
+    """Per-sample positions: the antenna keeps moving along the track during each
+    chirp (float32, as the reference API takes them)."""
+    d = dict(d)
+    vel = d["velocity"].astype(np.float64)
+    t = s.sample_times[None, :, None]
+    d["position"] = np.ascontiguousarray((d["position"].astype(np.float64) + vel * t)
+                                         .astype(np.float32))
+    return d
+
+
 # ---------------------------------------------------------------------------- plain form
 
 
@@ -118,16 +129,43 @@ def test_plain_other_range_windows(tdc, orc, window):
     check(orc, img, oracle_dropin(orc, d, rw=rw), PLAIN_TOL)
 
 
+@pytest.mark.parametrize("path", ["per_chirp", "per_term"])
 @pytest.mark.parametrize("ppl", [1, 2, 4])
 @pytest.mark.parametrize("splits", [0, 1, 3, 8])
-def test_plain_launch_shapes_ragged(tdc, orc, ppl, splits):
+def test_plain_launch_shapes_ragged(tdc, orc, ppl, splits, path):
+    """Stop-and-hop positions take the per-(pixel, chirp) range; antenna motion
+    within the chirps takes the per-term range."""
     s = ragged_scene()
     d = s.dropin()
+    if path == "per_term":
+        d = moving_within_chirp(d, s)
     img, st = form(tdc, d, ppl=ppl, splits=splits)
     assert st["pixels_per_lane"] == ppl
+    assert st["range_per_chirp"] == (path == "per_chirp")
+    assert not st["window_per_chirp"]
     if splits:
         assert st["chirp_splits"] == splits
     check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
+
+
+@pytest.mark.parametrize("ppl", [1, 2, 4])
+def test_plain_range_paths_bit_identical(tdc, ppl):
+    """The per-chirp range is the per-term range computed once: the same image bit
+    for bit. Sample 1 of every chirp carries zero raw data, so moving its position
+    by one float32 ulp changes no term's contribution but sends the data down the
+    per-term path."""
+    s = ragged_scene()
+    d = s.dropin()
+    d["rawData"] = d["rawData"].copy()
+    d["rawData"][:, 1] = 0
+    a, sa = form(tdc, d, ppl=ppl, rw=True)
+    d2 = dict(d)
+    d2["position"] = d["position"].copy()
+    d2["position"][:, 1, 0] = np.nextafter(d2["position"][:, 1, 0], np.float32(np.inf))
+    b, sb = form(tdc, d2, ppl=ppl, rw=True)
+    assert sa["range_per_chirp"] and not sb["range_per_chirp"]
+    assert sa["chirp_splits"] == sb["chirp_splits"]
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_plain_line_jitter_subset(tdc, orc):
@@ -394,17 +432,6 @@ def doppler_dict(d, center, bw, centroid, taper):
                 center=center, bandwidth=bw, centroid=centroid, taper=taper)
 
 
-def moving_within_chirp(d, s):
-    """Per-sample positions: the antenna keeps moving along the track during each
-    chirp (float32, as the reference API takes them)."""
-    d = dict(d)
-    vel = d["velocity"].astype(np.float64)
-    t = s.sample_times[None, :, None]
-    d["position"] = np.ascontiguousarray((d["position"].astype(np.float64) + vel * t)
-                                         .astype(np.float32))
-    return d
-
-
 @pytest.mark.parametrize("path", ["per_chirp", "per_term"])
 @pytest.mark.parametrize("centroid,taper,center,bw", [
     (True, True, 0.0, 8e3),
@@ -421,6 +448,7 @@ def test_doppler_window_and_skip(tdc, orc, centroid, taper, center, bw, path):
     img, st = form(tdc, d, center=center, bw=bw, centroid=centroid, taper=taper)
     assert st["doppler_active"]
     assert st["window_per_chirp"] == (path == "per_chirp")
+    assert st["range_per_chirp"] == (path == "per_chirp")
     dop = doppler_dict(d, center, bw, centroid, taper)
     ref = oracle_dropin(orc, d, doppler=dop)
     # skip counts against the oracle's own skip test
