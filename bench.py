@@ -42,8 +42,10 @@ PEAK_FP32_TFLOPS = 157.3        # MI355X published FP32 vector peak (packed)
 PEAK_FP32_NONPACKED = 78.6      # without packed FP32 issue
 PEAK_FP16_MFMA_TFLOPS = 2500.0  # MI355X published dense FP16 matrix peak
 # algorithmic FP32 flops per term: plain = 3 sub + 5 norm^2 + sqrt + scale +
-# sin + cos + 8 (complex MAC); precision = 6 (complex rotation) + 8 (MAC)
-FLOP_PER_TERM = {"plain": 20.0, "precision": 14.0}
+# sin + cos + 8 (complex MAC); range (plain on stop-and-hop input, the range
+# once per pixel and chirp) = scale + fract + sin + cos + 8; precision = 6
+# (complex rotation) + 8 (MAC)
+FLOP_PER_TERM = {"plain": 20.0, "range": 12.0, "precision": 14.0}
 # FP16 MFMA flops the matrix engine executes per term: 8 per product (one
 # complex MAC as a real 2x2 block) times hi*hi + hi*lo (+ lo*hi for c64 raw)
 MFMA_FLOP_PER_TERM = {"c64": 24.0, "f16": 16.0, "bf16": 16.0}
@@ -53,7 +55,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_proc(tdc, s, kind, storage="c64", engine="auto"):
+def make_proc(tdc, s, kind, storage="c64", engine="auto", moving=False):
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
     if kind == "precision":
@@ -66,6 +68,13 @@ def make_proc(tdc, s, kind, storage="c64", engine="auto"):
         keep = f
     else:
         d = s.dropin()
+        if moving:
+            # the antenna keeps moving along its velocity during each chirp: one
+            # position per sample, so the range is evaluated per term
+            vel = d["velocity"].astype(np.float64)
+            d["position"] = np.ascontiguousarray(
+                (d["position"].astype(np.float64) + vel * s.sample_times[None, :, None])
+                .astype(np.float32))
         p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"], d["velocity"],
                      d["attitude"], d["modRate"], d["startFreq"], d["bodyBoresight"])
         p.setFocusGrid(d["focusGrid"])
@@ -79,8 +88,9 @@ def make_proc(tdc, s, kind, storage="c64", engine="auto"):
 DOPPLER = dict(center=0.0, bandwidth=4000.0, centroid=True, taper=True)
 
 
-def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=None):
-    p, keep = make_proc(tdc, s, kind, storage, engine)
+def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=None,
+                 moving=False):
+    p, keep = make_proc(tdc, s, kind, storage, engine, moving)
     win = ((doppler["center"], doppler["bandwidth"], doppler["centroid"], False,
             doppler["taper"]) if doppler else (0.0, 1e30, False, False, False))
     # end-to-end start(): host arrays -> H2D -> compute -> D2H (PCIe inclusive); the
@@ -210,10 +220,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="arc", choices=["line_jitter", "arc", "helix"])
-    ap.add_argument("--variants", default="c64,f16,bf16,vector,plain,doppler",
+    ap.add_argument("--variants", default="c64,f16,bf16,vector,plain,doppler,plain_moving",
                     help="comma list of c64, f16, bf16 (precision form, matrix engine), vector "
-                         "(precision form, vector engine, c64), plain and doppler (the plain "
-                         "form with the Doppler window on)")
+                         "(precision form, vector engine, c64), plain, doppler (the plain "
+                         "form with the Doppler window on) and plain_moving (the plain form "
+                         "with the antenna moving during each chirp: range per term)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -263,11 +274,11 @@ def main():
 
     results = {}
     for v in [x.strip() for x in args.variants.split(",") if x.strip()]:
-        kind = "plain" if v in ("plain", "doppler") else "precision"
-        storage = "c64" if v in ("plain", "vector", "doppler") else v
+        kind = "plain" if v in ("plain", "doppler", "plain_moving") else "precision"
+        storage = "c64" if kind == "plain" or v == "vector" else v
         engine = "vector" if v == "vector" else "auto"
         r = time_variant(tdc, s, kind, storage, args.steps, args.warmup, engine,
-                         DOPPLER if v == "doppler" else None)
+                         DOPPLER if v == "doppler" else None, v == "plain_moving")
         r["gterms"] = r["terms"] / (r["ms_per_step"] * 1e-3) / 1e9
         r["gterms_kernel"] = r["terms"] / (r["main_kernel_ms"] * 1e-3) / 1e9
         r["gterms_e2e"] = r["terms"] / (r["e2e_ms"] * 1e-3) / 1e9
@@ -307,6 +318,11 @@ def main():
                                            d["focusGrid"], float(d["modRate"]),
                                            float(d["startFreq"]), pixels=px)
                 ref = o32
+            elif v == "plain_moving":
+                d = r["keep"]
+                ref = oracle.image_f64(d["rawData"], d["sampleTimes"], d["position"],
+                                       d["focusGrid"], float(d["modRate"]),
+                                       float(d["startFreq"]), pixels=px)
             else:
                 if o64 is None:
                     o64 = oracle.image_f64(s.raw, s.sample_times, s.track, s.grid, s.mod_rate,
@@ -347,12 +363,16 @@ def main():
 
     head = results.get("c64") or next(iter(results.values()))
     kind = head["kind"]
-    flop = FLOP_PER_TERM[kind]
+    fkey = "range" if kind == "plain" and head["range_per_chirp"] else kind
+    flop = FLOP_PER_TERM[fkey]
     terms_per_s = head["terms"] / (head["main_kernel_ms"] * 1e-3)
     algorithmic = terms_per_s * flop / 1e12
     traffic = None
     simd = None
-    kname = {"matrix": "precision_matrix", "vector": "precision"}.get(head["engine"], kind)
+    kname = {"matrix": "precision_matrix", "vector": "precision"}.get(head["engine"], fkey)
+    if kind == "plain" and head["doppler_active"]:
+        kname = "doppler_chirp" if head["window_per_chirp"] else "doppler"
+
     tfile = os.path.join(ROOT, "profiles", f"traffic_{kname}_{args.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:

@@ -282,7 +282,6 @@ bool antenna_constant_per_chirp(const tdc_processor *h)
 // |f_c| <= (2/lambda)|v| |q|^2 |b| (a rotation by a non-unit quaternion
 // scales by |q|^2), and excludes pixels that coincide with an antenna
 // position (0/0 there makes the reference's window NaN, hence 0).
-
 bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool centroid,
                            bool taper)
 {
@@ -311,8 +310,27 @@ bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool 
     if (!std::isfinite(bound) || bound > double(bw) / 2.0) return false;
     // with the taper on, cos(2 pi d / bw - pi) must round to exactly -1
     if (taper && 2.0 * kPi * bound / double(bw) > 1e-5) return false;
+    // a pixel at the origin meets the zero-filled probe rows; any other pixel
+    // can only coincide with an antenna position inside the positions' bounding
+    // box, so the set of positions is built only when such a pixel exists
+    float lo[3] = {h->pos[0], h->pos[1], h->pos[2]};
+    float hi[3] = {lo[0], lo[1], lo[2]};
+    for (size_t e = 1; e < nterm; ++e) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], h->pos[3 * e + a]);
+            hi[a] = std::max(hi[a], h->pos[3 * e + a]);
+        }
+    }
+    std::vector<size_t> inside;
+    for (size_t i = 0; i < npx; ++i) {
+        const float *g = h->grid + 3 * i;
+        if (g[0] == 0.0F && g[1] == 0.0F && g[2] == 0.0F) return false;
+        if (g[0] >= lo[0] && g[0] <= hi[0] && g[1] >= lo[1] && g[1] <= hi[1] && g[2] >= lo[2] &&
+            g[2] <= hi[2])
+            inside.push_back(i);
+    }
+    if (inside.empty()) return true;
     std::unordered_set<PosKey, PosKeyHash> seen;
-    seen.insert(key_of(0.0F, 0.0F, 0.0F)); // zero-filled probe rows
     const float *last = nullptr;
     for (size_t e = 0; e < nterm; ++e) {
         const float *p = h->pos + 3 * e;
@@ -320,7 +338,7 @@ bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool 
         seen.insert(key_of(p[0], p[1], p[2]));
         last = p;
     }
-    for (size_t i = 0; i < npx; ++i) {
+    for (size_t i : inside) {
         const float *g = h->grid + 3 * i;
         if (seen.count(key_of(g[0], g[1], g[2]))) return false;
     }
