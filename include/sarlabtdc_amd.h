@@ -45,13 +45,18 @@ extern "C" {
 #define TDC_FORM_PLAIN 0     /* direct per-term FP32 reference phase: the drop-in */
 #define TDC_FORM_PRECISION 1 /* FP64 range/anchor + FP32 rotation recurrence */
 
-/* Engines of the precision form (additive). Both compute the same sums to
- * FP32-level accuracy; the matrix engine forms the 8-sample group sums of
- * every 64-sample block with FP16 hi/lo MFMA (32x32x16) and the vector engine
- * runs a one-sample rotation recurrence on the VALU. */
-#define TDC_ENGINE_AUTO 0   /* matrix, unless pixels_per_lane > 1 is forced */
+/* Engines of the precision form (additive). All compute the same sums to
+ * FP32-level accuracy. The range-profile engine takes each chirp's
+ * oversampled spectrum once (an FFT in LDS) and interpolates it at every
+ * pixel's range with a 12-tap Gaussian kernel (a type-2 non-uniform FFT; up
+ * to 1024 samples per chirp). The matrix engine forms the 16-sample group
+ * sums of every 128-sample block with FP16 hi/lo MFMA (32x32x16). The vector
+ * engine runs a one-sample rotation recurrence on the VALU. */
+#define TDC_ENGINE_AUTO 0    /* range profile when the chirp has at most 1024 samples, else
+                                matrix; vector when pixels_per_lane > 1 is forced */
 #define TDC_ENGINE_VECTOR 1
 #define TDC_ENGINE_MATRIX 2
+#define TDC_ENGINE_PROFILE 3
 
 /* Raw sample storage on the device for the precision form (additive). */
 #define TDC_RAW_C64 0  /* complex64 */

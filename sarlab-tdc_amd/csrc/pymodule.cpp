@@ -59,6 +59,7 @@ int parse_engine(const py::object &o)
         if (s == "auto") return TDC_ENGINE_AUTO;
         if (s == "vector") return TDC_ENGINE_VECTOR;
         if (s == "matrix") return TDC_ENGINE_MATRIX;
+        if (s == "profile") return TDC_ENGINE_PROFILE;
         throw std::invalid_argument("engine must be 'auto', 'vector' or 'matrix'");
     }
     return o.cast<int>();
@@ -237,7 +238,10 @@ public:
         d["processed_chunks"] = st.processed_chunks;
         d["range_per_chirp"] = (st.per_chirp & TDC_PER_CHIRP_RANGE) != 0;
         d["window_per_chirp"] = (st.per_chirp & TDC_PER_CHIRP_WINDOW) != 0;
-        d["engine"] = st.engine == TDC_ENGINE_MATRIX ? "matrix" : (st.engine == TDC_ENGINE_VECTOR ? "vector" : "none");
+        d["engine"] = st.engine == TDC_ENGINE_MATRIX    ? "matrix"
+                      : st.engine == TDC_ENGINE_VECTOR  ? "vector"
+                      : st.engine == TDC_ENGINE_PROFILE ? "profile"
+                                                        : "none";
         return d;
     }
 

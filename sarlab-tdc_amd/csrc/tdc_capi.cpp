@@ -188,7 +188,7 @@ struct tdc_processor {
 
     // device buffers
     DevBuf d_raw, d_pos, d_vel, d_att, d_kt, d_wr, d_rec, d_grid, d_part, d_img, d_cpos,
-        d_kanchor, d_counters, d_atab, d_ascale, d_chirp;
+        d_kanchor, d_counters, d_atab, d_ascale, d_chirp, d_ptab, d_pcorr, d_ptw;
 
     // prepared state
     bool prepared = false;
@@ -199,6 +199,7 @@ struct tdc_processor {
     bool chirp_path = false; // plain form: range (and window) once per (pixel, chirp)
     tdc::Geom geom{};
     tdc::DopParams dp{};
+    tdc::ProfileParams pp{};
     int ppl = 1;
     double dk = 0.0;
     int64_t trivial_processed = 0;
@@ -511,8 +512,35 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
                                         ns, h->d_rec.p, s));
     h->doppler = false;
     int engine = h->engine;
-    if (engine == TDC_ENGINE_AUTO) engine = h->force_ppl > 1 ? TDC_ENGINE_VECTOR : TDC_ENGINE_MATRIX;
-    if (engine == TDC_ENGINE_MATRIX) {
+    const int plen = tdc::profile_length(ns);
+    if (engine == TDC_ENGINE_AUTO) {
+        if (h->force_ppl > 1) engine = TDC_ENGINE_VECTOR;
+        else engine = plen <= tdc::kProfileMaxM ? TDC_ENGINE_PROFILE : TDC_ENGINE_MATRIX;
+    }
+    if (engine == TDC_ENGINE_PROFILE) {
+        if (h->force_ppl > 1)
+            return fail(TDC_ERR_INVALID_ARGUMENT, "the range-profile engine runs one pixel per lane");
+        if (plen > tdc::kProfileMaxM)
+            return fail(TDC_ERR_UNSUPPORTED,
+                        "the range-profile engine takes at most " +
+                            std::to_string(tdc::kProfileMaxM / 2) + " samples per chirp");
+        std::vector<float> corr;
+        std::vector<float2> tw;
+        tdc::profile_tables(ns, plen, corr, tw);
+        if ((rc = upload(h->d_pcorr, corr.data(), corr.size(), s)) != TDC_OK) return rc;
+        if ((rc = upload(h->d_ptw, tw.data(), tw.size(), s)) != TDC_OK) return rc;
+        const int mp = plen + tdc::kProfilePad;
+        TDC_HIP(h->d_ptab.ensure(sizeof(float2) * static_cast<size_t>(npri) * mp));
+        TDC_HIP(tdc::launch_stage_profile(h->storage, h->d_rec.p, h->d_pcorr.as<float>(),
+                                          h->d_ptw.as<float2>(), npri, ns, plen, mp,
+                                          h->d_ptab.as<float2>(), s));
+        const double dt = ns > 1 ? (t[ns - 1] - t[0]) / (ns - 1) : 0.0;
+        h->pp.dk = h->dk;
+        h->pp.kh = 2.0 * (f0 + krate * (t[0] + (ns / 2) * dt)) / kSpeedOfLight;
+        h->pp.M = plen;
+        h->pp.Mp = mp;
+        h->ppl = 1;
+    } else if (engine == TDC_ENGINE_MATRIX) {
         if (h->force_ppl > 1)
             return fail(TDC_ERR_INVALID_ARGUMENT, "the matrix engine runs one pixel per lane");
         TDC_HIP(h->d_atab.ensure(tdc::matrix_table_bytes(npri, ns, h->storage)));
@@ -551,6 +579,64 @@ void clear_last_error() { t_err.clear(); }
 // advances by a constant phase step, so the sample times must be uniformly
 // spaced (a few float32 ulps of the largest time are allowed, so that times
 // given in float32 qualify; each anchor uses its own exact sample time).
+// Host tables of the range-profile engine for ns samples and an M-point FFT:
+// corr[i] = 1 / psi_hat((i - ns/2) / M), psi_hat the Fourier transform of the
+// truncated Gaussian exp(-b u^2), |u| <= taps/2 (64-point Gauss-Legendre
+// quadrature in FP64); twiddle[k] = exp(+j 2 pi k / M), k < M / 2.
+namespace {
+struct GaussLegendre64 {
+    static constexpr int kNodes = 64;
+    double node[kNodes], weight[kNodes];
+    GaussLegendre64()
+    {
+        // Legendre nodes by Newton's method from the Chebyshev guesses
+        for (int i = 0; i < kNodes; ++i) {
+            double x = std::cos(kPi * (i + 0.75) / (kNodes + 0.5));
+            double dp = 1.0;
+            for (int it = 0; it < 100; ++it) {
+                double p0 = 1.0, p1 = x;
+                for (int k = 2; k <= kNodes; ++k) {
+                    const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                dp = kNodes * (x * p1 - p0) / (x * x - 1.0);
+                const double dx = p1 / dp;
+                x -= dx;
+                if (std::fabs(dx) < 1e-16) break;
+            }
+            node[i] = x;
+            weight[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+        }
+    }
+};
+} // namespace
+
+void profile_tables(int ns, int M, std::vector<float> &corr, std::vector<float2> &twiddle)
+{
+    static const GaussLegendre64 gl;
+    constexpr int kNodes = GaussLegendre64::kNodes;
+    const double *node = gl.node, *weight = gl.weight;
+    const double half = kProfileTaps / 2;
+    const double b = 0.75 * kPi / half;
+    corr.resize(ns);
+    const int h = ns / 2;
+    for (int i = 0; i < ns; ++i) {
+        const double nu = double(i - h) / M;
+        double v = 0.0;
+        for (int k = 0; k < kNodes; ++k) {
+            const double u = half * node[k];
+            v += half * weight[k] * std::exp(-b * u * u) * std::cos(2.0 * kPi * nu * u);
+        }
+        corr[i] = static_cast<float>(1.0 / v);
+    }
+    twiddle.resize(M / 2);
+    for (int k = 0; k < M / 2; ++k) {
+        const double a = 2.0 * kPi * k / M;
+        twiddle[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    }
+}
+
 bool precision_tables(const double *t, int ns, double f0, double krate,
                       std::vector<double> &kanchor, double &dk, std::string &why)
 {
@@ -742,7 +828,8 @@ int tdc_set_raw_storage(tdc_processor *h, int storage)
 int tdc_set_engine(tdc_processor *h, int engine)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
-    if (engine != TDC_ENGINE_AUTO && engine != TDC_ENGINE_VECTOR && engine != TDC_ENGINE_MATRIX)
+    if (engine != TDC_ENGINE_AUTO && engine != TDC_ENGINE_VECTOR && engine != TDC_ENGINE_MATRIX &&
+        engine != TDC_ENGINE_PROFILE)
         return fail(TDC_ERR_INVALID_ARGUMENT, "unknown engine");
     h->engine = engine;
     h->prepared = false;
@@ -869,7 +956,11 @@ int tdc_run(tdc_processor *h, int iterations)
                 TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
                                                  h->d_img.as<float2>(), s));
         } else {
-            if (h->run_engine == TDC_ENGINE_MATRIX) {
+            if (h->run_engine == TDC_ENGINE_PROFILE) {
+                TDC_HIP(tdc::launch_profile(h->d_ptab.as<float2>(), h->d_cpos.as<tdc::ChirpPos>(),
+                                            h->d_grid.as<double>(), h->pp, g,
+                                            h->d_part.as<double2>(), s, &wgs));
+            } else if (h->run_engine == TDC_ENGINE_MATRIX) {
                 TDC_HIP(tdc::launch_precision_matrix(
                     tdc::matrix_parts(h->run_storage), h->d_atab.p, h->d_ascale.as<float>(),
                     h->d_cpos.as<tdc::ChirpPos>(), h->d_grid.as<double>(),

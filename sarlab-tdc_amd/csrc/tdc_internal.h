@@ -83,6 +83,32 @@ struct DopParams {
     float fc_fixed;  // dopplerWinCenter
 };
 
+// Range-profile engine of the precision form. With uniform sample times and
+// one phase centre per chirp, the chirp's sum for a pixel at range R is
+//   exp(j 2 pi k_h R) * sum_i c_i exp(j 2 pi (i - h) phi),  phi = dk R mod 1,
+// the discrete-time Fourier transform of the windowed raw samples c_i
+// (re-centred on sample h = ns / 2) at the frequency phi: the chirp's range
+// profile. It is evaluated as a type-2 non-uniform FFT: once per chirp the
+// samples, divided by the kernel's Fourier transform, go through an
+// M-point FFT (M >= 2 ns, a power of two); each pixel then sums
+// kProfileTaps table entries around phi * M weighted by the Gaussian kernel
+// exp(-b u^2), b = 0.75 pi / (kProfileTaps / 2).
+constexpr int kProfileTaps = 12;
+constexpr int kProfileMaxM = 2048; // the FFT of one chirp runs in LDS
+constexpr int kProfilePad = 16;    // wrap-around entries after the M bins of a table row
+struct ProfileParams {
+    double dk; // 2 K dt / c: cycles per sample per metre of range
+    double kh; // 2 (f0 + K t_h) / c: wavenumber at the re-centring sample h
+    int M;     // FFT length
+    int Mp;    // table row length, M + kProfilePad
+};
+inline int profile_length(int ns)
+{
+    int m = 16;
+    while (m < 2 * ns) m *= 2;
+    return m;
+}
+
 // Kernel launch wrappers (defined in tdc_kernels.hip). All enqueue on `s`.
 hipError_t launch_stage_plain(const float *raw, const float *pos, const float *kt, const float *wr,
                               int npri, int ns, PlainRec *out, hipStream_t s);
@@ -140,6 +166,15 @@ inline size_t matrix_table_bytes(int npri, int ns, int storage)
 inline int anchor_entries(int ns) { return 4 * matrix_quads(ns); }
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
+// Range-profile tables [npri][Mp] from the staged raw samples (in `storage`
+// format, range window folded in); corr[i] = 1 / psi_hat((i - h) / M);
+// twiddle[k] = exp(+j 2 pi k / M), k < M / 2.
+hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
+                                const float2 *twiddle, int npri, int ns, int M, int Mp,
+                                float2 *table, hipStream_t s);
+hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
+                          const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,
+                          int *workgroups);
 
 // A named roctx range over a scope: host phases (prepare, run, fetch, job
 // uploads, serving steps) appear beside the kernels in a profiler's marker
@@ -156,6 +191,7 @@ class Range {
 // The thread-local message returned by tdc_last_error().
 int set_last_error(int code, const std::string &msg);
 void clear_last_error();
+void profile_tables(int ns, int M, std::vector<float> &corr, std::vector<float2> &twiddle);
 bool precision_tables(const double *t, int ns, double f0, double krate,
                       std::vector<double> &kanchor, double &dk, std::string &why);
 

@@ -481,6 +481,72 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
     if (skipped) atomicAdd(&counters[1], skipped);
 }
 
+// ---------------------------------------------------------------------------
+// Range-profile engine: the per-chirp tables and the interpolation kernel.
+// ---------------------------------------------------------------------------
+
+// One workgroup per chirp: b[(i - h) mod M] = c_i * corr[i] (zero elsewhere),
+// then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-2 Stockham passes
+// between two LDS buffers (the output comes out in natural order), written
+// with kProfilePad wrap-around entries so that a pixel's taps m0 .. m0 + 11
+// never wrap.
+template <class RAW>
+__global__ __launch_bounds__(kBlock) void stage_profile_kernel(
+    const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
+    const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
+{
+    __shared__ float2 buf[2 * kProfileMaxM];
+    const int n = static_cast<int>(blockIdx.x);
+    const int tid = static_cast<int>(threadIdx.x);
+    const int h = ns / 2;
+    const int mask = M - 1;
+    float2 *x = buf;
+    float2 *y = buf + kProfileMaxM;
+    for (int j = tid; j < M; j += kBlock) x[j] = make_float2(0.0F, 0.0F);
+    __syncthreads();
+    const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
+    for (int i = tid; i < ns; i += kBlock) {
+        float re, im;
+        RAW::get(row, i, re, im);
+        const float c = corr[i];
+        x[(i - h) & mask] = make_float2(re * c, im * c);
+    }
+    __syncthreads();
+    for (int l = M / 2, s = 1, lg = 0; l >= 1; l >>= 1, s <<= 1, ++lg) {
+        for (int idx = tid; idx < M / 2; idx += kBlock) {
+            const int j = idx >> lg;
+            const int k = idx & (s - 1);
+            const float2 w = twiddle[j * s];
+            const float2 a = x[k + s * j];
+            const float2 c = x[k + s * (j + l)];
+            const float dr = a.x - c.x, di = a.y - c.y;
+            y[k + 2 * s * j] = make_float2(a.x + c.x, a.y + c.y);
+            y[k + 2 * s * j + s] = make_float2(fmaf(dr, w.x, -(di * w.y)), fmaf(dr, w.y, di * w.x));
+        }
+        __syncthreads();
+        float2 *tmp = x;
+        x = y;
+        y = tmp;
+    }
+    float2 *dst = table + static_cast<size_t>(n) * Mp;
+    for (int j = tid; j < Mp; j += kBlock) dst[j] = x[j & mask];
+}
+
+__global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restrict__ table,
+                                                         const ChirpPos *__restrict__ cpos,
+                                                         const double *__restrict__ grid,
+                                                         ProfileParams pp, Geom g,
+                                                         double2 *__restrict__ out)
+{
+    int tile, split;
+    map_block(g, tile, split);
+    if (tile >= g.ntiles) return;
+    const int n0 = split * g.cps;
+    const int n1 = min(g.npri, n0 + g.cps);
+    profile_tile(table, cpos, grid, pp, g.npx, n0, n1, tile * kBlock,
+                 out + static_cast<size_t>(split) * g.npx);
+}
+
 // Sum of the per-split partial images in split order.
 __global__ __launch_bounds__(kBlock) void finalize_f32_kernel(const float2 *__restrict__ partial,
                                                               int nsplit, int npx,
@@ -808,6 +874,44 @@ hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
+                                const float2 *twiddle, int npri, int ns, int M, int Mp,
+                                float2 *table, hipStream_t s)
+{
+    if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || Mp < M + kProfileTaps - 1)
+        return hipErrorInvalidValue;
+    switch (storage) {
+    case 0:
+        hipLaunchKernelGGL(stage_profile_kernel<RawC64>, dim3(npri), dim3(kBlock), 0, s,
+                           static_cast<const float2 *>(staged), corr, twiddle, ns, M, Mp, table);
+        break;
+    case 1:
+        hipLaunchKernelGGL(stage_profile_kernel<RawF16>, dim3(npri), dim3(kBlock), 0, s,
+                           static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
+                           table);
+        break;
+    case 2:
+        hipLaunchKernelGGL(stage_profile_kernel<RawBF16>, dim3(npri), dim3(kBlock), 0, s,
+                           static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
+                           table);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
+                          const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,
+                          int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    hipLaunchKernelGGL(profile_kernel, dim3(nb), dim3(kBlock), 0, s, table, cpos, grid, pp, g,
+                       out);
     return hipGetLastError();
 }
 

@@ -608,4 +608,75 @@ __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
     if (pix < npx) out[pix] = make_double2(ar0 + ar1, ai0 + ai1);
 }
 
+// Range-profile engine over one tile (one pixel per thread, chirps n0..n1):
+// per (pixel, chirp) the FP64 range R, the kernel position phi * M - taps/2
+// split into an integer bin m0 and a fraction g in (-1, 0], the twelve table
+// entries m0 .. m0 + 11 weighted by exp(-b (g - s)^2), s = t - 6 (computed as
+// exp(-b g^2) exp(2 b g)^s exp(-b s^2): three exp2 per pair, two multiplies
+// per tap), and the FP64 anchor exp(j 2 pi k_h R); chirps accumulate in FP64.
+namespace profile {
+constexpr float kE0 = -0.5665450177283992F; // -b log2(e), b = 0.75 pi / 6
+constexpr float kE1 = 1.1330900354567983F;  // 2 b log2(e)
+// exp(-b s^2) for s = -6 .. 5
+constexpr float kC[kProfileTaps] = {7.249472515987942e-07F, 5.449088911519817e-05F,
+                                    0.0018674427317079893F, 0.029179416423313852F,
+                                    0.20787957635076193F,   0.6752319066557773F,
+                                    1.0F,                   0.6752319066557773F,
+                                    0.20787957635076193F,   0.029179416423313852F,
+                                    0.0018674427317079893F, 5.449088911519817e-05F};
+} // namespace profile
+
+__device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
+                                             const ChirpPos *__restrict__ cpos,
+                                             const double *__restrict__ grid,
+                                             const ProfileParams &pp, int npx, int n0, int n1,
+                                             int pix0, double2 *__restrict__ out)
+{
+    constexpr int kHalf = kProfileTaps / 2;
+    const int pix = pix0 + static_cast<int>(threadIdx.x);
+    const int q = min(pix, npx - 1);
+    const double px = grid[3 * q], py = grid[3 * q + 1], pz = grid[3 * q + 2];
+    const int mask = pp.M - 1;
+    const double fm = static_cast<double>(pp.M);
+    double accr = 0.0, acci = 0.0;
+    for (int n = n0; n < n1; ++n) {
+        const ChirpPos c = as_const(cpos)[n];
+        const double dx = c.x - px, dy = c.y - py, dz = c.z - pz;
+        const double rng = range_from_sq(fma(dz, dz, fma(dy, dy, dx * dx)));
+        const double y = fma(fract64(pp.dk * rng), fm, -static_cast<double>(kHalf));
+        const double m0f = ceil(y);
+        const float g = static_cast<float>(y - m0f);
+        const int m0 = static_cast<int>(m0f) & mask;
+        const float2 *row = table + static_cast<size_t>(n) * pp.Mp + m0;
+        float2 v[kProfileTaps];
+#pragma unroll
+        for (int t = 0; t < kProfileTaps; ++t) v[t] = row[t];
+        const float e0 = __builtin_amdgcn_exp2f(profile::kE0 * (g * g));
+        const float e1 = __builtin_amdgcn_exp2f(profile::kE1 * g);
+        const float e1i = __builtin_amdgcn_exp2f(-profile::kE1 * g);
+        float ar = 0.0F, ai = 0.0F;
+        float up = e0;
+#pragma unroll
+        for (int t = kHalf; t < kProfileTaps; ++t) {
+            const float w = up * profile::kC[t];
+            ar = fmaf(w, v[t].x, ar);
+            ai = fmaf(w, v[t].y, ai);
+            up *= e1;
+        }
+        float dn = e0 * e1i;
+#pragma unroll
+        for (int t = kHalf - 1; t >= 0; --t) {
+            const float w = dn * profile::kC[t];
+            ar = fmaf(w, v[t].x, ar);
+            ai = fmaf(w, v[t].y, ai);
+            dn *= e1i;
+        }
+        const float xa = static_cast<float>(fract64(pp.kh * rng));
+        const float zr = cos_rev(xa), zi = sin_rev(xa);
+        accr += static_cast<double>(fmaf(zr, ar, -(zi * ai)));
+        acci += static_cast<double>(fmaf(zr, ai, zi * ar));
+    }
+    if (pix < npx) out[pix] = make_double2(accr, acci);
+}
+
 } // namespace tdc

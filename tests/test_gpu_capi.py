@@ -96,9 +96,9 @@ def test_capi_plain_golden_line_small(lib, orc):
     assert e <= 2e-3
 
 
-@pytest.mark.parametrize("engine", [1, 2])
+@pytest.mark.parametrize("engine", [1, 2, 3])
 def test_capi_precision_golden_ragged(lib, orc, engine):
-    """FP64 inputs, both precision engines, on the 130 x 100 ragged fixture."""
+    """FP64 inputs, the three precision engines, on the 130 x 100 ragged fixture."""
     z = np.load(os.path.join(GOLDEN, "precision_ragged.npz"))
     codes = status_codes()
     raw, p_raw = f32(z["rawData"].view(np.float32))

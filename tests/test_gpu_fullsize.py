@@ -46,13 +46,16 @@ def oracle_subset(orc, s, img, n=256, seed=11):
     return px, ref
 
 
-def test_arc_matrix_engine_vs_oracle(tdc, orc, arc):
-    img, st = precision(tdc, arc)
-    assert st["engine"] == "matrix" and st["terms"] == arc.terms
+@pytest.mark.parametrize("engine", ["auto", "matrix"])
+def test_arc_engine_vs_oracle(tdc, orc, arc, engine):
+    """The automatic choice for 512-sample chirps is the range-profile engine."""
+    img, st = precision(tdc, arc, engine=engine)
+    assert st["engine"] == ("profile" if engine == "auto" else engine)
+    assert st["terms"] == arc.terms
     px, ref = oracle_subset(orc, arc, img)
     e = orc.rel_l2(img[px], ref)
     ph = orc.peak_phase_error(img[px], ref)
-    print(f"arc matrix: rel-L2 {e:.3e}, peak phase {ph:.3e} rad")
+    print(f"arc {st['engine']}: rel-L2 {e:.3e}, peak phase {ph:.3e} rad")
     assert e <= PREC_TOL and ph <= PREC_TOL
 
 
@@ -72,11 +75,14 @@ def test_arc_linearity(tdc, arc):
 def test_arc_engines_and_storage(tdc, arc):
     m, _ = precision(tdc, arc, engine="matrix")
     v, _ = precision(tdc, arc, engine="vector")
+    p, _ = precision(tdc, arc, engine="profile")
     h, _ = precision(tdc, arc, storage="f16")
     ev = np.linalg.norm(m - v) / np.linalg.norm(v)
-    eh = np.linalg.norm(h - m) / np.linalg.norm(m)
-    print(f"matrix vs vector {ev:.3e}; f16 storage vs complex64 {eh:.3e}")
-    assert ev < 2e-6
+    ep = np.linalg.norm(p - v) / np.linalg.norm(v)
+    eh = np.linalg.norm(h - p) / np.linalg.norm(p)
+    print(f"matrix vs vector {ev:.3e}; profile vs vector {ep:.3e}; f16 storage vs complex64 "
+          f"{eh:.3e}")
+    assert ev < 2e-6 and ep < 2e-6
     assert eh < 3e-3
 
 
@@ -107,13 +113,15 @@ def test_arc_plain_vs_oracle(tdc, orc, arc):
     assert e <= PLAIN_TOL
 
 
-def test_helix_matrix_engine_vs_oracle(tdc, orc):
+@pytest.mark.parametrize("engine", ["profile", "matrix"])
+def test_helix_engine_vs_oracle(tdc, orc, engine):
     from sarlabtdc import scene
 
     s = scene.make("helix")
-    img, st = precision(tdc, s)
+    img, st = precision(tdc, s, engine=engine)
+    assert st["engine"] == engine
     assert st["terms"] == s.terms
     px, ref = oracle_subset(orc, s, img, n=128, seed=13)
     e = orc.rel_l2(img[px], ref)
-    print(f"helix matrix: rel-L2 {e:.3e}")
+    print(f"helix {engine}: rel-L2 {e:.3e}")
     assert e <= PREC_TOL

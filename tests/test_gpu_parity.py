@@ -280,7 +280,7 @@ def test_precision_ragged(tdc, orc, ppl, splits):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
-@pytest.mark.parametrize("engine", ["vector", "matrix"])
+@pytest.mark.parametrize("engine", ["vector", "matrix", "profile"])
 @pytest.mark.parametrize("storage,tol", [("c64", PREC_TOL), ("f16", 3e-3), ("bf16", 2e-2)])
 def test_precision_engines_ragged(tdc, orc, engine, storage, tol):
     s = ragged_scene()
@@ -298,19 +298,41 @@ def test_matrix_engine_sample_counts(tdc, orc, ns):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
+@pytest.mark.parametrize("ns", [1, 2, 7, 8, 9, 64, 65, 127, 300, 512, 513, 1000, 1024])
+def test_profile_engine_sample_counts(tdc, orc, ns):
+    """FFT lengths 16..2048 (M >= 2 ns, at least 16), odd and even re-centring
+    samples, one-sample chirps, and the 1024-sample limit."""
+    s = ragged_scene(npri=40, ns=ns, rows=9, cols=11, seed=ns)
+    img, st = form(tdc, f64=s.f64(), kind="precision", engine="profile")
+    assert st["engine"] == "profile"
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
+def test_profile_engine_sample_limit(tdc, orc):
+    """Beyond 1024 samples per chirp the range-profile engine declines with a
+    message and the automatic choice takes the matrix engine."""
+    s = ragged_scene(npri=12, ns=1025, rows=5, cols=6, seed=3)
+    with pytest.raises(Exception, match="at most 1024 samples"):
+        form(tdc, f64=s.f64(), kind="precision", engine="profile")
+    img, st = form(tdc, f64=s.f64(), kind="precision")
+    assert st["engine"] == "matrix"
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
+@pytest.mark.parametrize("engine", ["matrix", "profile"])
 @pytest.mark.parametrize("gain_exp", [-30, 0, 30])
-def test_matrix_engine_per_chirp_scale(tdc, orc, gain_exp):
+def test_matrix_engine_per_chirp_scale(tdc, orc, gain_exp, engine):
     """Chirp amplitudes spread over 1e-2..1e2 around 10**gain_exp: each chirp
     gets its own power-of-two FP16 scale."""
     s = ragged_scene(npri=61, ns=200, rows=7, cols=9, seed=11)
     rng = np.random.default_rng(1)
     gain = 10.0 ** (gain_exp + rng.uniform(-2, 2, s.n_chirps))
     s.raw = (s.raw * gain[:, None]).astype(np.complex64)
-    img, _ = form(tdc, f64=s.f64(), kind="precision", engine="matrix")
+    img, _ = form(tdc, f64=s.f64(), kind="precision", engine=engine)
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
-@pytest.mark.parametrize("engine", ["vector", "matrix"])
+@pytest.mark.parametrize("engine", ["vector", "matrix", "profile"])
 def test_precision_down_chirp(tdc, orc, engine):
     """Negative chirp rate: the rotation step dk and the phase fractions are negative."""
     from sarlabtdc import scene
@@ -328,7 +350,7 @@ def test_precision_down_chirp(tdc, orc, engine):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
-@pytest.mark.parametrize("engine", ["vector", "matrix"])
+@pytest.mark.parametrize("engine", ["vector", "matrix", "profile"])
 def test_precision_far_from_origin(tdc, orc, engine):
     """Coordinates 1e5 m from the origin (a projected map frame): FP64 ranges
     keep the phase; the image equals the one formed near the origin."""
@@ -343,7 +365,7 @@ def test_precision_far_from_origin(tdc, orc, engine):
 
 
 @pytest.mark.parametrize("kind,engine", [("precision", "matrix"), ("precision", "vector"),
-                                         ("plain", "auto")])
+                                         ("precision", "profile"), ("plain", "auto")])
 def test_more_splits_than_chirps(tdc, orc, kind, engine):
     """64 chirp ranges over 40 chirps: the empty ranges contribute zero partials."""
     s = ragged_scene(npri=40, ns=96, rows=5, cols=7, seed=17)
@@ -363,9 +385,12 @@ def test_engines_agree(tdc, orc):
     s = scene.line_jitter()
     a, _ = form(tdc, f64=s.f64(), kind="precision", engine="vector")
     b, _ = form(tdc, f64=s.f64(), kind="precision", engine="matrix")
+    c, st = form(tdc, f64=s.f64(), kind="precision")
+    assert st["engine"] == "profile"
     e = orc.rel_l2(b, a)
-    print(f"matrix vs vector rel-L2 {e:.3e}")
-    assert e < 1e-5
+    e2 = orc.rel_l2(c, a)
+    print(f"matrix vs vector rel-L2 {e:.3e}; profile vs vector {e2:.3e}")
+    assert e < 1e-5 and e2 < 1e-5
 
 
 def test_precision_range_window(tdc, orc):
