@@ -626,6 +626,23 @@ constexpr float kC[kProfileTaps] = {7.249472515987942e-07F, 5.449088911519817e-0
                                     0.0018674427317079893F, 5.449088911519817e-05F};
 } // namespace profile
 
+// Taps from a window of the table row staged by the wave itself: a wave's
+// 64 pixels are close together, so their first taps m0 usually lie within
+// kWinReach bins of lane 0's. Each lane then loads one entry of the 64-entry
+// window starting kWinReach bins below lane 0's m0 (one coalesced load) into
+// the wave's own LDS slot, and reads its twelve taps from there (LDS delivers
+// twice the bytes per clock of the vector L1, which the twelve gathered taps
+// per lane would otherwise saturate). A wave with a lane outside the window
+// gathers from the table directly.
+constexpr int kWinReach = (64 - kProfileTaps) / 2;
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                                              const ChirpPos *__restrict__ cpos,
                                              const double *__restrict__ grid,
@@ -633,6 +650,9 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                                              int pix0, double2 *__restrict__ out)
 {
     constexpr int kHalf = kProfileTaps / 2;
+    __shared__ float2 s_win[kBlock / 64][64];
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    float2 *win = s_win[threadIdx.x >> 6];
     const int pix = pix0 + static_cast<int>(threadIdx.x);
     const int q = min(pix, npx - 1);
     const double px = grid[3 * q], py = grid[3 * q + 1], pz = grid[3 * q + 2];
@@ -646,11 +666,25 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
         const double y = fma(fract64(pp.dk * rng), fm, -static_cast<double>(kHalf));
         const double m0f = ceil(y);
         const float g = static_cast<float>(y - m0f);
-        const int m0 = static_cast<int>(m0f) & mask;
-        const float2 *row = table + static_cast<size_t>(n) * pp.Mp + m0;
+        // indices in unsigned arithmetic: every value is masked to the row or
+        // checked against the window before it addresses memory
+        const unsigned m0u = static_cast<unsigned>(static_cast<int>(m0f));
+        const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
+        const unsigned lo = __builtin_amdgcn_readfirstlane(m0u) - kWinReach;
+        const unsigned off = m0u - lo;
         float2 v[kProfileTaps];
+        if (__all(off <= 64U - kProfileTaps)) {
+            const float2 e = row[(lo + lane) & mask];
+            wave_lds_sync(); // the previous chirp's reads of the slot are done
+            win[lane] = e;
+            wave_lds_sync();
 #pragma unroll
-        for (int t = 0; t < kProfileTaps; ++t) v[t] = row[t];
+            for (int t = 0; t < kProfileTaps; ++t) v[t] = win[off + t];
+        } else {
+            const float2 *r = row + (m0u & static_cast<unsigned>(mask));
+#pragma unroll
+            for (int t = 0; t < kProfileTaps; ++t) v[t] = r[t];
+        }
         const float e0 = __builtin_amdgcn_exp2f(profile::kE0 * (g * g));
         const float e1 = __builtin_amdgcn_exp2f(profile::kE1 * g);
         const float e1i = __builtin_amdgcn_exp2f(-profile::kE1 * g);

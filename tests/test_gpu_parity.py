@@ -308,6 +308,27 @@ def test_profile_engine_sample_counts(tdc, orc, ns):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
+@pytest.mark.parametrize("spacing", [4e-3, 0.02, 0.15])
+def test_profile_engine_window_and_gather_paths(tdc, orc, spacing):
+    """Pixel spacings from 4 mm to 15 cm: a wave's taps fit the 64-bin window
+    staged in LDS (close pixels), fit it for some chirps only, or never (a
+    wave's 64 pixels spread over metres), where the taps are gathered from the
+    table directly."""
+    from sarlabtdc import scene
+
+    rng = np.random.default_rng(31)
+    npri, ns = 48, 256
+    track = np.zeros((npri, 3))
+    track[:, 0] = np.linspace(-0.3, 0.3, npri)
+    track += rng.normal(scale=2e-4, size=track.shape)
+    grid = scene.plane_grid(12, 70, spacing, (0.0, 1.2, 0.0))
+    scat = scene._uniform_on_grid(rng, grid, 6)
+    s = scene.synthesize("spread", track, ns, grid, scat, scene._cn(rng, 6, 1.0), 30.0, rng)
+    img, st = form(tdc, f64=s.f64(), kind="precision", engine="profile")
+    assert st["engine"] == "profile"
+    check(orc, img, oracle_f64(orc, s), PREC_TOL)
+
+
 def test_profile_engine_sample_limit(tdc, orc):
     """Beyond 1024 samples per chirp the range-profile engine declines with a
     message and the automatic choice takes the matrix engine."""
