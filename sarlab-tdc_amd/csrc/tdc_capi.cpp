@@ -188,7 +188,7 @@ struct tdc_processor {
 
     // device buffers
     DevBuf d_raw, d_pos, d_vel, d_att, d_kt, d_wr, d_rec, d_grid, d_part, d_img, d_cpos,
-        d_kanchor, d_counters, d_atab, d_ascale, d_chirp, d_ptab, d_pcorr, d_ptw;
+        d_kanchor, d_counters, d_atab, d_ascale, d_chirp, d_ptab, d_pcorr, d_ptw, d_pcpos;
 
     // prepared state
     bool prepared = false;
@@ -534,6 +534,23 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
         TDC_HIP(tdc::launch_stage_profile(h->storage, h->d_rec.p, h->d_pcorr.as<float>(),
                                           h->d_ptw.as<float2>(), npri, ns, plen, mp,
                                           h->d_ptab.as<float2>(), s));
+        // chirp positions from an origin inside the scene (the midpoint of the
+        // first and last pixels), with |pos - o|^2 in the pad
+        const size_t last = 3 * (static_cast<size_t>(npx) - 1);
+        double o[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; ++a) {
+            o[a] = npx > 0 ? 0.5 * (gridp[a] + gridp[last + a]) : 0.0;
+            if (!std::isfinite(o[a])) o[a] = 0.0;
+        }
+        std::vector<tdc::ChirpPos> pc(npri);
+        for (int n = 0; n < npri; ++n) {
+            const double x = cp[n].x - o[0], y = cp[n].y - o[1], z = cp[n].z - o[2];
+            pc[n] = tdc::ChirpPos{x, y, z, std::fma(z, z, std::fma(y, y, x * x))};
+        }
+        if ((rc = upload(h->d_pcpos, pc.data(), pc.size(), s)) != TDC_OK) return rc;
+        h->pp.ox = o[0];
+        h->pp.oy = o[1];
+        h->pp.oz = o[2];
         const double dt = ns > 1 ? (t[ns - 1] - t[0]) / (ns - 1) : 0.0;
         h->pp.dk = h->dk;
         h->pp.kh = 2.0 * (f0 + krate * (t[0] + (ns / 2) * dt)) / kSpeedOfLight;
@@ -957,7 +974,7 @@ int tdc_run(tdc_processor *h, int iterations)
                                                  h->d_img.as<float2>(), s));
         } else {
             if (h->run_engine == TDC_ENGINE_PROFILE) {
-                TDC_HIP(tdc::launch_profile(h->d_ptab.as<float2>(), h->d_cpos.as<tdc::ChirpPos>(),
+                TDC_HIP(tdc::launch_profile(h->d_ptab.as<float2>(), h->d_pcpos.as<tdc::ChirpPos>(),
                                             h->d_grid.as<double>(), h->pp, g,
                                             h->d_part.as<double2>(), s, &wgs));
             } else if (h->run_engine == TDC_ENGINE_MATRIX) {

@@ -99,6 +99,7 @@ constexpr int kProfilePad = 16;    // wrap-around entries after the M bins of a 
 struct ProfileParams {
     double dk; // 2 K dt / c: cycles per sample per metre of range
     double kh; // 2 (f0 + K t_h) / c: wavenumber at the re-centring sample h
+    double ox, oy, oz; // origin the chirp positions are given from (a pixel of the grid)
     int M;     // FFT length
     int Mp;    // table row length, M + kProfilePad
 };
@@ -172,6 +173,7 @@ hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, floa
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp,
                                 float2 *table, hipStream_t s);
+// `cpos`: chirp positions minus the origin (pp.ox, oy, oz), with |pos - o|^2 in pad.
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
                           const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,
                           int *workgroups);
