@@ -18,17 +18,20 @@ constexpr int kMaxJobSlots = 8192;
 
 // One admitted job as the kernels see it.
 struct alignas(16) ServeJob {
-    const void *raw;          // precision: matrix engine A tiles; plain: (rr, ri, k, 0) terms
+    const void *raw;          // precision: range-profile table or matrix engine A tiles;
+                              // plain: (rr, ri, k, 0) terms
     const float *inv_scale;   // precision: per-chirp scale of the A tiles
-    const ChirpPos *cpos;     // precision: FP64 phase centre per chirp
+    const ChirpPos *cpos;     // precision: FP64 phase centre per chirp (range profile: from
+                              // the origin pp.o, with its squared norm in pad)
     const float4 *cpos32;     // plain: FP32 phase centre per chirp (x, y, z, 0)
     const double *grid64;     // precision: FP64 pixel coordinates
     const float *grid32;      // plain: FP32 pixel coordinates
     const double *kanchor;    // precision: 2 (f0 + K t_a) / c per 64-sample block
     void *partial;            // [nsplit][npx]: double2 (precision) or float2 (plain)
     float2 *image;            // [npx]
-    double dk;                // precision: 2 K dt / c
-    int npx, npri, ns, nsplit, cps, form, storage, pad;
+    double dk;                // precision, matrix engine: 2 K dt / c
+    ProfileParams pp;         // precision, range-profile engine
+    int npx, npri, ns, nsplit, cps, form, storage, profile;
 };
 
 struct ServeTile {
@@ -44,13 +47,15 @@ struct FinTile {
 };
 
 struct StepTable {
-    int ntiles;
+    int ntiles; // tiles[0, nfast): plain and range-profile; [nfast, ntiles): matrix engine
     int nfin;
-    int pad[2];
+    int nfast;
+    int pad;
     ServeTile tiles[kMaxStepTiles];
     FinTile fin[kMaxStepFin];
 };
 
+// `counter`: two counters, one per tile family.
 hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
                              int workgroups, hipStream_t s);
 hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,

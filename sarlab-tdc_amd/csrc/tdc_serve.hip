@@ -2,12 +2,13 @@
 //
 // A step of the server runs two kernels, captured once per grid-size bucket in
 // a hipGraph and replayed, and a finalize kernel on its own stream:
-//   1. serve_reset_kernel   zeroes the step's tile counter (a kernel, not a
+//   1. serve_reset_kernel   zeroes the step's two tile counters (a kernel, not a
 //                           memset node, so the graph replays cleanly);
 //   2. serve_kernel         persistent: every workgroup takes the next tile
 //                           from the step table with one atomic, runs the
-//                           tile body of its job (plain or precision, any raw
-//                           storage) and writes the job's partial image for
+//                           tile body of its job (plain; precision with the
+//                           range-profile or matrix engine; any raw storage)
+//                           and writes the job's partial image for
 //                           that (pixel tile, chirp range); all workgroups
 //                           leave once the counter passes the tile count;
 //   3. serve_finalize_kernel sums each finished job's partial images in
@@ -23,29 +24,47 @@
 namespace tdc {
 namespace {
 
-__global__ void serve_reset_kernel(unsigned int *counter) { *counter = 0U; }
+__global__ void serve_reset_kernel(unsigned int *counter)
+{
+    counter[0] = 0U;
+    counter[1] = 0U;
+}
 
+// FAMILY 0 runs the table's first nfast tiles (plain form and range-profile
+// engine, at most ~62 VGPRs, 8 waves per SIMD), FAMILY 1 the rest (matrix
+// engine, which needs ~130 VGPRs); each family has its own kernel so that
+// the matrix tile's registers do not limit the occupancy of the others.
+template <int FAMILY>
 __global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restrict__ jobs,
                                                        const StepTable *__restrict__ table,
                                                        unsigned int *__restrict__ counter)
 {
     __shared__ int s_tile;
     const cptr<StepTable> tab = as_const(table);
-    const int ntiles = tab->ntiles;
+    const int first = FAMILY == 0 ? 0 : tab->nfast;
+    const int count = FAMILY == 0 ? tab->nfast : tab->ntiles - tab->nfast;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(counter, 1U));
+        if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(counter + FAMILY, 1U));
         __syncthreads();
         const int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
-        if (t >= ntiles) break; // every wave of the workgroup takes this exit
-        const ServeTile tl = tab->tiles[t];
+        if (t >= count) break; // every wave of the workgroup takes this exit
+        const ServeTile tl = tab->tiles[first + t];
         const ServeJob j = as_const(jobs)[tl.job];
         const int n0 = tl.split * j.cps;
         const int n1 = min(j.npri, n0 + j.cps);
-        if (j.form == 0) {
-            range_tile<1>(static_cast<const float4 *>(j.raw), j.cpos32, j.grid32, j.npx, j.ns, n0,
-                          n1, tl.pix0,
-                          static_cast<float2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx);
+        if (FAMILY == 0) {
+            if (j.form == 0) {
+                range_tile<1>(static_cast<const float4 *>(j.raw), j.cpos32, j.grid32, j.npx, j.ns,
+                              n0, n1, tl.pix0,
+                              static_cast<float2 *>(j.partial) +
+                                  static_cast<size_t>(tl.split) * j.npx);
+            } else {
+                profile_tile(static_cast<const float2 *>(j.raw), j.cpos, j.grid64, j.pp, j.npx, n0,
+                             n1, tl.pix0,
+                             static_cast<double2 *>(j.partial) +
+                                 static_cast<size_t>(tl.split) * j.npx);
+            }
         } else {
             double2 *out = static_cast<double2 *>(j.partial) + static_cast<size_t>(tl.split) * j.npx;
             const auto *atab = static_cast<const uintx4_t *>(j.raw);
@@ -120,7 +139,8 @@ hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsig
                              int workgroups, hipStream_t s)
 {
     hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL(serve_kernel, dim3(workgroups), dim3(kBlock), 0, s, jobs, table, counter);
+    hipLaunchKernelGGL(serve_kernel<0>, dim3(workgroups), dim3(kBlock), 0, s, jobs, table, counter);
+    hipLaunchKernelGGL(serve_kernel<1>, dim3(workgroups), dim3(kBlock), 0, s, jobs, table, counter);
     return hipGetLastError();
 }
 

@@ -50,7 +50,13 @@ namespace {
 
 constexpr double kC = 299792458.0;
 constexpr double kDefaultStepTerms = 4e10;
-constexpr double kTileTerms = 8388608.0; // ~2^23 terms per tile
+constexpr double kTileTerms = 8388608.0; // ~2^23 work units per tile
+// Work units per term of each tile body, one unit being a term of the matrix
+// engine (measured on 1024 x 512 x 512^2: matrix 3.76 ms, plain per-chirp range
+// 30.7 ms) and, for the range-profile engine, units per (pixel, chirp)
+// (0.71 ms there: ~97 matrix terms), so that a step's budget is device time.
+constexpr double kPlainTermCost = 8.0;
+constexpr double kProfilePairCost = 97.0;
 constexpr int kFinWorkgroups = 256;
 constexpr int kBuckets[] = {256, 512, 1024, 2048};
 constexpr int kNumBuckets = 4;
@@ -124,9 +130,14 @@ struct Job {
     int npx = 0, tile_px = 0, npt = 0, cps = 0, nsplit = 0, nfin = 0;
     int64_t ntiles = 0, next_tile = 0;
     double terms = 0.0;
+    double pair_cost = 0.0; // work units per (pixel, chirp)
+    // precision jobs: range-profile engine (chirps of at most 1024 samples) or
+    // matrix engine
+    bool profile = false;
+    tdc::ProfileParams pp{};
     // device arena layout: [upload region | staged records or raw | partials]
     size_t off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
-           off_cpos32 = 0,
+           off_cpos32 = 0, off_pcorr = 0, off_ptw = 0, off_ptab = 0,
            off_atab = 0, off_ascale = 0, off_part = 0, up_bytes = 0, bytes = 0;
     bool needs_stage = false;
     char *arena = nullptr;
@@ -208,9 +219,12 @@ void plan(Job &j)
     const size_t npx = static_cast<size_t>(d.n_rows) * d.n_cols;
     j.npx = static_cast<int>(npx);
     const bool prec = d.form == TDC_FORM_PRECISION;
+    const int plen = tdc::profile_length(d.n_samples);
+    j.profile = prec && plen <= tdc::kProfileMaxM;
+    j.pair_cost = j.profile ? kProfilePairCost : (prec ? 1.0 : kPlainTermCost) * d.n_samples;
     j.tile_px = tdc::kServeTilePixels;
     j.npt = (j.npx + j.tile_px - 1) / j.tile_px;
-    const double per_chirp = static_cast<double>(std::min(j.npx, j.tile_px)) * d.n_samples;
+    const double per_chirp = static_cast<double>(std::min(j.npx, j.tile_px)) * j.pair_cost;
     j.cps = std::max(1, static_cast<int>(kTileTerms / std::max(per_chirp, 1.0)));
     j.cps = std::min(j.cps, d.n_pri);
     j.nsplit = (d.n_pri + j.cps - 1) / j.cps;
@@ -230,6 +244,12 @@ void plan(Job &j)
             j.off_ones = o;
             o += align256(4 * static_cast<size_t>(d.n_samples));
         }
+        if (j.profile) {
+            j.off_pcorr = o;
+            o += align256(4 * static_cast<size_t>(d.n_samples));
+            j.off_ptw = o;
+            o += align256(sizeof(float2) * static_cast<size_t>(plen / 2));
+        }
     } else {
         j.off_kt = o;
         o += align256(4 * static_cast<size_t>(d.n_samples));
@@ -246,7 +266,10 @@ void plan(Job &j)
     } else if (d.raw_storage != TDC_RAW_C64) {
         o += align256(4 * nterm);
     }
-    if (prec) {
+    if (j.profile) {
+        j.off_ptab = o;
+        o += align256(sizeof(float2) * static_cast<size_t>(d.n_pri) * (plen + tdc::kProfilePad));
+    } else if (prec) {
         j.off_atab = o;
         o += align256(tdc::matrix_table_bytes(d.n_pri, d.n_samples, d.raw_storage));
         j.off_ascale = o;
@@ -317,6 +340,36 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
             auto *w = reinterpret_cast<float *>(h + j.off_ones);
             for (int m = 0; m < d.n_samples; ++m) w[m] = 1.0F;
         }
+        if (j.profile) {
+            // the processor's range-profile set-up: chirp positions from an
+            // origin inside the scene with their squared norms, the kernel's
+            // correction and the FFT's twiddles
+            const int ns = d.n_samples;
+            const int plen = tdc::profile_length(ns);
+            const size_t last = 3 * (static_cast<size_t>(j.npx) - 1);
+            double o[3];
+            for (int a = 0; a < 3; ++a) {
+                o[a] = 0.5 * (d.grid[a] + d.grid[last + a]);
+                if (!std::isfinite(o[a])) o[a] = 0.0;
+            }
+            for (int n = 0; n < d.n_pri; ++n) {
+                const double x = hc[n].x - o[0], y = hc[n].y - o[1], z = hc[n].z - o[2];
+                hc[n] = tdc::ChirpPos{x, y, z, std::fma(z, z, std::fma(y, y, x * x))};
+            }
+            std::vector<float> corr;
+            std::vector<float2> tw;
+            tdc::profile_tables(ns, plen, corr, tw);
+            std::memcpy(h + j.off_pcorr, corr.data(), 4 * corr.size());
+            std::memcpy(h + j.off_ptw, tw.data(), sizeof(float2) * tw.size());
+            const double dt = ns > 1 ? (d.sample_times[ns - 1] - d.sample_times[0]) / (ns - 1) : 0.0;
+            j.pp.dk = dk;
+            j.pp.kh = 2.0 * (d.start_freq + d.mod_rate * (d.sample_times[0] + (ns / 2) * dt)) / kC;
+            j.pp.ox = o[0];
+            j.pp.oy = o[1];
+            j.pp.oz = o[2];
+            j.pp.M = plen;
+            j.pp.Mp = plen + tdc::kProfilePad;
+        }
     } else {
         auto *kt = reinterpret_cast<float *>(h + j.off_kt);
         for (int m = 0; m < d.n_samples; ++m)
@@ -353,7 +406,12 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
     sj.form = prec ? 1 : 0;
     sj.storage = d.raw_storage;
     sj.partial = j.arena + j.off_part;
-    if (prec) {
+    if (j.profile) {
+        sj.raw = j.arena + j.off_ptab;
+        sj.grid64 = reinterpret_cast<const double *>(j.arena + j.off_grid);
+        sj.pp = j.pp;
+        sj.profile = 1;
+    } else if (prec) {
         sj.raw = j.arena + j.off_atab;
         sj.inv_scale = reinterpret_cast<const float *>(j.arena + j.off_ascale);
         sj.grid64 = reinterpret_cast<const double *>(j.arena + j.off_grid);
@@ -376,7 +434,7 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
     const tdc_job_desc &d = j.d;
     if (d.form == TDC_FORM_PRECISION) {
         // raw samples in their storage format (complex64 as uploaded), then
-        // the matrix engine's A tiles
+        // the range-profile table or the matrix engine's A tiles
         const void *staged = j.arena;
         if (d.raw_storage != TDC_RAW_C64) {
             hipError_t e = tdc::launch_stage_precision(
@@ -386,6 +444,11 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
             if (e != hipSuccess) return e;
             staged = j.arena + j.off_stage;
         }
+        if (j.profile)
+            return tdc::launch_stage_profile(
+                d.raw_storage, staged, reinterpret_cast<const float *>(j.arena + j.off_pcorr),
+                reinterpret_cast<const float2 *>(j.arena + j.off_ptw), d.n_pri, d.n_samples,
+                j.pp.M, j.pp.Mp, reinterpret_cast<float2 *>(j.arena + j.off_ptab), ss);
         return tdc::launch_stage_matrix(d.raw_storage, staged, d.n_pri, d.n_samples,
                                         tdc::matrix_parts(d.raw_storage), j.arena + j.off_atab,
                                         reinterpret_cast<float *>(j.arena + j.off_ascale), ss);
@@ -559,6 +622,8 @@ void scheduler(tdc_server *s)
         st.buf = static_cast<int>(step_index % kDepth);
         tdc::StepTable *tab = s->h_table[st.buf];
         int nt = 0, nf = 0;
+        // matrix-engine tiles go after the others (their own persistent kernel)
+        std::vector<tdc::ServeTile> mtiles;
         double work = 0.0;
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -577,11 +642,14 @@ void scheduler(tdc_server *s)
                     const int64_t t = j->next_tile++;
                     const int split = static_cast<int>(t / j->npt);
                     const int ptile = static_cast<int>(t % j->npt);
-                    tab->tiles[nt++] = tdc::ServeTile{j->slot, ptile * j->tile_px, split, 0};
+                    const tdc::ServeTile tl{j->slot, ptile * j->tile_px, split, 0};
+                    if (j->d.form == TDC_FORM_PRECISION && !j->profile) mtiles.push_back(tl);
+                    else tab->tiles[nt - static_cast<int>(mtiles.size())] = tl;
+                    ++nt;
                     const int n0 = split * j->cps;
                     const int nch = std::min(j->d.n_pri, n0 + j->cps) - n0;
                     const int npx = std::min(j->tile_px, j->npx - ptile * j->tile_px);
-                    const double w = static_cast<double>(npx) * nch * j->d.n_samples;
+                    const double w = static_cast<double>(npx) * nch * j->pair_cost;
                     work += w;
                     got += w;
                     took = true;
@@ -624,6 +692,8 @@ void scheduler(tdc_server *s)
             }
             continue;
         }
+        tab->nfast = nt - static_cast<int>(mtiles.size());
+        std::copy(mtiles.begin(), mtiles.end(), tab->tiles + tab->nfast);
         tab->ntiles = nt;
         tab->nfin = nf;
         int bucket = kNumBuckets - 1;

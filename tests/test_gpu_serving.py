@@ -41,6 +41,37 @@ def test_single_job_parity(tdc, orc, form, storage, tol):
     assert info["latency_ms"] > 0 and info["tiles"] >= 1
 
 
+def test_long_chirp_job_parity(tdc, orc):
+    """1100 samples per chirp: beyond the range-profile engine's 1024, the job
+    runs on the matrix engine's tiles (their own persistent kernel)."""
+    s = job_scene(2, 60, 1100, 40)
+    srv = tdc.Server(0, 8e9)
+    img, _ = srv.wait(submit(srv, s))
+    srv.close()
+    e = orc.rel_l2(img.reshape(-1), oracle_img(orc, s).reshape(-1))
+    print("matrix-engine job rel-L2", e)
+    assert e < 1e-4
+
+
+def test_engines_bit_identical_alone_and_in_a_mix(tdc):
+    """A range-profile job and a matrix-engine job, each alone and then in one
+    mix with each other and a plain job: every image is the same bit for bit."""
+    prof = job_scene(11, 150, 256, 80)
+    mat = job_scene(12, 40, 1100, 48)
+    plain = job_scene(13, 120, 128, 64)
+    alone = []
+    for s in (prof, mat):
+        srv = tdc.Server(0, 8e9)
+        alone.append(srv.wait(submit(srv, s))[0])
+        srv.close()
+    srv = tdc.Server(0, 8e9)
+    ids = [submit(srv, mat), submit(srv, plain, form="plain"), submit(srv, prof)]
+    mixed = [srv.wait(i)[0] for i in ids]
+    srv.close()
+    assert np.array_equal(alone[0], mixed[2])
+    assert np.array_equal(alone[1], mixed[0])
+
+
 def test_job_is_bit_identical_alone_and_in_a_mix(tdc):
     target = job_scene(7, 300, 256, 90)
     others = [job_scene(100 + k, nc, ns, side) for k, (nc, ns, side) in
