@@ -198,6 +198,14 @@ def serve_leg(tdc, args):
         f"p99 {rec['p99_ms']:.2f} ms, {rec['gterms']:.1f} G terms/s over {rec['makespan_s']:.2f} s "
         f"(device busy {rec['device_busy_s']:.2f} s, {rec['steps']} steps)")
     out["precision"] = rec
+    if args.serve_capacity_rate > 0:
+        # the same jobs offered faster, to show throughput and latency near the
+        # device's capacity
+        rec_c, _ = serve_stream(tdc, scenes, args.serve_capacity_rate, "precision", seed=1)
+        log(f"serving precision at {rec_c['rate']:.0f}/s: p50 {rec_c['p50_ms']:.2f} ms, "
+            f"p99 {rec_c['p99_ms']:.2f} ms, {rec_c['gterms']:.1f} G terms/s over "
+            f"{rec_c['makespan_s']:.2f} s (device busy {rec_c['device_busy_s']:.2f} s)")
+        out["precision_capacity"] = rec_c
     n_plain = min(len(scenes), args.serve_plain_jobs)
     if n_plain > 0:
         rec_p, _ = serve_stream(tdc, scenes[:n_plain], args.serve_rate, "plain")
@@ -243,6 +251,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2400.0, help="offered jobs/s")
     ap.add_argument("--serve-p99-target-ms", type=float, default=60.0,
                     help="stated p99 latency target of the serving stream")
+    ap.add_argument("--serve-capacity-rate", type=float, default=9600.0,
+                    help="offered jobs/s of a second precision stream of the same jobs (0: skip)")
     ap.add_argument("--serve-plain-jobs", type=int, default=200,
                     help="jobs of the stream also served in the plain form")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -494,6 +504,12 @@ def main():
             pr["gterms"] * 1e9 * FLOP_PER_TERM["precision"] / (PEAK_FP32_TFLOPS * 1e12), 4)
         out["serving"]["precision"]["p99_target_ms"] = args.serve_p99_target_ms
         out["serving"]["precision"]["p99_target_met"] = bool(pr["p99_ms"] <= args.serve_p99_target_ms)
+        if "precision_capacity" in serving:
+            pc = serving["precision_capacity"]
+            out["serving"]["precision_capacity"]["fp32_peak_frac"] = round(
+                pc["gterms"] * 1e9 * FLOP_PER_TERM["precision"] / (PEAK_FP32_TFLOPS * 1e12), 4)
+            out["serving"]["precision_capacity"]["p99_target_met"] = bool(
+                pc["p99_ms"] <= args.serve_p99_target_ms)
     if "plain" in results and "c64" in results:
         out["precision_vs_plain"] = round(results["c64"]["gterms"] / results["plain"]["gterms"], 3)
     if "vector" in results and "matrix" in results:

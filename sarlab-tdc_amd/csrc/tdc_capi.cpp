@@ -15,6 +15,9 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_set>
 #include <vector>
@@ -524,11 +527,10 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
             return fail(TDC_ERR_UNSUPPORTED,
                         "the range-profile engine takes at most " +
                             std::to_string(tdc::kProfileMaxM / 2) + " samples per chirp");
-        std::vector<float> corr;
-        std::vector<float2> tw;
-        tdc::profile_tables(ns, plen, corr, tw);
-        if ((rc = upload(h->d_pcorr, corr.data(), corr.size(), s)) != TDC_OK) return rc;
-        if ((rc = upload(h->d_ptw, tw.data(), tw.size(), s)) != TDC_OK) return rc;
+        const auto tabs = tdc::profile_tables(ns, plen);
+        if ((rc = upload(h->d_pcorr, tabs->corr.data(), tabs->corr.size(), s)) != TDC_OK) return rc;
+        if ((rc = upload(h->d_ptw, tabs->twiddle.data(), tabs->twiddle.size(), s)) != TDC_OK)
+            return rc;
         const int mp = plen + tdc::kProfilePad;
         TDC_HIP(h->d_ptab.ensure(sizeof(float2) * static_cast<size_t>(npri) * mp));
         TDC_HIP(tdc::launch_stage_profile(h->storage, h->d_rec.p, h->d_pcorr.as<float>(),
@@ -596,10 +598,6 @@ void clear_last_error() { t_err.clear(); }
 // advances by a constant phase step, so the sample times must be uniformly
 // spaced (a few float32 ulps of the largest time are allowed, so that times
 // given in float32 qualify; each anchor uses its own exact sample time).
-// Host tables of the range-profile engine for ns samples and an M-point FFT:
-// corr[i] = 1 / psi_hat((i - ns/2) / M), psi_hat the Fourier transform of the
-// truncated Gaussian exp(-b u^2), |u| <= taps/2 (64-point Gauss-Legendre
-// quadrature in FP64); twiddle[k] = exp(+j 2 pi k / M), k < M / 2.
 namespace {
 struct GaussLegendre64 {
     static constexpr int kNodes = 64;
@@ -629,29 +627,47 @@ struct GaussLegendre64 {
 };
 } // namespace
 
-void profile_tables(int ns, int M, std::vector<float> &corr, std::vector<float2> &twiddle)
+std::shared_ptr<const ProfileTables> profile_tables(int ns, int M)
 {
+    // computed once per (ns, M) in the process: the tables depend on nothing else
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::shared_ptr<const ProfileTables>> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find({ns, M});
+        if (it != cache.end()) return it->second;
+    }
     static const GaussLegendre64 gl;
     constexpr int kNodes = GaussLegendre64::kNodes;
-    const double *node = gl.node, *weight = gl.weight;
     const double half = kProfileTaps / 2;
     const double b = 0.75 * kPi / half;
-    corr.resize(ns);
-    const int h = ns / 2;
-    for (int i = 0; i < ns; ++i) {
-        const double nu = double(i - h) / M;
-        double v = 0.0;
-        for (int k = 0; k < kNodes; ++k) {
-            const double u = half * node[k];
-            v += half * weight[k] * std::exp(-b * u * u) * std::cos(2.0 * kPi * nu * u);
-        }
-        corr[i] = static_cast<float>(1.0 / v);
+    // psi_hat is even in nu and the integrand even in u: sum the nodes of one
+    // half twice, once per distance |i - h|
+    double wexp[kNodes / 2], unode[kNodes / 2];
+    for (int k = 0; k < kNodes / 2; ++k) {
+        const double u = half * gl.node[k];
+        unode[k] = u;
+        wexp[k] = 2.0 * half * gl.weight[k] * std::exp(-b * u * u);
     }
-    twiddle.resize(M / 2);
+    auto tab = std::make_shared<ProfileTables>();
+    const int h = ns / 2;
+    const int far = std::max(h, ns - 1 - h);
+    std::vector<float> by_dist(far + 1);
+    for (int d = 0; d <= far; ++d) {
+        const double nu = double(d) / M;
+        double v = 0.0;
+        for (int k = 0; k < kNodes / 2; ++k) v += wexp[k] * std::cos(2.0 * kPi * nu * unode[k]);
+        by_dist[d] = static_cast<float>(1.0 / v);
+    }
+    tab->corr.resize(ns);
+    for (int i = 0; i < ns; ++i) tab->corr[i] = by_dist[std::abs(i - h)];
+    tab->twiddle.resize(M / 2);
     for (int k = 0; k < M / 2; ++k) {
         const double a = 2.0 * kPi * k / M;
-        twiddle[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+        tab->twiddle[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
     }
+    std::lock_guard<std::mutex> lk(mu);
+    return cache.emplace(std::make_pair(ns, M), std::move(tab)).first->second;
 }
 
 bool precision_tables(const double *t, int ns, double f0, double krate,

@@ -3,6 +3,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -193,7 +194,16 @@ class Range {
 // The thread-local message returned by tdc_last_error().
 int set_last_error(int code, const std::string &msg);
 void clear_last_error();
-void profile_tables(int ns, int M, std::vector<float> &corr, std::vector<float2> &twiddle);
+// Host tables of the range-profile engine for ns samples and an M-point FFT:
+// corr[i] = 1 / psi_hat((i - ns/2) / M), psi_hat the Fourier transform of the
+// truncated Gaussian exp(-b u^2), |u| <= taps/2 (64-point Gauss-Legendre
+// quadrature in FP64); twiddle[k] = exp(+j 2 pi k / M), k < M / 2. Shared,
+// computed once per (ns, M).
+struct ProfileTables {
+    std::vector<float> corr;
+    std::vector<float2> twiddle;
+};
+std::shared_ptr<const ProfileTables> profile_tables(int ns, int M);
 bool precision_tables(const double *t, int ns, double f0, double krate,
                       std::vector<double> &kanchor, double &dk, std::string &why);
 

@@ -62,7 +62,7 @@ constexpr int kBuckets[] = {256, 512, 1024, 2048};
 constexpr int kNumBuckets = 4;
 // Upload threads: packing a job into pinned memory runs at host memcpy speed,
 // so several jobs are packed and copied at once.
-constexpr int kUploaders = 4;
+constexpr int kUploaders = 8;
 // Steps in flight: each has its own tile table, counter and events, so the
 // host builds and launches the next steps while earlier ones run and their
 // completions wait; steps alternate between two streams.
@@ -356,11 +356,9 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
                 const double x = hc[n].x - o[0], y = hc[n].y - o[1], z = hc[n].z - o[2];
                 hc[n] = tdc::ChirpPos{x, y, z, std::fma(z, z, std::fma(y, y, x * x))};
             }
-            std::vector<float> corr;
-            std::vector<float2> tw;
-            tdc::profile_tables(ns, plen, corr, tw);
-            std::memcpy(h + j.off_pcorr, corr.data(), 4 * corr.size());
-            std::memcpy(h + j.off_ptw, tw.data(), sizeof(float2) * tw.size());
+            const auto tabs = tdc::profile_tables(ns, plen);
+            std::memcpy(h + j.off_pcorr, tabs->corr.data(), 4 * tabs->corr.size());
+            std::memcpy(h + j.off_ptw, tabs->twiddle.data(), sizeof(float2) * tabs->twiddle.size());
             const double dt = ns > 1 ? (d.sample_times[ns - 1] - d.sample_times[0]) / (ns - 1) : 0.0;
             j.pp.dk = dk;
             j.pp.kh = 2.0 * (d.start_freq + d.mod_rate * (d.sample_times[0] + (ns / 2) * dt)) / kC;
