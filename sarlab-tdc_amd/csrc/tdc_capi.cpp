@@ -9,6 +9,7 @@
 // reused across calls (the reference reallocates every buffer on every
 // start(), tdcprocessor.cpp:65-66).
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <chrono>
 #include <cmath>
@@ -183,7 +184,9 @@ bool all_finite(const float *p, size_t n)
 struct tdc_processor {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
+    // per-iteration (start, main kernel end, step end) events of tdc_run: the
+    // iterations are enqueued back to back and timed after one synchronisation
+    std::vector<std::array<hipEvent_t, 3>> ev;
 
     // FP32 drop-in inputs (borrowed)
     const float *raw = nullptr;
@@ -755,9 +758,6 @@ int tdc_create(int gpu_num, tdc_processor **out)
     h->device = gpu_num;
     e = hipSetDevice(gpu_num);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&h->ev_a);
-    if (e == hipSuccess) e = hipEventCreate(&h->ev_b);
-    if (e == hipSuccess) e = hipEventCreate(&h->ev_c);
     if (e != hipSuccess) {
         tdc_destroy(h);
         return fail(TDC_ERR_DEVICE, std::string("Failed to set HIP device: ") + hipGetErrorString(e));
@@ -771,9 +771,9 @@ int tdc_destroy(tdc_processor *h)
     if (!h) return ok();
     (void) hipSetDevice(h->device);
     if (h->stream) (void) hipStreamSynchronize(h->stream);
-    if (h->ev_a) (void) hipEventDestroy(h->ev_a);
-    if (h->ev_b) (void) hipEventDestroy(h->ev_b);
-    if (h->ev_c) (void) hipEventDestroy(h->ev_c);
+    for (auto &t : h->ev)
+        for (hipEvent_t x : t)
+            if (x) (void) hipEventDestroy(x);
     if (h->stream) (void) hipStreamDestroy(h->stream);
     delete h;
     return ok();
@@ -990,10 +990,38 @@ int tdc_run(tdc_processor *h, int iterations)
     hipStream_t s = h->stream;
     double main_ms = 0.0, total_ms = 0.0;
     int wgs = 0;
+    constexpr int kEventRing = 64;
+    while (static_cast<int>(h->ev.size()) < std::min(iterations, kEventRing)) {
+        std::array<hipEvent_t, 3> t{nullptr, nullptr, nullptr};
+        for (hipEvent_t &x : t) {
+            const hipError_t e = hipEventCreate(&x);
+            if (e != hipSuccess) {
+                for (hipEvent_t y : t)
+                    if (y) (void) hipEventDestroy(y);
+                return fail(TDC_ERR_DEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e));
+            }
+        }
+        h->ev.push_back(t);
+    }
+    int pending = 0;
+    auto drain = [&]() -> int {
+        if (pending == 0) return TDC_OK;
+        TDC_HIP(hipEventSynchronize(h->ev[pending - 1][2]));
+        for (int k = 0; k < pending; ++k) {
+            float a = 0.0F, b = 0.0F;
+            TDC_HIP(hipEventElapsedTime(&a, h->ev[k][0], h->ev[k][1]));
+            TDC_HIP(hipEventElapsedTime(&b, h->ev[k][0], h->ev[k][2]));
+            main_ms += a;
+            total_ms += b;
+        }
+        pending = 0;
+        return TDC_OK;
+    };
     for (int it = 0; it < iterations; ++it) {
         if (g.npx == 0) break;
+        const std::array<hipEvent_t, 3> &evt = h->ev[pending];
         if (h->doppler) TDC_HIP(hipMemsetAsync(h->d_counters.p, 0, 2 * sizeof(unsigned long long), s));
-        TDC_HIP(hipEventRecord(h->ev_a, s));
+        TDC_HIP(hipEventRecord(evt[0], s));
         if (h->run_form == TDC_FORM_PLAIN) {
             float2 *dst = g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>();
             const float4 *cpos = h->d_chirp.as<float4>();
@@ -1013,7 +1041,7 @@ int tdc_run(tdc_processor *h, int iterations)
                 TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), h->d_grid.as<float>(),
                                           g, dst, s, &wgs));
             }
-            TDC_HIP(hipEventRecord(h->ev_b, s));
+            TDC_HIP(hipEventRecord(evt[1], s));
             if (g.nsplit > 1)
                 TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
                                                  h->d_img.as<float2>(), s));
@@ -1033,18 +1061,14 @@ int tdc_run(tdc_processor *h, int iterations)
                                               h->d_grid.as<double>(), h->d_kanchor.as<double>(),
                                               h->dk, g, h->d_part.as<double2>(), s, &wgs));
             }
-            TDC_HIP(hipEventRecord(h->ev_b, s));
+            TDC_HIP(hipEventRecord(evt[1], s));
             TDC_HIP(tdc::launch_finalize_f64(h->d_part.as<double2>(), g.nsplit, g.npx,
                                              h->d_img.as<float2>(), s));
         }
-        TDC_HIP(hipEventRecord(h->ev_c, s));
-        TDC_HIP(hipEventSynchronize(h->ev_c));
-        float a = 0.0F, b = 0.0F;
-        TDC_HIP(hipEventElapsedTime(&a, h->ev_a, h->ev_b));
-        TDC_HIP(hipEventElapsedTime(&b, h->ev_a, h->ev_c));
-        main_ms += a;
-        total_ms += b;
+        TDC_HIP(hipEventRecord(evt[2], s));
+        if (++pending == static_cast<int>(h->ev.size()) && (rc = drain()) != TDC_OK) return rc;
     }
+    if ((rc = drain()) != TDC_OK) return rc;
     TDC_HIP(hipStreamSynchronize(s));
     tdc_stats &st = h->stats;
     st.main_kernel_ms = main_ms / iterations;
