@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2400.0, help="offered jobs/s")
     ap.add_argument("--serve-p99-target-ms", type=float, default=60.0,
                     help="stated p99 latency target of the serving stream")
-    ap.add_argument("--serve-capacity-rate", type=float, default=3600.0,
+    ap.add_argument("--serve-capacity-rate", type=float, default=9600.0,
                     help="offered jobs/s of a second precision stream of the same jobs (0: skip)")
     ap.add_argument("--serve-plain-jobs", type=int, default=200,
                     help="jobs of the stream also served in the plain form")
