@@ -13,11 +13,12 @@
 // resident jobs -- an equal share of the step's work for every job first,
 // then the rest in arrival order -- plus the finalize blocks of the jobs
 // whose last tiles are in it. Steps alternate between two compute streams,
-// each replaying the step graph of its grid-size bucket (counter reset +
-// persistent correlate-sum kernel), so the next step's workgroups fill the
-// CUs while the previous step drains. A job's finalize follows the step
-// holding its last tiles on that step's stream, after the step before it (on
-// the other stream) has ended too; that covers all of its tiles.
+// each replaying the step graph of its grid-size bucket (counter reset + two
+// persistent correlate-sum kernels, one per tile family), so the next step's
+// workgroups fill the CUs while the previous step drains. A job's finalize
+// follows the step holding its last tiles on that step's stream, after the
+// step before it (on the other stream) has ended too; that covers all of its
+// tiles.
 //
 // Nothing besides the steps needs a CU while steps run back to back: small
 // host-to-device copies would run as blit kernels and wait behind the
@@ -25,7 +26,8 @@
 // host memory that the kernels read directly (a few hundred bytes per tile),
 // finished images are written by the finalize straight into pinned host
 // memory, and the staging kernels of a job (plain records, FP16/BF16
-// conversion) run on the step stream ahead of the job's first step.
+// conversion, range-profile FFT tables or matrix A tiles) run on the step
+// stream ahead of the job's first step.
 #include <algorithm>
 #include <chrono>
 #include <climits>
