@@ -137,6 +137,29 @@ def test_admission_respects_the_budget(tdc, orc):
     assert e < 1e-4
 
 
+def test_arenas_are_recycled_within_the_budget(tdc):
+    """Jobs of alternating sizes under a budget that holds one large job: the
+    server reuses freed device blocks of a size and releases idle blocks of
+    another size before allocating, never holding more than the budget, and
+    every image equals the same job's image on a fresh server."""
+    small = [job_scene(40 + k, 120, 128, 48) for k in range(3)]
+    large = [job_scene(50 + k, 300, 256, 160) for k in range(3)]
+    srv = tdc.Server(0, 64e9)
+    alone = [srv.wait(submit(srv, s))[0] for s in (small[0], large[0])]
+    big = srv.wait(submit(srv, large[1]))[1]["device_bytes"]
+    srv.close()
+    srv = tdc.Server(0, big * 1.2)
+    order = [small[0], large[0], small[1], large[1], small[2], large[2]]
+    res = []
+    for s in order:  # one at a time: each size class is freed before the other returns
+        res.append(srv.wait(submit(srv, s))[0])
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 6 and st["failed"] == 0
+    assert st["bytes_peak"] <= big * 1.2
+    assert np.array_equal(res[0], alone[0]) and np.array_equal(res[1], alone[1])
+
+
 def test_small_jobs_overtake_a_large_one(tdc):
     """Jobs submitted while a large one runs join it on the device and finish first."""
     big = job_scene(30, 2048, 512, 768)
