@@ -95,7 +95,7 @@ struct DopParams {
 // kProfileTaps table entries around phi * M weighted by the Gaussian kernel
 // exp(-b u^2), b = 0.75 pi / (kProfileTaps / 2).
 constexpr int kProfileTaps = 12;
-constexpr int kProfileMaxM = 2048; // the FFT of one chirp runs in LDS
+constexpr int kProfileMaxM = 4096; // the FFT of one chirp runs in LDS (2 M entries, <= 64 KiB)
 constexpr int kProfilePad = 16;    // wrap-around entries after the M bins of a table row
 struct ProfileParams {
     double dk; // 2 K dt / c: cycles per sample per metre of range

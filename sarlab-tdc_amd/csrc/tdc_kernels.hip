@@ -495,13 +495,13 @@ __global__ __launch_bounds__(kBlock) void stage_profile_kernel(
     const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
     const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
 {
-    __shared__ float2 buf[2 * kProfileMaxM];
+    extern __shared__ float2 buf[]; // 2 M entries, sized at launch
     const int n = static_cast<int>(blockIdx.x);
     const int tid = static_cast<int>(threadIdx.x);
     const int h = ns / 2;
     const int mask = M - 1;
     float2 *x = buf;
-    float2 *y = buf + kProfileMaxM;
+    float2 *y = buf + M;
     for (int j = tid; j < M; j += kBlock) x[j] = make_float2(0.0F, 0.0F);
     __syncthreads();
     const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
@@ -883,18 +883,19 @@ hipError_t launch_stage_profile(int storage, const void *staged, const float *co
 {
     if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || Mp < M + kProfileTaps - 1)
         return hipErrorInvalidValue;
+    const size_t lds = 2 * sizeof(float2) * static_cast<size_t>(M);
     switch (storage) {
     case 0:
-        hipLaunchKernelGGL(stage_profile_kernel<RawC64>, dim3(npri), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(stage_profile_kernel<RawC64>, dim3(npri), dim3(kBlock), lds, s,
                            static_cast<const float2 *>(staged), corr, twiddle, ns, M, Mp, table);
         break;
     case 1:
-        hipLaunchKernelGGL(stage_profile_kernel<RawF16>, dim3(npri), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(stage_profile_kernel<RawF16>, dim3(npri), dim3(kBlock), lds, s,
                            static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
                            table);
         break;
     case 2:
-        hipLaunchKernelGGL(stage_profile_kernel<RawBF16>, dim3(npri), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(stage_profile_kernel<RawBF16>, dim3(npri), dim3(kBlock), lds, s,
                            static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
                            table);
         break;

@@ -298,10 +298,11 @@ def test_matrix_engine_sample_counts(tdc, orc, ns):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
-@pytest.mark.parametrize("ns", [1, 2, 7, 8, 9, 64, 65, 127, 300, 512, 513, 1000, 1024])
+@pytest.mark.parametrize("ns", [1, 2, 7, 8, 9, 64, 65, 127, 300, 512, 513, 1000, 1024, 1025,
+                                1500, 2048])
 def test_profile_engine_sample_counts(tdc, orc, ns):
-    """FFT lengths 16..2048 (M >= 2 ns, at least 16), odd and even re-centring
-    samples, one-sample chirps, and the 1024-sample limit."""
+    """FFT lengths 16..4096 (M >= 2 ns, at least 16), odd and even re-centring
+    samples, one-sample chirps, and the 2048-sample limit."""
     s = ragged_scene(npri=40, ns=ns, rows=9, cols=11, seed=ns)
     img, st = form(tdc, f64=s.f64(), kind="precision", engine="profile")
     assert st["engine"] == "profile"
@@ -330,10 +331,10 @@ def test_profile_engine_window_and_gather_paths(tdc, orc, spacing):
 
 
 def test_profile_engine_sample_limit(tdc, orc):
-    """Beyond 1024 samples per chirp the range-profile engine declines with a
+    """Beyond 2048 samples per chirp the range-profile engine declines with a
     message and the automatic choice takes the matrix engine."""
-    s = ragged_scene(npri=12, ns=1025, rows=5, cols=6, seed=3)
-    with pytest.raises(Exception, match="at most 1024 samples"):
+    s = ragged_scene(npri=12, ns=2049, rows=5, cols=6, seed=3)
+    with pytest.raises(Exception, match="at most 2048 samples"):
         form(tdc, f64=s.f64(), kind="precision", engine="profile")
     img, st = form(tdc, f64=s.f64(), kind="precision")
     assert st["engine"] == "matrix"
