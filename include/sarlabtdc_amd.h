@@ -80,7 +80,8 @@ typedef struct tdc_processor tdc_processor;
 #define TDC_PER_CHIRP_WINDOW 2
 
 typedef struct tdc_stats {
-    double kernel_ms;       /* device time of the last run's hot-path kernels (HIP events) */
+    double kernel_ms;       /* device time of one pass of the last run: staging, correlate-sum and
+                               split reduction kernels (HIP events) */
     double main_kernel_ms;  /* device time of the correlate-sum kernel alone */
     double end_to_end_ms;   /* host wall time of the last tdc_start (H2D + compute + D2H) */
     double terms;           /* pixels * chirps * samples of the last run */
@@ -171,10 +172,12 @@ int tdc_set_raw_data_f64(tdc_processor *h, const float *raw_iq, const double *pr
 int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int n_cols);
 
 /* Split of tdc_start for callers that keep inputs resident in HBM:
- * tdc_prepare uploads and stages the inputs; tdc_run forms the image on the
- * device `iterations` times (each run recomputes it from the staged inputs);
- * tdc_fetch_image copies it into the image buffer. tdc_start is exactly
- * prepare + run(1) + fetch. */
+ * tdc_prepare uploads the inputs and sizes the device buffers; tdc_run forms
+ * the image on the device `iterations` times, each one a full pass over the
+ * resident inputs (the kernels that fold windows into the raw samples and
+ * build per-chirp records, then the correlate-sum, then the split
+ * reduction); tdc_fetch_image copies it into the image buffer. tdc_start is
+ * exactly prepare + run(1) + fetch. */
 int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
                 int dop_centroid_win, int apply_range_win, int doppler_win_taper);
 int tdc_run(tdc_processor *h, int iterations);

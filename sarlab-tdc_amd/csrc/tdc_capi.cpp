@@ -184,9 +184,10 @@ bool all_finite(const float *p, size_t n)
 struct tdc_processor {
     int device = 0;
     hipStream_t stream = nullptr;
-    // per-iteration (start, main kernel end, step end) events of tdc_run: the
-    // iterations are enqueued back to back and timed after one synchronisation
-    std::vector<std::array<hipEvent_t, 3>> ev;
+    // per-iteration (start, staging end, main kernel end, step end) events of
+    // tdc_run: the iterations are enqueued back to back and timed after one
+    // synchronisation
+    std::vector<std::array<hipEvent_t, 4>> ev;
 
     // FP32 drop-in inputs (borrowed)
     const float *raw = nullptr;
@@ -234,6 +235,9 @@ struct tdc_processor {
     tdc::Geom geom{};
     tdc::DopParams dp{};
     tdc::ProfileParams pp{};
+    // Doppler staging parameters: 2 / lambda_c, centroid or fixed centre
+    float st_tol = 0.0F, st_fc = 0.0F;
+    int st_centroid = 0;
     int ppl = 1;
     double dk = 0.0;
     int64_t trivial_processed = 0;
@@ -388,6 +392,56 @@ template <class T> int upload(DevBuf &b, const T *src, size_t count, hipStream_t
     return TDC_OK;
 }
 
+// Kernels that derive the device records from the raw samples and antenna
+// states (window folding, storage conversion, per-chirp spectra or MFMA
+// operands): part of every pass over a batch of input, so each tdc_run
+// iteration launches them ahead of the correlate-sum.
+int stage_inputs(tdc_processor *h)
+{
+    const int npri = h->npri, ns = h->ns;
+    hipStream_t s = h->stream;
+    if (h->run_form == TDC_FORM_PLAIN) {
+        if (h->chirp_path) {
+            TDC_HIP(tdc::launch_stage_terms(h->d_raw.as<float>(), h->d_kt.as<float>(),
+                                            h->d_wr.as<float>(), npri, ns, h->d_rec.as<float4>(),
+                                            s));
+            float4 *cpos = h->d_chirp.as<float4>();
+            if (h->doppler) {
+                TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), h->d_vel.as<float>(),
+                                                 h->d_att.as<float>(), npri, ns, h->st_tol,
+                                                 h->bore, h->st_centroid, h->st_fc, cpos,
+                                                 cpos + npri, s));
+            } else {
+                TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), nullptr, nullptr, npri, ns,
+                                                 0.0F, nullptr, 0, 0.0F, cpos, nullptr, s));
+            }
+        } else if (h->doppler) {
+            TDC_HIP(tdc::launch_stage_doppler(h->d_raw.as<float>(), h->d_pos.as<float>(),
+                                              h->d_vel.as<float>(), h->d_att.as<float>(),
+                                              h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
+                                              h->st_tol, h->bore, h->st_centroid, h->st_fc,
+                                              h->d_rec.as<tdc::DopRec>(), s));
+        } else {
+            TDC_HIP(tdc::launch_stage_plain(h->d_raw.as<float>(), h->d_pos.as<float>(),
+                                            h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
+                                            h->d_rec.as<tdc::PlainRec>(), s));
+        }
+        return TDC_OK;
+    }
+    TDC_HIP(tdc::launch_stage_precision(h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(),
+                                        npri, ns, h->d_rec.p, s));
+    if (h->run_engine == TDC_ENGINE_PROFILE) {
+        TDC_HIP(tdc::launch_stage_profile(h->run_storage, h->d_rec.p, h->d_pcorr.as<float>(),
+                                          h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp,
+                                          h->d_ptab.as<float2>(), s));
+    } else if (h->run_engine == TDC_ENGINE_MATRIX) {
+        TDC_HIP(tdc::launch_stage_matrix(h->run_storage, h->d_rec.p, npri, ns,
+                                         tdc::matrix_parts(h->run_storage), h->d_atab.p,
+                                         h->d_ascale.as<float>(), s));
+    }
+    return TDC_OK;
+}
+
 int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroid, bool apply_rw,
                   bool taper)
 {
@@ -417,24 +471,14 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         h->chirp_path = antenna_constant_per_chirp(h);
         if ((rc = upload(h->d_vel, h->vel, nterm * 3, s)) != TDC_OK) return rc;
         if ((rc = upload(h->d_att, h->att, nterm * 4, s)) != TDC_OK) return rc;
-        const float tol = 2.0F / h->lambda_c; // windowing.h:21, :33
+        h->st_tol = 2.0F / h->lambda_c; // windowing.h:21, :33
+        h->st_centroid = centroid ? 1 : 0;
+        h->st_fc = dop_center;
         if (h->chirp_path) {
             TDC_HIP(h->d_rec.ensure(sizeof(float4) * nterm));
             TDC_HIP(h->d_chirp.ensure(sizeof(float4) * 2 * static_cast<size_t>(npri)));
-            TDC_HIP(tdc::launch_stage_terms(h->d_raw.as<float>(), h->d_kt.as<float>(),
-                                            h->d_wr.as<float>(), npri, ns, h->d_rec.as<float4>(),
-                                            s));
-            TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), h->d_vel.as<float>(),
-                                             h->d_att.as<float>(), npri, ns, tol, h->bore,
-                                             centroid ? 1 : 0, dop_center, h->d_chirp.as<float4>(),
-                                             h->d_chirp.as<float4>() + npri, s));
         } else {
             TDC_HIP(h->d_rec.ensure(sizeof(tdc::DopRec) * nterm));
-            TDC_HIP(tdc::launch_stage_doppler(h->d_raw.as<float>(), h->d_pos.as<float>(),
-                                              h->d_vel.as<float>(), h->d_att.as<float>(),
-                                              h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
-                                              tol, h->bore, centroid ? 1 : 0, dop_center,
-                                              h->d_rec.as<tdc::DopRec>(), s));
         }
         h->dp.half_bw = dop_bw / 2.0F;
         h->dp.inv_bw = 1.0F / dop_bw;
@@ -446,15 +490,8 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     } else if ((h->chirp_path = position_constant_per_chirp(h))) {
         TDC_HIP(h->d_rec.ensure(sizeof(float4) * nterm));
         TDC_HIP(h->d_chirp.ensure(sizeof(float4) * static_cast<size_t>(npri)));
-        TDC_HIP(tdc::launch_stage_terms(h->d_raw.as<float>(), h->d_kt.as<float>(),
-                                        h->d_wr.as<float>(), npri, ns, h->d_rec.as<float4>(), s));
-        TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), nullptr, nullptr, npri, ns, 0.0F,
-                                         nullptr, 0, 0.0F, h->d_chirp.as<float4>(), nullptr, s));
     } else {
         TDC_HIP(h->d_rec.ensure(sizeof(tdc::PlainRec) * nterm));
-        TDC_HIP(tdc::launch_stage_plain(h->d_raw.as<float>(), h->d_pos.as<float>(),
-                                        h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
-                                        h->d_rec.as<tdc::PlainRec>(), s));
     }
     // measured best on 1024 chirps x 512 samples x 512^2 pixels: two pixels per
     // lane for the per-term plain records, one for the per-chirp range
@@ -543,8 +580,6 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     if ((rc = upload(h->d_grid, gridp, static_cast<size_t>(npx) * 3, s)) != TDC_OK) return rc;
     const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
     TDC_HIP(h->d_rec.ensure(esz * nterm));
-    TDC_HIP(tdc::launch_stage_precision(h->storage, h->d_raw.as<float>(), h->d_wr.as<float>(), npri,
-                                        ns, h->d_rec.p, s));
     h->doppler = false;
     int engine = h->engine;
     const int plen = tdc::profile_length(ns);
@@ -565,9 +600,6 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
             return rc;
         const int mp = plen + tdc::kProfilePad;
         TDC_HIP(h->d_ptab.ensure(sizeof(float2) * static_cast<size_t>(npri) * mp));
-        TDC_HIP(tdc::launch_stage_profile(h->storage, h->d_rec.p, h->d_pcorr.as<float>(),
-                                          h->d_ptw.as<float2>(), npri, ns, plen, mp,
-                                          h->d_ptab.as<float2>(), s));
         // chirp positions from an origin inside the scene (the midpoint of the
         // first and last pixels), with |pos - o|^2 in the pad
         const size_t last = 3 * (static_cast<size_t>(npx) - 1);
@@ -596,9 +628,6 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
             return fail(TDC_ERR_INVALID_ARGUMENT, "the matrix engine runs one pixel per lane");
         TDC_HIP(h->d_atab.ensure(tdc::matrix_table_bytes(npri, ns, h->storage)));
         TDC_HIP(h->d_ascale.ensure(sizeof(float) * static_cast<size_t>(npri)));
-        TDC_HIP(tdc::launch_stage_matrix(h->storage, h->d_rec.p, npri, ns,
-                                         tdc::matrix_parts(h->storage), h->d_atab.p,
-                                         h->d_ascale.as<float>(), s));
         h->ppl = 1;
     } else {
         h->ppl = h->force_ppl ? h->force_ppl : 2;
@@ -992,7 +1021,7 @@ int tdc_run(tdc_processor *h, int iterations)
     int wgs = 0;
     constexpr int kEventRing = 64;
     while (static_cast<int>(h->ev.size()) < std::min(iterations, kEventRing)) {
-        std::array<hipEvent_t, 3> t{nullptr, nullptr, nullptr};
+        std::array<hipEvent_t, 4> t{nullptr, nullptr, nullptr, nullptr};
         for (hipEvent_t &x : t) {
             const hipError_t e = hipEventCreate(&x);
             if (e != hipSuccess) {
@@ -1006,11 +1035,11 @@ int tdc_run(tdc_processor *h, int iterations)
     int pending = 0;
     auto drain = [&]() -> int {
         if (pending == 0) return TDC_OK;
-        TDC_HIP(hipEventSynchronize(h->ev[pending - 1][2]));
+        TDC_HIP(hipEventSynchronize(h->ev[pending - 1][3]));
         for (int k = 0; k < pending; ++k) {
             float a = 0.0F, b = 0.0F;
-            TDC_HIP(hipEventElapsedTime(&a, h->ev[k][0], h->ev[k][1]));
-            TDC_HIP(hipEventElapsedTime(&b, h->ev[k][0], h->ev[k][2]));
+            TDC_HIP(hipEventElapsedTime(&a, h->ev[k][1], h->ev[k][2]));
+            TDC_HIP(hipEventElapsedTime(&b, h->ev[k][0], h->ev[k][3]));
             main_ms += a;
             total_ms += b;
         }
@@ -1019,9 +1048,11 @@ int tdc_run(tdc_processor *h, int iterations)
     };
     for (int it = 0; it < iterations; ++it) {
         if (g.npx == 0) break;
-        const std::array<hipEvent_t, 3> &evt = h->ev[pending];
+        const std::array<hipEvent_t, 4> &evt = h->ev[pending];
         if (h->doppler) TDC_HIP(hipMemsetAsync(h->d_counters.p, 0, 2 * sizeof(unsigned long long), s));
         TDC_HIP(hipEventRecord(evt[0], s));
+        if ((rc = stage_inputs(h)) != TDC_OK) return rc;
+        TDC_HIP(hipEventRecord(evt[1], s));
         if (h->run_form == TDC_FORM_PLAIN) {
             float2 *dst = g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>();
             const float4 *cpos = h->d_chirp.as<float4>();
@@ -1041,7 +1072,7 @@ int tdc_run(tdc_processor *h, int iterations)
                 TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), h->d_grid.as<float>(),
                                           g, dst, s, &wgs));
             }
-            TDC_HIP(hipEventRecord(evt[1], s));
+            TDC_HIP(hipEventRecord(evt[2], s));
             if (g.nsplit > 1)
                 TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
                                                  h->d_img.as<float2>(), s));
@@ -1061,11 +1092,11 @@ int tdc_run(tdc_processor *h, int iterations)
                                               h->d_grid.as<double>(), h->d_kanchor.as<double>(),
                                               h->dk, g, h->d_part.as<double2>(), s, &wgs));
             }
-            TDC_HIP(hipEventRecord(evt[1], s));
+            TDC_HIP(hipEventRecord(evt[2], s));
             TDC_HIP(tdc::launch_finalize_f64(h->d_part.as<double2>(), g.nsplit, g.npx,
                                              h->d_img.as<float2>(), s));
         }
-        TDC_HIP(hipEventRecord(evt[2], s));
+        TDC_HIP(hipEventRecord(evt[3], s));
         if (++pending == static_cast<int>(h->ev.size()) && (rc = drain()) != TDC_OK) return rc;
     }
     if ((rc = drain()) != TDC_OK) return rc;
