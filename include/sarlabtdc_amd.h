@@ -82,7 +82,8 @@ typedef struct tdc_processor tdc_processor;
 typedef struct tdc_stats {
     double kernel_ms;       /* device time of one pass of the last run: staging, correlate-sum and
                                split reduction kernels (HIP events) */
-    double main_kernel_ms;  /* device time of the correlate-sum kernel alone */
+    double main_kernel_ms;  /* device time of the correlate-sum kernel alone (after tdc_start:
+                               of its slabs, with the waits for their parts of the grid) */
     double end_to_end_ms;   /* host wall time of the last tdc_start (H2D + compute + D2H) */
     double terms;           /* pixels * chirps * samples of the last run */
     int32_t form;           /* TDC_FORM_* actually run */
@@ -176,8 +177,13 @@ int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int
  * the image on the device `iterations` times, each one a full pass over the
  * resident inputs (the kernels that fold windows into the raw samples and
  * build per-chirp records, then the correlate-sum, then the split
- * reduction); tdc_fetch_image copies it into the image buffer. tdc_start is
- * exactly prepare + run(1) + fetch. */
+ * reduction); tdc_fetch_image copies it into the image buffer. tdc_start
+ * forms the same image as prepare + run(1) + fetch, bit for bit, but streams
+ * the focus grid behind the chirp inputs in two pixel slabs and launches
+ * each slab's correlate-sum as soon as its part of the grid has landed (once
+ * the image has at least 8192 (tile, split) workgroups; below that, one
+ * slab). Uploads go through page-locked double buffers on a copy stream of
+ * the processor's own. */
 int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
                 int dop_centroid_win, int apply_range_win, int doppler_win_taper);
 int tdc_run(tdc_processor *h, int iterations);

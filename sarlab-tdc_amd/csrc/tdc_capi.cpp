@@ -172,12 +172,6 @@ PosKey key_of(float x, float y, float z)
     return PosKey{bits(x), bits(y), bits(z)};
 }
 
-bool all_finite(const float *p, size_t n)
-{
-    for (size_t i = 0; i < n; ++i)
-        if (!std::isfinite(p[i])) return false;
-    return true;
-}
 
 } // namespace
 
@@ -224,6 +218,9 @@ struct tdc_processor {
     // device buffers
     DevBuf d_raw, d_pos, d_vel, d_att, d_kt, d_wr, d_rec, d_grid, d_part, d_img, d_cpos,
         d_kanchor, d_counters, d_atab, d_ascale, d_chirp, d_ptab, d_pcorr, d_ptw, d_pcpos;
+    // host-to-device copies of the inputs (declared after the buffers it
+    // fills, so it is destroyed, and its copies drained, before they are freed)
+    tdc::Ingest ingest;
 
     // prepared state
     bool prepared = false;
@@ -232,6 +229,13 @@ struct tdc_processor {
     int run_engine = 0;
     bool doppler = false;
     bool chirp_path = false; // plain form: range (and window) once per (pixel, chirp)
+    int ant_stride = 0;      // samples between the antenna states of consecutive chirps on the device
+    // tdc_start uploads the focus grid behind the chirp inputs, in slabs that
+    // overlap the correlate-sum of the slabs before them
+    bool defer_grid = false;
+    const void *grid_src = nullptr; // host grid in the device layout (FP32 or FP64 x, y, z)
+    size_t grid_elem = 0;
+    std::vector<double> grid64_host; // FP64 copy of an FP32 grid for the precision form
     tdc::Geom geom{};
     tdc::DopParams dp{};
     tdc::ProfileParams pp{};
@@ -278,6 +282,7 @@ void make_geom(tdc::Geom &g, int npx, int npri, int ns, int ppl, int nsplit)
     g.ntiles = (npx + tdc::kBlock * ppl - 1) / (tdc::kBlock * ppl);
     g.nsplit = nsplit;
     g.cps = (npri + nsplit - 1) / nsplit;
+    g.pstride = npx;
     const bool pow2 = (nsplit & (nsplit - 1)) == 0;
     g.xcd_map = pow2 ? 1 : 0;
     if (g.xcd_map && nsplit < 8) {
@@ -288,33 +293,6 @@ void make_geom(tdc::Geom &g, int npx, int npri, int ns, int ppl, int nsplit)
     }
 }
 
-// True when `v` (`width` floats per term) is the same for all samples of
-// every chirp, bit for bit.
-bool constant_per_chirp(const float *v, int width, int npri, int ns)
-{
-    const size_t w = static_cast<size_t>(width);
-    for (int n = 0; n < npri; ++n) {
-        const float *first = v + w * static_cast<size_t>(n) * ns;
-        for (int m = 1; m < ns; ++m) {
-            if (std::memcmp(first + w * m, first, w * sizeof(float)) != 0) return false;
-        }
-    }
-    return true;
-}
-
-// Stop-and-hop input: with the position constant within every chirp the range
-// of a term depends only on its pixel and chirp; with velocity and attitude
-// constant too, so does its Doppler window.
-bool position_constant_per_chirp(const tdc_processor *h)
-{
-    return constant_per_chirp(h->pos, 3, h->npri, h->ns);
-}
-
-bool antenna_constant_per_chirp(const tdc_processor *h)
-{
-    return position_constant_per_chirp(h) && constant_per_chirp(h->vel, 3, h->npri, h->ns) &&
-           constant_per_chirp(h->att, 4, h->npri, h->ns);
-}
 
 // True when every term's Doppler window value is exactly 1 and no (pixel,
 // chunk) can be skipped, so the per-term window and the skip test can be
@@ -322,24 +300,14 @@ bool antenna_constant_per_chirp(const tdc_processor *h)
 // |f_c| <= (2/lambda)|v| |q|^2 |b| (a rotation by a non-unit quaternion
 // scales by |q|^2), and excludes pixels that coincide with an antenna
 // position (0/0 there makes the reference's window NaN, hence 0).
-bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool centroid,
-                           bool taper)
+bool doppler_window_is_one(const tdc_processor *h, const tdc::AntennaScan &scan, float center,
+                           float bw, bool centroid, bool taper)
 {
     if (!(bw > 0.0F) || !std::isfinite(center)) return false;
     const size_t nterm = static_cast<size_t>(h->npri) * h->ns;
     const size_t npx = static_cast<size_t>(h->rows) * h->cols;
-    if (!h->vel || !h->att || !h->pos) return false;
-    if (!all_finite(h->pos, nterm * 3) || !all_finite(h->vel, nterm * 3) ||
-        !all_finite(h->att, nterm * 4) || !all_finite(h->grid, npx * 3))
-        return false;
-    double vmax2 = 0.0, qmax2 = 0.0;
-    for (size_t e = 0; e < nterm; ++e) {
-        const float *v = h->vel + 3 * e;
-        vmax2 = std::max(vmax2, double(v[0]) * v[0] + double(v[1]) * v[1] + double(v[2]) * v[2]);
-        const float *q = h->att + 4 * e;
-        qmax2 = std::max(qmax2, double(q[0]) * q[0] + double(q[1]) * q[1] + double(q[2]) * q[2] +
-                                    double(q[3]) * q[3]);
-    }
+    if (!scan.has_vel || !scan.has_att || !h->pos || !scan.finite) return false;
+    const double vmax2 = scan.vmax2, qmax2 = scan.qmax2;
     const double tol = 2.0 / h->lambda_c;
     const double vmax = std::sqrt(vmax2);
     const double bmag = std::sqrt(double(h->bore[0]) * h->bore[0] + double(h->bore[1]) * h->bore[1] +
@@ -353,22 +321,29 @@ bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool 
     // a pixel at the origin meets the zero-filled probe rows; any other pixel
     // can only coincide with an antenna position inside the positions' bounding
     // box, so the set of positions is built only when such a pixel exists
-    float lo[3] = {h->pos[0], h->pos[1], h->pos[2]};
-    float hi[3] = {lo[0], lo[1], lo[2]};
-    for (size_t e = 1; e < nterm; ++e) {
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], h->pos[3 * e + a]);
-            hi[a] = std::max(hi[a], h->pos[3 * e + a]);
-        }
-    }
+    const float *lo = scan.lo, *hi = scan.hi;
     std::vector<size_t> inside;
-    for (size_t i = 0; i < npx; ++i) {
-        const float *g = h->grid + 3 * i;
-        if (g[0] == 0.0F && g[1] == 0.0F && g[2] == 0.0F) return false;
-        if (g[0] >= lo[0] && g[0] <= hi[0] && g[1] >= lo[1] && g[1] <= hi[1] && g[2] >= lo[2] &&
-            g[2] <= hi[2])
-            inside.push_back(i);
-    }
+    bool grid_ok = true;
+    std::mutex mu;
+    tdc::parallel_for(npx, 4096, [&](size_t i0, size_t i1) {
+        std::vector<size_t> in;
+        bool good = true;
+        for (size_t i = i0; i < i1; ++i) {
+            const float *g = h->grid + 3 * i;
+            if (!std::isfinite(g[0]) || !std::isfinite(g[1]) || !std::isfinite(g[2]) ||
+                (g[0] == 0.0F && g[1] == 0.0F && g[2] == 0.0F)) {
+                good = false;
+                break;
+            }
+            if (g[0] >= lo[0] && g[0] <= hi[0] && g[1] >= lo[1] && g[1] <= hi[1] &&
+                g[2] >= lo[2] && g[2] <= hi[2])
+                in.push_back(i);
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        grid_ok = grid_ok && good;
+        inside.insert(inside.end(), in.begin(), in.end());
+    });
+    if (!grid_ok) return false;
     if (inside.empty()) return true;
     std::unordered_set<PosKey, PosKeyHash> seen;
     const float *last = nullptr;
@@ -385,10 +360,26 @@ bool doppler_window_is_one(const tdc_processor *h, float center, float bw, bool 
     return true;
 }
 
-template <class T> int upload(DevBuf &b, const T *src, size_t count, hipStream_t s)
+// Queues a copy of `count` host values into `b` on the processor's ingest;
+// `src` may be released on return. The compute stream waits for it after
+// the next ingest fence.
+template <class T> int upload(tdc_processor *h, DevBuf &b, const T *src, size_t count)
 {
     TDC_HIP(b.ensure(sizeof(T) * count));
-    if (count) TDC_HIP(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    if (count) TDC_HIP(h->ingest.put(b.p, src, sizeof(T) * count));
+    return TDC_OK;
+}
+
+// The focus grid (x, y, z per pixel, in the form's precision) on the device:
+// queued on the ingest now, or, for tdc_start, sized and left to the
+// streamed pass.
+template <class T> int place_grid(tdc_processor *h, const T *grid, int npx)
+{
+    const size_t count = 3 * static_cast<size_t>(npx);
+    if (!h->defer_grid) return upload(h, h->d_grid, grid, count);
+    TDC_HIP(h->d_grid.ensure(sizeof(T) * count));
+    h->grid_src = grid;
+    h->grid_elem = sizeof(T);
     return TDC_OK;
 }
 
@@ -408,11 +399,12 @@ int stage_inputs(tdc_processor *h)
             float4 *cpos = h->d_chirp.as<float4>();
             if (h->doppler) {
                 TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), h->d_vel.as<float>(),
-                                                 h->d_att.as<float>(), npri, ns, h->st_tol,
+                                                 h->d_att.as<float>(), npri, h->ant_stride, h->st_tol,
                                                  h->bore, h->st_centroid, h->st_fc, cpos,
                                                  cpos + npri, s));
             } else {
-                TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), nullptr, nullptr, npri, ns,
+                TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), nullptr, nullptr, npri,
+                                                 h->ant_stride,
                                                  0.0F, nullptr, 0, 0.0F, cpos, nullptr, s));
             }
         } else if (h->doppler) {
@@ -456,21 +448,50 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         const double f = double(h->start_freq) + double(h->mod_rate) * double(h->sample_times[m]);
         kt[m] = static_cast<float>(2.0 * f / kSpeedOfLight);
     }
-    h->doppler = !doppler_window_is_one(h, dop_center, dop_bw, centroid, taper);
-    h->chirp_path = false;
+    const tdc::AntennaScan scan = tdc::scan_antenna(h->pos, h->vel, h->att, npri, ns);
+    h->doppler = !doppler_window_is_one(h, scan, dop_center, dop_bw, centroid, taper);
+    if (h->doppler && (!h->vel || !h->att))
+        return fail(TDC_ERR_INVALID_ARGUMENT, "the Doppler window needs velocity and attitude");
+    // stop-and-hop input: with the position constant within every chirp the
+    // range of a term depends only on its pixel and chirp; with velocity and
+    // attitude constant too, so does its Doppler window
+    h->chirp_path = scan.pos_const && (!h->doppler || (scan.vel_const && scan.att_const));
     hipStream_t s = h->stream;
     int rc;
-    if ((rc = upload(h->d_raw, h->raw, nterm * 2, s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_pos, h->pos, nterm * 3, s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_kt, kt.data(), ns, s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_wr, wr.data(), ns, s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_grid, h->grid, static_cast<size_t>(npx) * 3, s)) != TDC_OK) return rc;
+    TDC_HIP(h->ingest.begin(s));
+    if ((rc = upload(h, h->d_raw, h->raw, nterm * 2)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_kt, kt.data(), ns)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_wr, wr.data(), ns)) != TDC_OK) return rc;
+    if ((rc = place_grid(h, h->grid, npx)) != TDC_OK) return rc;
+    if (h->chirp_path) {
+        // one antenna state per chirp is all the per-chirp kernels read
+        std::vector<float> cp(3 * static_cast<size_t>(npri)), cv, ca;
+        if (h->doppler) {
+            cv.resize(3 * static_cast<size_t>(npri));
+            ca.resize(4 * static_cast<size_t>(npri));
+        }
+        for (size_t n = 0; n < static_cast<size_t>(npri); ++n) {
+            const size_t e = n * ns;
+            std::copy(h->pos + 3 * e, h->pos + 3 * e + 3, cp.begin() + 3 * n);
+            if (h->doppler) {
+                std::copy(h->vel + 3 * e, h->vel + 3 * e + 3, cv.begin() + 3 * n);
+                std::copy(h->att + 4 * e, h->att + 4 * e + 4, ca.begin() + 4 * n);
+            }
+        }
+        if ((rc = upload(h, h->d_pos, cp.data(), cp.size())) != TDC_OK) return rc;
+        if ((rc = upload(h, h->d_vel, cv.data(), cv.size())) != TDC_OK) return rc;
+        if ((rc = upload(h, h->d_att, ca.data(), ca.size())) != TDC_OK) return rc;
+        h->ant_stride = 1;
+    } else {
+        if ((rc = upload(h, h->d_pos, h->pos, nterm * 3)) != TDC_OK) return rc;
+        if (h->doppler) {
+            if ((rc = upload(h, h->d_vel, h->vel, nterm * 3)) != TDC_OK) return rc;
+            if ((rc = upload(h, h->d_att, h->att, nterm * 4)) != TDC_OK) return rc;
+        }
+        h->ant_stride = ns;
+    }
+    TDC_HIP(h->ingest.fence(s));
     if (h->doppler) {
-        if (!h->vel || !h->att)
-            return fail(TDC_ERR_INVALID_ARGUMENT, "the Doppler window needs velocity and attitude");
-        h->chirp_path = antenna_constant_per_chirp(h);
-        if ((rc = upload(h->d_vel, h->vel, nterm * 3, s)) != TDC_OK) return rc;
-        if ((rc = upload(h->d_att, h->att, nterm * 4, s)) != TDC_OK) return rc;
         h->st_tol = 2.0F / h->lambda_c; // windowing.h:21, :33
         h->st_centroid = centroid ? 1 : 0;
         h->st_fc = dop_center;
@@ -487,7 +508,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         h->dp.centroid = centroid ? 1 : 0;
         h->dp.fc_fixed = dop_center;
         TDC_HIP(h->d_counters.ensure(2 * sizeof(unsigned long long)));
-    } else if ((h->chirp_path = position_constant_per_chirp(h))) {
+    } else if (h->chirp_path) {
         TDC_HIP(h->d_rec.ensure(sizeof(float4) * nterm));
         TDC_HIP(h->d_chirp.ensure(sizeof(float4) * static_cast<size_t>(npri)));
     } else {
@@ -515,7 +536,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
 {
     const int npri = h->npri, ns = h->ns, npx = h->rows * h->cols;
     const size_t nterm = static_cast<size_t>(npri) * ns;
-    std::vector<double> t(ns), cpos(static_cast<size_t>(npri) * 3), grid64;
+    std::vector<double> t(ns), cpos(static_cast<size_t>(npri) * 3);
     double f0, krate;
     const double *gridp = nullptr;
     if (h->have_raw64) {
@@ -533,13 +554,16 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     } else {
         if (!h->have_raw32) return fail(TDC_ERR_STATE, "setRawData must be called before start()");
         if (!h->grid) return fail(TDC_ERR_STATE, "setFocusGrid must be called before start()");
-        if (!doppler_window_is_one(h, dop_center, dop_bw, centroid, taper))
+        const tdc::AntennaScan scan = tdc::scan_antenna(h->pos, h->vel, h->att, npri, ns);
+        if (!doppler_window_is_one(h, scan, dop_center, dop_bw, centroid, taper))
             return fail(TDC_ERR_UNSUPPORTED,
                         "the precision form needs a Doppler window that is identically 1 over "
                         "the data (use the plain form for Doppler windowing)");
         for (int n = 0; n < npri; ++n) {
             const float *p0 = h->pos + static_cast<size_t>(n) * ns * 3;
-            for (int m = 1; m < ns; ++m) {
+            // the scan compares bit patterns; positions that differ only in
+            // the sign of zero are the same position here
+            for (int m = 1; !scan.pos_const && m < ns; ++m) {
                 const float *pm = p0 + 3 * static_cast<size_t>(m);
                 if (pm[0] != p0[0] || pm[1] != p0[1] || pm[2] != p0[2])
                     return fail(TDC_ERR_UNSUPPORTED,
@@ -558,8 +582,11 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     if (h->grid64) {
         gridp = h->grid64;
     } else {
+        std::vector<double> &grid64 = h->grid64_host;
         grid64.resize(static_cast<size_t>(npx) * 3);
-        for (size_t i = 0; i < grid64.size(); ++i) grid64[i] = h->grid[i];
+        tdc::parallel_for(grid64.size(), size_t(1) << 16, [&](size_t i0, size_t i1) {
+            for (size_t i = i0; i < i1; ++i) grid64[i] = h->grid[i];
+        });
         gridp = grid64.data();
     }
     std::vector<double> kanchor;
@@ -573,11 +600,12 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
 
     hipStream_t s = h->stream;
     int rc;
-    if ((rc = upload(h->d_raw, h->raw, nterm * 2, s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_wr, wr.data(), ns, s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_cpos, cp.data(), cp.size(), s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_kanchor, kanchor.data(), kanchor.size(), s)) != TDC_OK) return rc;
-    if ((rc = upload(h->d_grid, gridp, static_cast<size_t>(npx) * 3, s)) != TDC_OK) return rc;
+    TDC_HIP(h->ingest.begin(s));
+    if ((rc = upload(h, h->d_raw, h->raw, nterm * 2)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_wr, wr.data(), ns)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_cpos, cp.data(), cp.size())) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_kanchor, kanchor.data(), kanchor.size())) != TDC_OK) return rc;
+    if ((rc = place_grid(h, gridp, npx)) != TDC_OK) return rc;
     const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
     TDC_HIP(h->d_rec.ensure(esz * nterm));
     h->doppler = false;
@@ -595,8 +623,8 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
                         "the range-profile engine takes at most " +
                             std::to_string(tdc::kProfileMaxM / 2) + " samples per chirp");
         const auto tabs = tdc::profile_tables(ns, plen);
-        if ((rc = upload(h->d_pcorr, tabs->corr.data(), tabs->corr.size(), s)) != TDC_OK) return rc;
-        if ((rc = upload(h->d_ptw, tabs->twiddle.data(), tabs->twiddle.size(), s)) != TDC_OK)
+        if ((rc = upload(h, h->d_pcorr, tabs->corr.data(), tabs->corr.size())) != TDC_OK) return rc;
+        if ((rc = upload(h, h->d_ptw, tabs->twiddle.data(), tabs->twiddle.size())) != TDC_OK)
             return rc;
         const int mp = plen + tdc::kProfilePad;
         TDC_HIP(h->d_ptab.ensure(sizeof(float2) * static_cast<size_t>(npri) * mp));
@@ -613,7 +641,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
             const double x = cp[n].x - o[0], y = cp[n].y - o[1], z = cp[n].z - o[2];
             pc[n] = tdc::ChirpPos{x, y, z, std::fma(z, z, std::fma(y, y, x * x))};
         }
-        if ((rc = upload(h->d_pcpos, pc.data(), pc.size(), s)) != TDC_OK) return rc;
+        if ((rc = upload(h, h->d_pcpos, pc.data(), pc.size())) != TDC_OK) return rc;
         h->pp.ox = o[0];
         h->pp.oy = o[1];
         h->pp.oz = o[2];
@@ -632,6 +660,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     } else {
         h->ppl = h->force_ppl ? h->force_ppl : 2;
     }
+    TDC_HIP(h->ingest.fence(s));
     h->run_engine = engine;
     const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
     const int nsplit = choose_split(ntiles, npri, h->force_split);
@@ -983,44 +1012,84 @@ int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits)
     return ok();
 }
 
+namespace {
+int prepare_inputs(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
+                   int dop_centroid_win, int apply_range_win, int doppler_win_taper,
+                   bool defer_grid);
+} // namespace
+
 int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
                 int dop_centroid_win, int apply_range_win, int doppler_win_taper)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     tdc::Range range("tdc_prepare");
-    if (!h->have_raw32 && !h->have_raw64)
-        return fail(TDC_ERR_STATE, "setRawData must be called before start()");
-    int rc = ensure_device(h);
-    if (rc != TDC_OK) return rc;
-    h->prepared = false;
-    if (h->form == TDC_FORM_PRECISION || h->have_raw64) {
-        rc = prepare_precision(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
-                               apply_range_win != 0, doppler_win_taper != 0);
-    } else {
-        rc = prepare_plain(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
-                           apply_range_win != 0, doppler_win_taper != 0);
-    }
-    if (rc != TDC_OK) return rc;
-    const int nchunks = (h->npri + tdc::kChunk - 1) / tdc::kChunk;
-    h->trivial_processed = static_cast<int64_t>(h->rows) * h->cols * nchunks;
-    h->prepared = true;
-    return ok();
+    const int rc = prepare_inputs(h, doppler_win_center, doppler_bandwidth, dop_centroid_win,
+                                  apply_range_win, doppler_win_taper, false);
+    return rc == TDC_OK ? ok() : rc;
 }
 
-int tdc_run(tdc_processor *h, int iterations)
+namespace {
+
+// The correlate-sum kernel of the prepared form over the pixels
+// [p0, p0 + g.npx): grid and partial images offset by p0, the partial
+// images g.pstride apart (the image itself when the plain form has one split).
+int launch_main(tdc_processor *h, const tdc::Geom &g, size_t p0, int *wgs)
 {
-    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
-    tdc::Range range("tdc_run");
-    if (!h->prepared) return fail(TDC_ERR_STATE, "tdc_prepare must succeed before tdc_run");
-    if (iterations < 1) return fail(TDC_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
-    int rc = ensure_device(h);
-    if (rc != TDC_OK) return rc;
-    const tdc::Geom &g = h->geom;
     hipStream_t s = h->stream;
-    double main_ms = 0.0, total_ms = 0.0;
-    int wgs = 0;
-    constexpr int kEventRing = 64;
-    while (static_cast<int>(h->ev.size()) < std::min(iterations, kEventRing)) {
+    if (h->run_form == TDC_FORM_PLAIN) {
+        float2 *dst = (g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>()) + p0;
+        const float *grid = h->d_grid.as<float>() + 3 * p0;
+        const float4 *cpos = h->d_chirp.as<float4>();
+        if (h->doppler && h->chirp_path) {
+            TDC_HIP(tdc::launch_doppler_chirp(h->ppl, h->d_rec.as<float4>(), cpos, cpos + g.npri,
+                                              grid, g, h->dp, dst,
+                                              h->d_counters.as<unsigned long long>(), s, wgs));
+        } else if (h->doppler) {
+            TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(), grid, g, h->dp, dst,
+                                        h->d_counters.as<unsigned long long>(), s, wgs));
+        } else if (h->chirp_path) {
+            TDC_HIP(tdc::launch_range(h->ppl, h->d_rec.as<float4>(), cpos, grid, g, dst, s, wgs));
+        } else {
+            TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), grid, g, dst, s, wgs));
+        }
+        return TDC_OK;
+    }
+    const double *grid = h->d_grid.as<double>() + 3 * p0;
+    double2 *dst = h->d_part.as<double2>() + p0;
+    if (h->run_engine == TDC_ENGINE_PROFILE) {
+        TDC_HIP(tdc::launch_profile(h->d_ptab.as<float2>(), h->d_pcpos.as<tdc::ChirpPos>(), grid,
+                                    h->pp, g, dst, s, wgs));
+    } else if (h->run_engine == TDC_ENGINE_MATRIX) {
+        TDC_HIP(tdc::launch_precision_matrix(
+            tdc::matrix_parts(h->run_storage), h->d_atab.p, h->d_ascale.as<float>(),
+            h->d_cpos.as<tdc::ChirpPos>(), grid, h->d_kanchor.as<double>(), h->dk, g, dst, s,
+            wgs));
+    } else {
+        TDC_HIP(tdc::launch_precision(h->ppl, h->run_storage, h->d_rec.p,
+                                      h->d_cpos.as<tdc::ChirpPos>(), grid,
+                                      h->d_kanchor.as<double>(), h->dk, g, dst, s, wgs));
+    }
+    return TDC_OK;
+}
+
+// The fixed-order sum of the splits' partial images into the image.
+int launch_reduce(tdc_processor *h)
+{
+    const tdc::Geom &g = h->geom;
+    if (h->run_form == TDC_FORM_PLAIN) {
+        if (g.nsplit > 1)
+            TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
+                                             h->d_img.as<float2>(), h->stream));
+    } else {
+        TDC_HIP(tdc::launch_finalize_f64(h->d_part.as<double2>(), g.nsplit, g.npx,
+                                         h->d_img.as<float2>(), h->stream));
+    }
+    return TDC_OK;
+}
+
+int ensure_events(tdc_processor *h, int count)
+{
+    while (static_cast<int>(h->ev.size()) < count) {
         std::array<hipEvent_t, 4> t{nullptr, nullptr, nullptr, nullptr};
         for (hipEvent_t &x : t) {
             const hipError_t e = hipEventCreate(&x);
@@ -1032,78 +1101,15 @@ int tdc_run(tdc_processor *h, int iterations)
         }
         h->ev.push_back(t);
     }
-    int pending = 0;
-    auto drain = [&]() -> int {
-        if (pending == 0) return TDC_OK;
-        TDC_HIP(hipEventSynchronize(h->ev[pending - 1][3]));
-        for (int k = 0; k < pending; ++k) {
-            float a = 0.0F, b = 0.0F;
-            TDC_HIP(hipEventElapsedTime(&a, h->ev[k][1], h->ev[k][2]));
-            TDC_HIP(hipEventElapsedTime(&b, h->ev[k][0], h->ev[k][3]));
-            main_ms += a;
-            total_ms += b;
-        }
-        pending = 0;
-        return TDC_OK;
-    };
-    for (int it = 0; it < iterations; ++it) {
-        if (g.npx == 0) break;
-        const std::array<hipEvent_t, 4> &evt = h->ev[pending];
-        if (h->doppler) TDC_HIP(hipMemsetAsync(h->d_counters.p, 0, 2 * sizeof(unsigned long long), s));
-        TDC_HIP(hipEventRecord(evt[0], s));
-        if ((rc = stage_inputs(h)) != TDC_OK) return rc;
-        TDC_HIP(hipEventRecord(evt[1], s));
-        if (h->run_form == TDC_FORM_PLAIN) {
-            float2 *dst = g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>();
-            const float4 *cpos = h->d_chirp.as<float4>();
-            if (h->doppler && h->chirp_path) {
-                TDC_HIP(tdc::launch_doppler_chirp(h->ppl, h->d_rec.as<float4>(), cpos,
-                                                  cpos + g.npri, h->d_grid.as<float>(), g, h->dp,
-                                                  dst, h->d_counters.as<unsigned long long>(), s,
-                                                  &wgs));
-            } else if (h->doppler) {
-                TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(),
-                                            h->d_grid.as<float>(), g, h->dp, dst,
-                                            h->d_counters.as<unsigned long long>(), s, &wgs));
-            } else if (h->chirp_path) {
-                TDC_HIP(tdc::launch_range(h->ppl, h->d_rec.as<float4>(), cpos, h->d_grid.as<float>(),
-                                          g, dst, s, &wgs));
-            } else {
-                TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), h->d_grid.as<float>(),
-                                          g, dst, s, &wgs));
-            }
-            TDC_HIP(hipEventRecord(evt[2], s));
-            if (g.nsplit > 1)
-                TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
-                                                 h->d_img.as<float2>(), s));
-        } else {
-            if (h->run_engine == TDC_ENGINE_PROFILE) {
-                TDC_HIP(tdc::launch_profile(h->d_ptab.as<float2>(), h->d_pcpos.as<tdc::ChirpPos>(),
-                                            h->d_grid.as<double>(), h->pp, g,
-                                            h->d_part.as<double2>(), s, &wgs));
-            } else if (h->run_engine == TDC_ENGINE_MATRIX) {
-                TDC_HIP(tdc::launch_precision_matrix(
-                    tdc::matrix_parts(h->run_storage), h->d_atab.p, h->d_ascale.as<float>(),
-                    h->d_cpos.as<tdc::ChirpPos>(), h->d_grid.as<double>(),
-                    h->d_kanchor.as<double>(), h->dk, g, h->d_part.as<double2>(), s, &wgs));
-            } else {
-                TDC_HIP(tdc::launch_precision(h->ppl, h->run_storage, h->d_rec.p,
-                                              h->d_cpos.as<tdc::ChirpPos>(),
-                                              h->d_grid.as<double>(), h->d_kanchor.as<double>(),
-                                              h->dk, g, h->d_part.as<double2>(), s, &wgs));
-            }
-            TDC_HIP(hipEventRecord(evt[2], s));
-            TDC_HIP(tdc::launch_finalize_f64(h->d_part.as<double2>(), g.nsplit, g.npx,
-                                             h->d_img.as<float2>(), s));
-        }
-        TDC_HIP(hipEventRecord(evt[3], s));
-        if (++pending == static_cast<int>(h->ev.size()) && (rc = drain()) != TDC_OK) return rc;
-    }
-    if ((rc = drain()) != TDC_OK) return rc;
-    TDC_HIP(hipStreamSynchronize(s));
+    return TDC_OK;
+}
+
+int finish_stats(tdc_processor *h, double main_ms, double total_ms, int wgs)
+{
+    const tdc::Geom &g = h->geom;
     tdc_stats &st = h->stats;
-    st.main_kernel_ms = main_ms / iterations;
-    st.kernel_ms = total_ms / iterations;
+    st.main_kernel_ms = main_ms;
+    st.kernel_ms = total_ms;
     st.terms = double(g.npx) * g.npri * g.ns;
     st.form = h->run_form;
     st.raw_storage = h->run_storage;
@@ -1124,6 +1130,132 @@ int tdc_run(tdc_processor *h, int iterations)
         st.processed_chunks = h->trivial_processed;
         st.skipped_chunks = 0;
     }
+    return TDC_OK;
+}
+
+// One pass of tdc_start with the focus grid still on the host: the staging
+// kernels start as soon as the chirp inputs have landed, and the grid follows
+// in two pixel slabs (the first a quarter of the tiles), each slab's
+// correlate-sum launched as soon as its part of the grid has landed, so the
+// larger part of the grid crosses PCIe while the first slab is formed. Every
+// pixel is computed by the same tile code over the same splits as in
+// tdc_run, so the image is the same bit for bit.
+int run_streamed(tdc_processor *h)
+{
+    tdc::Range range("tdc_run");
+    const tdc::Geom &g = h->geom;
+    hipStream_t s = h->stream;
+    int rc = ensure_events(h, 1);
+    if (rc != TDC_OK) return rc;
+    const std::array<hipEvent_t, 4> &evt = h->ev[0];
+    const size_t tile = static_cast<size_t>(tdc::kBlock) * h->ppl;
+    const size_t npx = static_cast<size_t>(g.npx);
+    std::vector<size_t> cut{0};
+    if (static_cast<int64_t>(g.ntiles) * g.nsplit >= 8192 && g.ntiles >= 8)
+        cut.push_back(static_cast<size_t>(g.ntiles / 4) * tile);
+    cut.push_back(npx);
+    if (h->doppler) TDC_HIP(hipMemsetAsync(h->d_counters.p, 0, 2 * sizeof(unsigned long long), s));
+    TDC_HIP(hipEventRecord(evt[0], s));
+    if ((rc = stage_inputs(h)) != TDC_OK) return rc;
+    TDC_HIP(hipEventRecord(evt[1], s));
+    int wgs = 0;
+    const char *src = static_cast<const char *>(h->grid_src);
+    char *dst = static_cast<char *>(h->d_grid.p);
+    const size_t px_bytes = 3 * h->grid_elem;
+    for (size_t k = 0; k + 1 < cut.size(); ++k) {
+        const size_t p0 = cut[k], p1 = cut[k + 1];
+        if (p1 <= p0) continue;
+        TDC_HIP(h->ingest.put(dst + p0 * px_bytes, src + p0 * px_bytes, (p1 - p0) * px_bytes));
+        TDC_HIP(h->ingest.fence(s));
+        tdc::Geom gs{};
+        make_geom(gs, static_cast<int>(p1 - p0), g.npri, g.ns, h->ppl, g.nsplit);
+        gs.pstride = g.npx;
+        int w = 0;
+        if ((rc = launch_main(h, gs, p0, &w)) != TDC_OK) return rc;
+        wgs += w;
+    }
+    TDC_HIP(hipEventRecord(evt[2], s));
+    if ((rc = launch_reduce(h)) != TDC_OK) return rc;
+    TDC_HIP(hipEventRecord(evt[3], s));
+    TDC_HIP(hipEventSynchronize(evt[3]));
+    float a = 0.0F, b = 0.0F;
+    TDC_HIP(hipEventElapsedTime(&a, evt[1], evt[2]));
+    TDC_HIP(hipEventElapsedTime(&b, evt[0], evt[3]));
+    return finish_stats(h, a, b, wgs);
+}
+
+int prepare_inputs(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
+                   int dop_centroid_win, int apply_range_win, int doppler_win_taper,
+                   bool defer_grid)
+{
+    if (!h->have_raw32 && !h->have_raw64)
+        return fail(TDC_ERR_STATE, "setRawData must be called before start()");
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    h->prepared = false;
+    h->defer_grid = defer_grid;
+    if (h->form == TDC_FORM_PRECISION || h->have_raw64) {
+        rc = prepare_precision(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
+                               apply_range_win != 0, doppler_win_taper != 0);
+    } else {
+        rc = prepare_plain(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
+                           apply_range_win != 0, doppler_win_taper != 0);
+    }
+    if (rc != TDC_OK) return rc;
+    const int nchunks = (h->npri + tdc::kChunk - 1) / tdc::kChunk;
+    h->trivial_processed = static_cast<int64_t>(h->rows) * h->cols * nchunks;
+    h->prepared = !defer_grid;
+    return TDC_OK;
+}
+
+} // namespace
+
+int tdc_run(tdc_processor *h, int iterations)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    tdc::Range range("tdc_run");
+    if (!h->prepared) return fail(TDC_ERR_STATE, "tdc_prepare must succeed before tdc_run");
+    if (iterations < 1) return fail(TDC_ERR_INVALID_ARGUMENT, "iterations must be >= 1");
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    const tdc::Geom &g = h->geom;
+    hipStream_t s = h->stream;
+    double main_ms = 0.0, total_ms = 0.0;
+    int wgs = 0;
+    constexpr int kEventRing = 64;
+    if ((rc = ensure_events(h, std::min(iterations, kEventRing))) != TDC_OK) return rc;
+    int pending = 0;
+    auto drain = [&]() -> int {
+        if (pending == 0) return TDC_OK;
+        TDC_HIP(hipEventSynchronize(h->ev[pending - 1][3]));
+        for (int k = 0; k < pending; ++k) {
+            float a = 0.0F, b = 0.0F;
+            TDC_HIP(hipEventElapsedTime(&a, h->ev[k][1], h->ev[k][2]));
+            TDC_HIP(hipEventElapsedTime(&b, h->ev[k][0], h->ev[k][3]));
+            main_ms += a;
+            total_ms += b;
+        }
+        pending = 0;
+        return TDC_OK;
+    };
+    const int ring = std::min(iterations, kEventRing);
+    for (int it = 0; it < iterations; ++it) {
+        if (g.npx == 0) break;
+        const std::array<hipEvent_t, 4> &evt = h->ev[pending];
+        if (h->doppler) TDC_HIP(hipMemsetAsync(h->d_counters.p, 0, 2 * sizeof(unsigned long long), s));
+        TDC_HIP(hipEventRecord(evt[0], s));
+        if ((rc = stage_inputs(h)) != TDC_OK) return rc;
+        TDC_HIP(hipEventRecord(evt[1], s));
+        if ((rc = launch_main(h, g, 0, &wgs)) != TDC_OK) return rc;
+        TDC_HIP(hipEventRecord(evt[2], s));
+        if ((rc = launch_reduce(h)) != TDC_OK) return rc;
+        TDC_HIP(hipEventRecord(evt[3], s));
+        if (++pending == ring && (rc = drain()) != TDC_OK) return rc;
+    }
+    if ((rc = drain()) != TDC_OK) return rc;
+    TDC_HIP(hipStreamSynchronize(s));
+    if ((rc = finish_stats(h, main_ms / iterations, total_ms / iterations, wgs)) != TDC_OK)
+        return rc;
     return ok();
 }
 
@@ -1153,9 +1285,21 @@ int tdc_start(tdc_processor *h, float doppler_win_center, float doppler_bandwidt
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     tdc::Range range("tdc_start");
     const auto t0 = std::chrono::steady_clock::now();
-    int rc = tdc_prepare(h, doppler_win_center, doppler_bandwidth, dop_centroid_win,
-                         apply_range_win, doppler_win_taper);
-    if (rc == TDC_OK) rc = tdc_run(h, 1);
+    int rc;
+    {
+        tdc::Range prep("tdc_prepare");
+        rc = prepare_inputs(h, doppler_win_center, doppler_bandwidth, dop_centroid_win,
+                            apply_range_win, doppler_win_taper, true);
+    }
+    if (rc == TDC_OK) {
+        if (h->geom.npx > 0) {
+            rc = run_streamed(h);
+            if (rc == TDC_OK) h->prepared = true;
+        } else {
+            h->prepared = true;
+            rc = tdc_run(h, 1);
+        }
+    }
     if (rc == TDC_OK) rc = tdc_fetch_image(h);
     if (rc != TDC_OK) return rc;
     const auto t1 = std::chrono::steady_clock::now();

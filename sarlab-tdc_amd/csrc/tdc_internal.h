@@ -3,6 +3,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -73,6 +74,7 @@ struct Geom {
     int cps;           // chirps per split
     int xcd_map;       // 1: XCD-aware block mapping (nsplit a power of two)
     int tiles_padded;  // tiles including the padding of the XCD mapping
+    int pstride;       // distance between the partial images of consecutive splits (>= npx)
 };
 
 struct DopParams {
@@ -206,5 +208,53 @@ struct ProfileTables {
 std::shared_ptr<const ProfileTables> profile_tables(int ns, int M);
 bool precision_tables(const double *t, int ns, double f0, double krate,
                       std::vector<double> &kanchor, double &dk, std::string &why);
+
+// Host ingest (tdc_host.cpp). Host threads for passes over one image's inputs.
+constexpr unsigned kHostThreads = 8;
+// Runs fn(begin, end) over contiguous parts of [0, n), each of at least
+// `grain` items, on the host pool and the calling thread; returns when all
+// parts have run.
+void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_t)> &fn);
+
+// One pass over per-sample antenna states [npri][ns][3|3|4] (vel and att may
+// be null): finiteness, the largest |v|^2 and |q|^2, the bounding box of the
+// positions, and whether each of pos, vel, att is bitwise the same for all
+// samples of every chirp.
+struct AntennaScan {
+    bool has_vel = false, has_att = false;
+    bool finite = true;
+    bool pos_const = true, vel_const = true, att_const = true;
+    double vmax2 = 0.0, qmax2 = 0.0;
+    float lo[3] = {0.0F, 0.0F, 0.0F}, hi[3] = {0.0F, 0.0F, 0.0F};
+};
+AntennaScan scan_antenna(const float *pos, const float *vel, const float *att, int npri, int ns);
+
+// Host-to-device copies through two page-locked slots on a copy stream of
+// their own: put() returns once the source has been read (into a slot) and
+// the DMA from that slot is queued; fence() makes a compute stream wait for
+// every copy queued so far; begin() makes later copies wait for the work
+// queued on a compute stream so far (buffers it still reads).
+class Ingest {
+  public:
+    Ingest() = default;
+    Ingest(const Ingest &) = delete;
+    Ingest &operator=(const Ingest &) = delete;
+    ~Ingest();
+    hipError_t begin(hipStream_t compute);
+    hipError_t put(void *dst, const void *src, size_t bytes);
+    hipError_t fence(hipStream_t compute);
+
+  private:
+    static constexpr size_t kSlotBytes = size_t(4) << 20;
+    static constexpr size_t kPieceBytes = size_t(2) << 20;
+    static constexpr size_t kParallelCopyBytes = size_t(256) << 10;
+    hipError_t init();
+    hipStream_t copy_ = nullptr;
+    void *slot_[2] = {nullptr, nullptr};
+    hipEvent_t free_[2] = {nullptr, nullptr};
+    hipEvent_t tail_ = nullptr;
+    int cur_ = 0;
+    size_t used_ = 0;
+};
 
 } // namespace tdc

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void plain_kernel(const PlainRec *__restric
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
     plain_tile<P>(rec, grid, g.npx, g.ns, n0, n1, tile * P * kBlock,
-                  out + static_cast<size_t>(split) * g.npx);
+                  out + static_cast<size_t>(split) * g.pstride);
 }
 
 // ---------------------------------------------------------------------------
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
         n = nend;
     }
 
-    float2 *o = out + static_cast<size_t>(split) * g.npx;
+    float2 *o = out + static_cast<size_t>(split) * g.pstride;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         if (live[j]) o[pix[j]] = make_float2(ar[j], ai[j]);
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void range_kernel(const float4 *__restrict_
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
     range_tile<P>(rec, cpos, grid, g.npx, g.ns, n0, n1, tile * P * kBlock,
-                  out + static_cast<size_t>(split) * g.npx);
+                  out + static_cast<size_t>(split) * g.pstride);
 }
 
 // chunk_processed on stop-and-hop input: the three probe samples of a row
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
         n = nend;
     }
 
-    float2 *o = out + static_cast<size_t>(split) * g.npx;
+    float2 *o = out + static_cast<size_t>(split) * g.pstride;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         if (live[j]) o[pix[j]] = make_float2(ar[j], ai[j]);
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restric
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
     profile_tile(table, cpos, grid, pp, g.npx, n0, n1, tile * kBlock,
-                 out + static_cast<size_t>(split) * g.npx);
+                 out + static_cast<size_t>(split) * g.pstride);
 }
 
 // Sum of the per-split partial images in split order.
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void precision_kernel(
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
     precision_tile<P, RAW>(raw, cpos, grid, kanchor, dk, g.npx, g.ns, n0, n1, tile * P * kBlock,
-                           out + static_cast<size_t>(split) * g.npx);
+                           out + static_cast<size_t>(split) * g.pstride);
 }
 
 // Matrix engine staging: one workgroup per chirp. The chirp's samples (in
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(kBlock) void precision_matrix_kernel(
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
     matrix_tile<PARTS>(atab, inv_scale, cpos, grid, kanchor, dk, g.npx, matrix_quads(g.ns), n0, n1,
-                       tile * kBlock, out + static_cast<size_t>(split) * g.npx);
+                       tile * kBlock, out + static_cast<size_t>(split) * g.pstride);
 }
 
 __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const double2 *__restrict__ partial,
