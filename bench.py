@@ -52,13 +52,14 @@ FLOP_PER_TERM = {"plain": 20.0, "range": 12.0, "precision": 14.0}
 # complex MAC as a real 2x2 block) times hi*hi + hi*lo (+ lo*hi for c64 raw)
 MFMA_FLOP_PER_TERM = {"c64": 24.0, "f16": 16.0, "bf16": 16.0}
 # Arithmetic the range-profile kernel executes per (pixel, chirp): FP64 --
-# |pos|^2 - 2 pos.p + |p|^2 (7), the Newton range step (5), the bin position
-# (4), the anchor phase (1), the chirp accumulation (2); FP32 -- the kernel
-# exponent arguments and exp2 (7), 23 weight products, 12 complex-by-real
-# MACs (48), the anchor sin/cos and rotation (8). Counted as FP32-equivalent
-# flops (an FP64 flop issues at half the FP32 rate) against the FP32 peak.
-PROFILE_FP64_PER_PAIR = 19.0
-PROFILE_FP32_PER_PAIR = 86.0
+# |pos|^2 - 2 pos.p + |p|^2 (7), the clamped Newton range step (5), the bin
+# position (2), the anchor phase (1), the block sums (2 per 8 chirps); FP32 --
+# the kernel exponent arguments and exp2 (6), 23 weight products, 12
+# complex-by-real MACs (48), the anchor sin/cos and the rotation into the
+# block sum (10). Counted as FP32-equivalent flops (an FP64 flop issues at
+# half the FP32 rate) against the FP32 peak.
+PROFILE_FP64_PER_PAIR = 15.25
+PROFILE_FP32_PER_PAIR = 87.0
 
 
 def log(msg):

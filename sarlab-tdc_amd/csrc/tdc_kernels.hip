@@ -488,8 +488,10 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
 // One workgroup per chirp: b[(i - h) mod M] = c_i * corr[i] (zero elsewhere),
 // then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-2 Stockham passes
 // between two LDS buffers (the output comes out in natural order), written
-// with kProfilePad wrap-around entries so that a pixel's taps m0 .. m0 + 11
-// never wrap.
+// shifted by profile::kLead bins, row[j] = B[(j - kLead) mod M], over M +
+// kProfilePad entries: a pixel whose bin position has integer part f reads
+// its taps B[f - kLead .. f - kLead + 11] as row[f .. f + 11], which never
+// wrap.
 template <class RAW>
 __global__ __launch_bounds__(kBlock) void stage_profile_kernel(
     const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
@@ -529,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void stage_profile_kernel(
         y = tmp;
     }
     float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += kBlock) dst[j] = x[j & mask];
+    for (int j = tid; j < Mp; j += kBlock) dst[j] = x[(j - profile::kLead) & mask];
 }
 
 __global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restrict__ table,

@@ -45,14 +45,16 @@ __device__ __forceinline__ double range_from_sq(double r2)
 
 // |d| from |d|^2 in FP64 with three FP64 operations: r0 = r2 y0 from the FP32
 // reciprocal-square-root seed y0, then one Newton step
-// r0 + (r2 - r0^2) y0 / 2 (relative error ~1e-14); zero (or a negative
-// rounding of zero) gives zero.
+// r0 + (r2 - r0^2) y0 / 2 (relative error ~1e-14). |d|^2 is first raised to
+// at least 1e-30 m^2, so that zero (or a negative rounding of zero, at a
+// pixel on the antenna's position) gives 1e-15 m, a phase of ~1e-12 cycles,
+// with a finite seed.
 __device__ __forceinline__ double range_newton(double r2)
 {
+    r2 = fmax(r2, 1e-30);
     const float y0f = __builtin_amdgcn_rsqf(static_cast<float>(r2));
     const double r0 = r2 * static_cast<double>(y0f);
-    const double r1 = fma(fma(-r0, r0, r2), static_cast<double>(0.5F * y0f), r0);
-    return r2 > 0.0 ? r1 : 0.0;
+    return fma(fma(-r0, r0, r2), static_cast<double>(0.5F * y0f), r0);
 }
 
 // sin and cos of 2 pi x for x in [0, 1): reduction to the nearest quarter
@@ -627,15 +629,22 @@ __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
 // exp(-b g^2) exp(2 b g)^s exp(-b s^2): three exp2 per pair, two multiplies
 // per tap), and the FP64 anchor exp(j 2 pi k_h R); chirps accumulate in FP64.
 namespace profile {
+// A pixel's bin position u = frac(dk R) M has integer part f and fraction g;
+// tap t (0 .. 11) is bin f - kLead + t, at distance g + kLead - t from u, and
+// weighs exp(-b (g + kLead - t)^2) = up0 e1^t kC[t] with
+// up0 = exp(-b g^2 - 2 kLead b g), e1 = exp(2 b g), kC[t] = exp(-b (kLead - t)^2)
+// (fast Gaussian gridding: two exponentials per pixel and chirp, then one
+// product per tap).
+constexpr int kLead = kProfileTaps / 2 - 1;
+constexpr int kProfileBlock = 8;
 constexpr float kE0 = -0.5665450177283992F; // -b log2(e), b = 0.75 pi / 6
 constexpr float kE1 = 1.1330900354567983F;  // 2 b log2(e)
-// exp(-b s^2) for s = -6 .. 5
-constexpr float kC[kProfileTaps] = {7.249472515987942e-07F, 5.449088911519817e-05F,
-                                    0.0018674427317079893F, 0.029179416423313852F,
-                                    0.20787957635076193F,   0.6752319066557773F,
-                                    1.0F,                   0.6752319066557773F,
-                                    0.20787957635076193F,   0.029179416423313852F,
-                                    0.0018674427317079893F, 5.449088911519817e-05F};
+constexpr float kC[kProfileTaps] = {5.449088911519817e-05F, 0.0018674427317079893F,
+                                    0.029179416423313852F,  0.20787957635076193F,
+                                    0.6752319066557773F,    1.0F,
+                                    0.6752319066557773F,    0.20787957635076193F,
+                                    0.029179416423313852F,  0.0018674427317079893F,
+                                    5.449088911519817e-05F, 7.249472515987942e-07F};
 } // namespace profile
 
 // Taps from a window of the table row staged by the wave itself: a wave's
@@ -661,7 +670,6 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                                              const ProfileParams &pp, int npx, int n0, int n1,
                                              int pix0, double2 *__restrict__ out)
 {
-    constexpr int kHalf = kProfileTaps / 2;
     __shared__ float2 s_win[kBlock / 64][64];
     const int lane = static_cast<int>(threadIdx.x) & 63;
     float2 *win = s_win[threadIdx.x >> 6];
@@ -675,56 +683,57 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     const double qx = -2.0 * px, qy = -2.0 * py, qz = -2.0 * pz;
     const int mask = pp.M - 1;
     const double fm = static_cast<double>(pp.M);
+    // chirp contributions are summed in FP32 over blocks of kProfileBlock
+    // chirps and the blocks in FP64
     double accr = 0.0, acci = 0.0;
-    for (int n = n0; n < n1; ++n) {
-        const ChirpPos c = as_const(cpos)[n];
-        const double rng = range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
-        const double y = fma(fract64(pp.dk * rng), fm, -static_cast<double>(kHalf));
-        const double m0f = ceil(y);
-        const float g = static_cast<float>(y - m0f);
-        // indices in unsigned arithmetic: every value is masked to the row or
-        // checked against the window before it addresses memory
-        const unsigned m0u = static_cast<unsigned>(static_cast<int>(m0f));
-        const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
-        const unsigned lo = __builtin_amdgcn_readfirstlane(m0u) - kWinReach;
-        const unsigned off = m0u - lo;
-        float2 v[kProfileTaps];
-        if (__all(off <= 64U - kProfileTaps)) {
-            const float2 e = row[(lo + lane) & mask];
-            wave_lds_sync(); // the previous chirp's reads of the slot are done
-            win[lane] = e;
-            wave_lds_sync();
+    for (int nb = n0; nb < n1; nb += profile::kProfileBlock) {
+        const int ne = min(n1, nb + profile::kProfileBlock);
+        float br = 0.0F, bi = 0.0F;
+        for (int n = nb; n < ne; ++n) {
+            const ChirpPos c = as_const(cpos)[n];
+            const double rng =
+                range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
+            const double u = fract64(pp.dk * rng) * fm; // in [0, M)
+            const float g = static_cast<float>(fract64(u));
+            // indices in unsigned arithmetic: f < M, so row[f .. f + 11] lies
+            // in the row; window indices are masked to the row
+            const unsigned f = static_cast<unsigned>(static_cast<int>(u));
+            const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
+            const unsigned lo = __builtin_amdgcn_readfirstlane(f) - kWinReach;
+            const unsigned off = f - lo;
+            float2 v[kProfileTaps];
+            if (__all(off <= 64U - kProfileTaps)) {
+                const float2 e = row[(lo + lane) & mask];
+                wave_lds_sync(); // the previous chirp's reads of the slot are done
+                win[lane] = e;
+                wave_lds_sync();
 #pragma unroll
-            for (int t = 0; t < kProfileTaps; ++t) v[t] = win[off + t];
-        } else {
-            const float2 *r = row + (m0u & static_cast<unsigned>(mask));
+                for (int t = 0; t < kProfileTaps; ++t) v[t] = win[off + t];
+            } else {
+                const float2 *r = row + f;
 #pragma unroll
-            for (int t = 0; t < kProfileTaps; ++t) v[t] = r[t];
+                for (int t = 0; t < kProfileTaps; ++t) v[t] = r[t];
+            }
+            // up0 = exp2(g (kE0 g - kLead kE1)), e1 = exp2(kE1 g): for g in
+            // [0, 1) up0 lies in (0.013, 1] and up0 e1^11 below 6e3
+            float up = __builtin_amdgcn_exp2f(
+                g * fmaf(profile::kE0, g, -profile::kLead * profile::kE1));
+            const float e1 = __builtin_amdgcn_exp2f(profile::kE1 * g);
+            float ar = 0.0F, ai = 0.0F;
+#pragma unroll
+            for (int t = 0; t < kProfileTaps; ++t) {
+                const float w = up * profile::kC[t];
+                ar = fmaf(w, v[t].x, ar);
+                ai = fmaf(w, v[t].y, ai);
+                up *= e1;
+            }
+            const float xa = static_cast<float>(fract64(pp.kh * rng));
+            const float zr = cos_rev(xa), zi = sin_rev(xa);
+            br = fmaf(zr, ar, fmaf(-zi, ai, br));
+            bi = fmaf(zr, ai, fmaf(zi, ar, bi));
         }
-        const float e0 = __builtin_amdgcn_exp2f(profile::kE0 * (g * g));
-        const float e1 = __builtin_amdgcn_exp2f(profile::kE1 * g);
-        const float e1i = __builtin_amdgcn_exp2f(-profile::kE1 * g);
-        float ar = 0.0F, ai = 0.0F;
-        float up = e0;
-#pragma unroll
-        for (int t = kHalf; t < kProfileTaps; ++t) {
-            const float w = up * profile::kC[t];
-            ar = fmaf(w, v[t].x, ar);
-            ai = fmaf(w, v[t].y, ai);
-            up *= e1;
-        }
-        float dn = e0 * e1i;
-#pragma unroll
-        for (int t = kHalf - 1; t >= 0; --t) {
-            const float w = dn * profile::kC[t];
-            ar = fmaf(w, v[t].x, ar);
-            ai = fmaf(w, v[t].y, ai);
-            dn *= e1i;
-        }
-        const float xa = static_cast<float>(fract64(pp.kh * rng));
-        const float zr = cos_rev(xa), zi = sin_rev(xa);
-        accr += static_cast<double>(fmaf(zr, ar, -(zi * ai)));
-        acci += static_cast<double>(fmaf(zr, ai, zi * ar));
+        accr += static_cast<double>(br);
+        acci += static_cast<double>(bi);
     }
     if (pix < npx) out[pix] = make_double2(accr, acci);
 }
