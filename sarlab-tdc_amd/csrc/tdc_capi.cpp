@@ -448,8 +448,12 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         const double f = double(h->start_freq) + double(h->mod_rate) * double(h->sample_times[m]);
         kt[m] = static_cast<float>(2.0 * f / kSpeedOfLight);
     }
-    const tdc::AntennaScan scan = tdc::scan_antenna(h->pos, h->vel, h->att, npri, ns);
-    h->doppler = !doppler_window_is_one(h, scan, dop_center, dop_bw, centroid, taper);
+    tdc::AntennaScan scan;
+    {
+        tdc::Range checks("tdc_check_inputs");
+        scan = tdc::scan_antenna(h->pos, h->vel, h->att, npri, ns);
+        h->doppler = !doppler_window_is_one(h, scan, dop_center, dop_bw, centroid, taper);
+    }
     if (h->doppler && (!h->vel || !h->att))
         return fail(TDC_ERR_INVALID_ARGUMENT, "the Doppler window needs velocity and attitude");
     // stop-and-hop input: with the position constant within every chirp the
@@ -458,6 +462,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     h->chirp_path = scan.pos_const && (!h->doppler || (scan.vel_const && scan.att_const));
     hipStream_t s = h->stream;
     int rc;
+    tdc::Range uploads("tdc_upload_inputs");
     TDC_HIP(h->ingest.begin(s));
     if ((rc = upload(h, h->d_raw, h->raw, nterm * 2)) != TDC_OK) return rc;
     if ((rc = upload(h, h->d_kt, kt.data(), ns)) != TDC_OK) return rc;
@@ -554,6 +559,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     } else {
         if (!h->have_raw32) return fail(TDC_ERR_STATE, "setRawData must be called before start()");
         if (!h->grid) return fail(TDC_ERR_STATE, "setFocusGrid must be called before start()");
+        tdc::Range checks("tdc_check_inputs");
         const tdc::AntennaScan scan = tdc::scan_antenna(h->pos, h->vel, h->att, npri, ns);
         if (!doppler_window_is_one(h, scan, dop_center, dop_bw, centroid, taper))
             return fail(TDC_ERR_UNSUPPORTED,
@@ -600,6 +606,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
 
     hipStream_t s = h->stream;
     int rc;
+    tdc::Range uploads("tdc_upload_inputs");
     TDC_HIP(h->ingest.begin(s));
     if ((rc = upload(h, h->d_raw, h->raw, nterm * 2)) != TDC_OK) return rc;
     if ((rc = upload(h, h->d_wr, wr.data(), ns)) != TDC_OK) return rc;

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdint>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -155,42 +156,78 @@ void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_
 
 namespace {
 
-bool same_bits(const float *a, const float *b, int w) { return std::memcmp(a, b, sizeof(float) * w) == 0; }
+inline uint32_t bits_of(float v)
+{
+    uint32_t b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+}
 
+// 1 when the float with bit pattern b is an infinity or a NaN
+inline uint32_t nonfinite(uint32_t b) { return (b & 0x7F800000U) == 0x7F800000U ? 1U : 0U; }
+
+// The samples of chirps [n0, n1): bit patterns are compared with the chirp's
+// first sample by XOR and OR-accumulated, exponents tested for all ones, and
+// no branch depends on the data, so that the inner loops vectorise.
 void scan_chirps(const float *pos, const float *vel, const float *att, int ns, size_t n0,
                  size_t n1, AntennaScan &s)
 {
+    uint32_t bad = 0, dpos = 0, dvel = 0, datt = 0;
+    float lo0 = s.lo[0], lo1 = s.lo[1], lo2 = s.lo[2], hi0 = s.hi[0], hi1 = s.hi[1], hi2 = s.hi[2];
+    double vmax2 = s.vmax2, qmax2 = s.qmax2;
     for (size_t n = n0; n < n1; ++n) {
         const size_t e0 = n * static_cast<size_t>(ns);
+        const float *p = pos + 3 * e0;
+        const uint32_t px = bits_of(p[0]), py = bits_of(p[1]), pz = bits_of(p[2]);
         for (int m = 0; m < ns; ++m) {
-            const size_t e = e0 + m;
-            const float *p = pos + 3 * e;
-            for (int a = 0; a < 3; ++a) {
-                const float v = p[a];
-                if (!std::isfinite(v)) s.finite = false;
-                s.lo[a] = std::min(s.lo[a], v);
-                s.hi[a] = std::max(s.hi[a], v);
+            const float x = p[3 * m], y = p[3 * m + 1], z = p[3 * m + 2];
+            const uint32_t bx = bits_of(x), by = bits_of(y), bz = bits_of(z);
+            bad |= nonfinite(bx) | nonfinite(by) | nonfinite(bz);
+            dpos |= (bx ^ px) | (by ^ py) | (bz ^ pz);
+            lo0 = std::min(lo0, x);
+            lo1 = std::min(lo1, y);
+            lo2 = std::min(lo2, z);
+            hi0 = std::max(hi0, x);
+            hi1 = std::max(hi1, y);
+            hi2 = std::max(hi2, z);
+        }
+        if (vel) {
+            const float *v = vel + 3 * e0;
+            const uint32_t vx = bits_of(v[0]), vy = bits_of(v[1]), vz = bits_of(v[2]);
+            for (int m = 0; m < ns; ++m) {
+                const float x = v[3 * m], y = v[3 * m + 1], z = v[3 * m + 2];
+                const uint32_t bx = bits_of(x), by = bits_of(y), bz = bits_of(z);
+                bad |= nonfinite(bx) | nonfinite(by) | nonfinite(bz);
+                dvel |= (bx ^ vx) | (by ^ vy) | (bz ^ vz);
+                vmax2 = std::max(vmax2, double(x) * x + double(y) * y + double(z) * z);
             }
-            if (m > 0 && s.pos_const && !same_bits(p, pos + 3 * e0, 3)) s.pos_const = false;
-            if (vel) {
-                const float *v = vel + 3 * e;
-                if (!std::isfinite(v[0]) || !std::isfinite(v[1]) || !std::isfinite(v[2]))
-                    s.finite = false;
-                s.vmax2 = std::max(s.vmax2, double(v[0]) * v[0] + double(v[1]) * v[1] +
-                                                double(v[2]) * v[2]);
-                if (m > 0 && s.vel_const && !same_bits(v, vel + 3 * e0, 3)) s.vel_const = false;
-            }
-            if (att) {
-                const float *q = att + 4 * e;
-                if (!std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2]) ||
-                    !std::isfinite(q[3]))
-                    s.finite = false;
-                s.qmax2 = std::max(s.qmax2, double(q[0]) * q[0] + double(q[1]) * q[1] +
-                                                double(q[2]) * q[2] + double(q[3]) * q[3]);
-                if (m > 0 && s.att_const && !same_bits(q, att + 4 * e0, 4)) s.att_const = false;
+        }
+        if (att) {
+            const float *q = att + 4 * e0;
+            const uint32_t qa = bits_of(q[0]), qb = bits_of(q[1]), qc = bits_of(q[2]),
+                           qd = bits_of(q[3]);
+            for (int m = 0; m < ns; ++m) {
+                const float a = q[4 * m], b = q[4 * m + 1], c = q[4 * m + 2], d = q[4 * m + 3];
+                const uint32_t ba = bits_of(a), bb = bits_of(b), bc = bits_of(c), bd = bits_of(d);
+                bad |= nonfinite(ba) | nonfinite(bb) | nonfinite(bc) | nonfinite(bd);
+                datt |= (ba ^ qa) | (bb ^ qb) | (bc ^ qc) | (bd ^ qd);
+                qmax2 = std::max(qmax2, double(a) * a + double(b) * b + double(c) * c +
+                                            double(d) * d);
             }
         }
     }
+    s.finite = s.finite && bad == 0;
+    s.pos_const = s.pos_const && dpos == 0;
+    s.vel_const = s.vel_const && dvel == 0;
+    s.att_const = s.att_const && datt == 0;
+    s.lo[0] = lo0;
+    s.lo[1] = lo1;
+    s.lo[2] = lo2;
+    s.hi[0] = hi0;
+    s.hi[1] = hi1;
+    s.hi[2] = hi2;
+    s.vmax2 = vmax2;
+    s.qmax2 = qmax2;
 }
 
 } // namespace
@@ -270,18 +307,22 @@ hipError_t Ingest::put(void *dst, const void *src, size_t bytes)
         const size_t piece = std::min(bytes, kPieceBytes);
         if (used_ + piece > kSlotBytes) {
             // the other slot is free once the copies issued from it have ended
+            Range wait("tdc_ingest_slot_wait");
             if ((e = hipEventRecord(free_[cur_], copy_)) != hipSuccess) return e;
             cur_ ^= 1;
             if ((e = hipEventSynchronize(free_[cur_])) != hipSuccess) return e;
             used_ = 0;
         }
         char *slot = static_cast<char *>(slot_[cur_]) + used_;
-        if (piece >= kParallelCopyBytes) {
-            parallel_for(piece, kParallelCopyBytes / 2, [&](size_t b, size_t end) {
-                std::memcpy(slot + b, from + b, end - b);
-            });
-        } else {
-            std::memcpy(slot, from, piece);
+        {
+            Range fill("tdc_ingest_slot_fill");
+            if (piece >= kParallelCopyBytes) {
+                parallel_for(piece, kParallelCopyBytes / 2, [&](size_t b, size_t end) {
+                    std::memcpy(slot + b, from + b, end - b);
+                });
+            } else {
+                std::memcpy(slot, from, piece);
+            }
         }
         if ((e = hipMemcpyAsync(to, slot, piece, hipMemcpyHostToDevice, copy_)) != hipSuccess)
             return e;
