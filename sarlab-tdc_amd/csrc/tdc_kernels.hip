@@ -886,17 +886,30 @@ hipError_t launch_stage_profile(int storage, const void *staged, const float *co
     if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || Mp < M + kProfileTaps - 1)
         return hipErrorInvalidValue;
     const size_t lds = 2 * sizeof(float2) * static_cast<size_t>(M);
+    // more than 64 KiB of dynamic LDS is granted to a kernel on request
+    auto allow = [lds](const void *kernel) {
+        if (lds <= (size_t(64) << 10)) return hipSuccess;
+        return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(lds));
+    };
+    hipError_t e = hipSuccess;
     switch (storage) {
     case 0:
+        if ((e = allow(reinterpret_cast<const void *>(&stage_profile_kernel<RawC64>))) != hipSuccess)
+            return e;
         hipLaunchKernelGGL(stage_profile_kernel<RawC64>, dim3(npri), dim3(kBlock), lds, s,
                            static_cast<const float2 *>(staged), corr, twiddle, ns, M, Mp, table);
         break;
     case 1:
+        if ((e = allow(reinterpret_cast<const void *>(&stage_profile_kernel<RawF16>))) != hipSuccess)
+            return e;
         hipLaunchKernelGGL(stage_profile_kernel<RawF16>, dim3(npri), dim3(kBlock), lds, s,
                            static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
                            table);
         break;
     case 2:
+        if ((e = allow(reinterpret_cast<const void *>(&stage_profile_kernel<RawBF16>))) != hipSuccess)
+            return e;
         hipLaunchKernelGGL(stage_profile_kernel<RawBF16>, dim3(npri), dim3(kBlock), lds, s,
                            static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
                            table);

@@ -299,10 +299,11 @@ def test_matrix_engine_sample_counts(tdc, orc, ns):
 
 
 @pytest.mark.parametrize("ns", [1, 2, 7, 8, 9, 64, 65, 127, 300, 512, 513, 1000, 1024, 1025,
-                                1500, 2048])
+                                1500, 2048, 2049, 3000, 4096])
 def test_profile_engine_sample_counts(tdc, orc, ns):
-    """FFT lengths 16..4096 (M >= 2 ns, at least 16), odd and even re-centring
-    samples, one-sample chirps, and the 2048-sample limit."""
+    """FFT lengths 16..8192 (M >= 2 ns, at least 16), odd and even re-centring
+    samples, one-sample chirps, FFTs whose two LDS buffers exceed 64 KiB
+    (M = 8192 from 2049 samples), and the 4096-sample limit."""
     s = ragged_scene(npri=40, ns=ns, rows=9, cols=11, seed=ns)
     img, st = form(tdc, f64=s.f64(), kind="precision", engine="profile")
     assert st["engine"] == "profile"
@@ -331,10 +332,10 @@ def test_profile_engine_window_and_gather_paths(tdc, orc, spacing):
 
 
 def test_profile_engine_sample_limit(tdc, orc):
-    """Beyond 2048 samples per chirp the range-profile engine declines with a
+    """Beyond 4096 samples per chirp the range-profile engine declines with a
     message and the automatic choice takes the matrix engine."""
-    s = ragged_scene(npri=12, ns=2049, rows=5, cols=6, seed=3)
-    with pytest.raises(Exception, match="at most 2048 samples"):
+    s = ragged_scene(npri=12, ns=4097, rows=5, cols=6, seed=3)
+    with pytest.raises(Exception, match="at most 4096 samples"):
         form(tdc, f64=s.f64(), kind="precision", engine="profile")
     img, st = form(tdc, f64=s.f64(), kind="precision")
     assert st["engine"] == "matrix"
