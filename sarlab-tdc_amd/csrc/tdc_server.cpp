@@ -55,11 +55,11 @@ constexpr double kC = 299792458.0;
 constexpr double kDefaultStepTerms = 4e10;
 constexpr double kTileTerms = 8388608.0; // ~2^23 work units per tile
 // Work units per term of each tile body, one unit being a term of the matrix
-// engine (measured on 1024 x 512 x 512^2: matrix 3.76 ms, plain per-chirp range
-// 30.7 ms) and, for the range-profile engine, units per (pixel, chirp)
-// (0.71 ms there: ~97 matrix terms), so that a step's budget is device time.
+// engine (measured on 1024 x 512 x 512^2: matrix 3.70 ms, plain per-chirp range
+// 30.6 ms) and, for the range-profile engine, units per (pixel, chirp)
+// (0.59 ms there: ~82 matrix terms), so that a step's budget is device time.
 constexpr double kPlainTermCost = 8.0;
-constexpr double kProfilePairCost = 97.0;
+constexpr double kProfilePairCost = 82.0;
 constexpr int kFinWorkgroups = 256;
 constexpr int kBuckets[] = {256, 512, 1024, 2048};
 constexpr int kNumBuckets = 4;
