@@ -798,7 +798,7 @@ extern "C" {
 
 const char *tdc_last_error(void) { return t_err.c_str(); }
 
-int tdc_abi_version(void) { return 10300; }
+int tdc_abi_version(void) { return 10400; }
 
 int tdc_device_count(void)
 {

@@ -42,7 +42,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_abi_version_and_device_count(lib):
-    assert lib.tdc_abi_version() == 10300
+    assert lib.tdc_abi_version() == 10400
     assert lib.tdc_device_count() >= 0
 
 
