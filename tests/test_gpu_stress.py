@@ -83,3 +83,112 @@ def test_random_geometry(tdc, orc, case):
             p.fetchImage()
             again = p.getFocusedImage()
             assert np.array_equal(img.view(np.uint32), again.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------- plain form
+
+N_PLAIN = 16
+PLAIN_SAMPLES = [1, 2, 3, 17, 64, 100, 128, 255, 256, 512]
+
+
+def random_dropin(case):
+    """The reference's float32 inputs for a random stop-and-hop or moving
+    antenna, with a random attitude near the identity and a random speed."""
+    from sarlabtdc import scene
+
+    rng = np.random.default_rng([4052, case])
+    npri = int(rng.integers(1, 301))
+    ns = int(rng.choice(PLAIN_SAMPLES))
+    rows = int(rng.integers(1, 26))
+    cols = int(rng.integers(1, min(60, 1000 // rows) + 1))
+    spacing = float(np.exp(rng.uniform(np.log(1e-3), np.log(0.05))))
+    stand_off = float(rng.uniform(0.3, 3.0))
+    track = np.zeros((npri, 3))
+    length = float(rng.uniform(0.05, 1.5))
+    track[:, 0] = np.linspace(-length / 2, length / 2, npri)
+    track += rng.normal(scale=float(rng.uniform(0, 1e-3)), size=track.shape)
+    grid = scene.plane_grid(rows, cols, spacing, (float(rng.uniform(-0.3, 0.3)), stand_off, 0.0))
+    scat = scene._uniform_on_grid(rng, grid, 5)
+    speed = float(rng.uniform(0.0, 40.0))
+    s = scene.synthesize("stress", track, ns, grid, scat, scene._cn(rng, 5, 1.0), 30.0, rng,
+                         speed=speed)
+    d = s.dropin()
+    q = np.array([1.0, *rng.normal(scale=0.05, size=3)], dtype=np.float32)
+    d["attitude"] = np.ascontiguousarray(np.broadcast_to(q, d["attitude"].shape))
+    moving = bool(rng.random() < 0.4)
+    if moving:
+        vel = d["velocity"].astype(np.float64)
+        d["position"] = np.ascontiguousarray(
+            (d["position"].astype(np.float64) + vel * s.sample_times[None, :, None])
+            .astype(np.float32))
+    # bands from 200 Hz (most chunks skipped) to 20 kHz (the window rarely bites)
+    bandwidth = float(np.exp(rng.uniform(np.log(200.0), np.log(2e4))))
+    window = dict(center=float(rng.uniform(-2e3, 2e3)), bandwidth=bandwidth,
+                  centroid=bool(rng.random() < 0.5), taper=bool(rng.random() < 0.5))
+    desc = dict(npri=npri, ns=ns, rows=rows, cols=cols, spacing=spacing, range=stand_off,
+                speed=speed, moving=moving, **window)
+    return d, window, desc
+
+
+skipped_any = []
+
+
+@pytest.mark.parametrize("case", range(N_PLAIN))
+def test_random_dropin(tdc, orc, case):
+    """The plain form (the drop-in) on random inputs, without and with the
+    Doppler window: the image within the plain form's tolerances of the FP64
+    oracle (2e-3; 5e-3 with the window, whose band edge FP32 and FP64 may put
+    a term on either side of), the skip counts equal to the oracle's skip
+    test, the path taken as the inputs dictate, and start() bitwise equal to
+    the resident pass."""
+    d, w, desc = random_dropin(case)
+    print("case", case, desc, flush=True)
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setRawData(d["rawData"], d["priTimes"], d["sampleTimes"], d["position"], d["velocity"],
+                 d["attitude"], d["modRate"], d["startFreq"], d["bodyBoresight"])
+    p.setFocusGrid(d["focusGrid"])
+    dop = dict(velocity=d["velocity"], attitude=d["attitude"], boresight=d["bodyBoresight"],
+               **w)
+    g = d["focusGrid"].reshape(-1, 3).astype(np.float64)
+    nchunks = (d["rawData"].shape[0] + 127) // 128
+    for window in (None, w):
+        args = ((w["center"], w["bandwidth"], w["centroid"], False, w["taper"]) if window
+                else (0.0, 1e30, False, False, False))
+        p.start(*args)
+        img = p.getFocusedImage().copy()
+        st = p.stats()
+        assert st["range_per_chirp"] == (not desc["moving"])
+        ref = orc.image_f64(d["rawData"], d["sampleTimes"], d["position"], d["focusGrid"],
+                            float(d["modRate"]), float(d["startFreq"]),
+                            doppler=dop if window else None)
+        if np.linalg.norm(ref) == 0.0:
+            # the band excludes every term (the probes of the skip test may still
+            # pass every chunk): the image is exactly zero, as the oracle's
+            assert not np.any(img), np.abs(img).max()
+            e = 0.0
+        else:
+            e = float(orc.rel_l2(img, ref))
+        print(f"  window {'on ' if window else 'off'}: rel-L2 {e:.2e}, doppler "
+              f"{st['doppler_active']}, skipped {st['skipped_chunks']}", flush=True)
+        assert e <= (5e-3 if window else 2e-3)
+        if window and st["doppler_active"]:
+            processed = sum(
+                orc.chunk_processed(d["rawData"], d["sampleTimes"], d["position"],
+                                    float(d["modRate"]), float(d["startFreq"]), dop, c, g[i])
+                for i in range(g.shape[0]) for c in range(nchunks))
+            assert st["processed_chunks"] == processed
+            assert st["processed_chunks"] + st["skipped_chunks"] == g.shape[0] * nchunks
+            skipped_any.append(st["skipped_chunks"] > 0)
+        p.prepare(*args)
+        p.run(1)
+        p.fetchImage()
+        assert np.array_equal(img.view(np.uint32), p.getFocusedImage().view(np.uint32))
+
+
+def test_random_dropin_cases_skip_chunks():
+    """The random bands above make the skip test drop whole chunks in several
+    cases (collected by the cases run before this one)."""
+    if len(skipped_any) == 0:
+        pytest.skip("no random drop-in case ran in this session")
+    assert sum(skipped_any) >= 3, skipped_any
