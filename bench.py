@@ -66,6 +66,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or the platform's."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or platform.machine()
+
+
 def make_proc(tdc, s, kind, storage="c64", engine="auto", moving=False):
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
@@ -370,16 +384,30 @@ def main():
         t0 = time.perf_counter()
         oracle.refmethod_f32(d, 0.0, 1e30, False, False, False, pixels=cal, nthreads=cores)
         dt = time.perf_counter() - t0
-        n_px = int(min(s.n_pixels, max(n_cal, args.cpu_seconds / max(dt, 1e-6) * n_cal)))
+        # three timed runs of a third of the time budget each; the median is reported
+        n_px = int(min(s.n_pixels, max(n_cal, args.cpu_seconds / 3 / max(dt, 1e-6) * n_cal)))
         n_px = max(cores, n_px // cores * cores)
         sample = rng.choice(s.n_pixels, size=n_px, replace=False)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.refmethod_f32(d, 0.0, 1e30, False, False, False, pixels=sample, nthreads=cores)
+            runs.append(time.perf_counter() - t0)
+        dt = sorted(runs)[1]
+        # the same method on one thread, over the calibration pixels
         t0 = time.perf_counter()
-        oracle.refmethod_f32(d, 0.0, 1e30, False, False, False, pixels=sample, nthreads=cores)
-        dt = time.perf_counter() - t0
+        oracle.refmethod_f32(d, 0.0, 1e30, False, False, False, pixels=cal, nthreads=1)
+        dt1 = time.perf_counter() - t0
         cpu = dict(value=n_px * per_px / dt / 1e9, unit=UNIT, cores=cores, kind="port",
+                   single_thread=n_cal * per_px / dt1 / 1e9, nproc=os.cpu_count(),
+                   cpu_model=cpu_model(),
                    sample=f"{args.workload}: {n_px} random pixels x {s.n_chirps} chirps x "
-                          f"{s.n_samples} samples, FP32 reference method, {dt:.1f} s")
-        log(f"cpu baseline: {cpu['value']:.4f} G terms/s on {cores} cores ({dt:.1f} s)")
+                          f"{s.n_samples} samples, FP32 reference method, median of 3 runs "
+                          f"({', '.join(f'{r:.1f}' for r in runs)} s); single thread over "
+                          f"{n_cal} pixels ({dt1:.1f} s)")
+        log(f"cpu baseline: {cpu['value']:.4f} G terms/s on {cores} cores (median of "
+            f"{', '.join(f'{r:.1f}' for r in runs)} s), {cpu['single_thread']:.4f} on one thread; "
+            f"{cpu['cpu_model']}, nproc {cpu['nproc']}")
 
     serving = serve_leg(tdc, args) if args.serve_jobs > 0 else None
 
