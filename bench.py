@@ -449,7 +449,8 @@ def main():
         if simd is not None and sq.get("SQ_INSTS_VALU"):
             wave_chirps = s.n_pixels / 64.0 * s.n_chirps
             sq = dict(sq, **tj.get("sq64", {}))
-            simd = dict(valu_insts_per_wave_chirp=round(sq["SQ_INSTS_VALU"] / wave_chirps, 2),
+            simd = dict(valu_issue_frac=simd["valu_issue_frac"],
+                        valu_insts_per_wave_chirp=round(sq["SQ_INSTS_VALU"] / wave_chirps, 2),
                         **{k.lower().replace("sq_insts_", "") + "_per_wave_chirp":
                            round(sq[k] / wave_chirps, 2)
                            for k in ("SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_FMA_F64",
@@ -487,6 +488,9 @@ def main():
                     traffic=traffic, frac_nonpacked=round(algorithmic / PEAK_FP32_NONPACKED, 4),
                     flop_per_term=flop, kernel=f"{kname}_kernel ({head['storage']})",
                     simd_cycles=simd)
+    # HBM bytes per launch (committed PMC pass) over this run's kernel time
+    roof["hbm_gbs"] = (round(traffic / (head["main_kernel_ms"] * 1e-3) / 1e9, 1)
+                       if traffic else None)
     out = {
         "metric": METRIC,
         "value": round(head["gterms"], 3),
