@@ -276,16 +276,31 @@ Ingest::~Ingest()
     if (copy_) (void) hipStreamDestroy(copy_);
 }
 
+// Creates what is still missing (a failed call leaves the rest for the next).
 hipError_t Ingest::init()
 {
-    if (copy_) return hipSuccess;
-    hipError_t e = hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking);
-    if (e != hipSuccess) return e;
-    for (int i = 0; i < 2; ++i) {
-        if ((e = hipHostMalloc(&slot_[i], kSlotBytes, hipHostMallocDefault)) != hipSuccess) return e;
-        if ((e = hipEventCreateWithFlags(&free_[i], hipEventDisableTiming)) != hipSuccess) return e;
+    if (ready_) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (!copy_ && (e = hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking)) != hipSuccess) {
+        copy_ = nullptr;
+        return e;
     }
-    return hipEventCreateWithFlags(&tail_, hipEventDisableTiming);
+    for (int i = 0; i < 2; ++i) {
+        if (!slot_[i] && (e = hipHostMalloc(&slot_[i], kSlotBytes, hipHostMallocDefault)) != hipSuccess) {
+            slot_[i] = nullptr;
+            return e;
+        }
+        if (!free_[i] && (e = hipEventCreateWithFlags(&free_[i], hipEventDisableTiming)) != hipSuccess) {
+            free_[i] = nullptr;
+            return e;
+        }
+    }
+    if (!tail_ && (e = hipEventCreateWithFlags(&tail_, hipEventDisableTiming)) != hipSuccess) {
+        tail_ = nullptr;
+        return e;
+    }
+    ready_ = true;
+    return hipSuccess;
 }
 
 hipError_t Ingest::begin(hipStream_t compute)

@@ -257,6 +257,7 @@ class Ingest {
     hipEvent_t tail_ = nullptr;
     int cur_ = 0;
     size_t used_ = 0;
+    bool ready_ = false;
 };
 
 } // namespace tdc
