@@ -49,10 +49,10 @@ extern "C" {
  * FP32-level accuracy. The range-profile engine takes each chirp's
  * oversampled spectrum once (an FFT in LDS) and interpolates it at every
  * pixel's range with a 12-tap Gaussian kernel (a type-2 non-uniform FFT; up
- * to 4096 samples per chirp). The matrix engine forms the 16-sample group
+ * to 8192 samples per chirp). The matrix engine forms the 16-sample group
  * sums of every 128-sample block with FP16 hi/lo MFMA (32x32x16). The vector
  * engine runs a one-sample rotation recurrence on the VALU. */
-#define TDC_ENGINE_AUTO 0    /* range profile when the chirp has at most 4096 samples, else
+#define TDC_ENGINE_AUTO 0    /* range profile when the chirp has at most 8192 samples, else
                                 matrix; vector when pixels_per_lane > 1 is forced */
 #define TDC_ENGINE_VECTOR 1
 #define TDC_ENGINE_MATRIX 2

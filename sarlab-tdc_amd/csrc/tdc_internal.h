@@ -97,9 +97,12 @@ struct DopParams {
 // kProfileTaps table entries around phi * M weighted by the Gaussian kernel
 // exp(-b u^2), b = 0.75 pi / (kProfileTaps / 2).
 constexpr int kProfileTaps = 12;
-// The FFT of one chirp runs in LDS: 2 M entries, <= 128 KiB of the 160 KiB a
-// gfx950 workgroup may address (chirps of up to 4096 samples)
-constexpr int kProfileMaxM = 8192;
+// The FFT of one chirp runs in the LDS, of which a gfx950 workgroup may
+// address 160 KiB: radix-2 Stockham between two buffers (2 M entries) up to
+// M = 8192, in place in one buffer (M entries, 128 KiB) for M = 16384 (chirps
+// of up to 8192 samples)
+constexpr int kProfileStockhamMaxM = 8192;
+constexpr int kProfileMaxM = 16384;
 constexpr int kProfilePad = 16;    // wrap-around entries after the M bins of a table row
 struct ProfileParams {
     double dk; // 2 K dt / c: cycles per sample per metre of range

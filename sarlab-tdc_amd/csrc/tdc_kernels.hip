@@ -534,6 +534,53 @@ __global__ __launch_bounds__(kBlock) void stage_profile_kernel(
     for (int j = tid; j < Mp; j += kBlock) dst[j] = x[(j - profile::kLead) & mask];
 }
 
+// The same transform for chirps whose two Stockham buffers would not fit the
+// LDS (M above kProfileStockhamMaxM): in place in one buffer, radix-2
+// decimation in time from a bit-reversed placement of b, then written like
+// the Stockham kernel's rows.
+template <class RAW>
+__global__ __launch_bounds__(kBlock) void stage_profile_inplace_kernel(
+    const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
+    const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
+{
+    extern __shared__ float2 x[]; // M entries, sized at launch
+    const int n = static_cast<int>(blockIdx.x);
+    const int tid = static_cast<int>(threadIdx.x);
+    const int h = ns / 2;
+    const int mask = M - 1;
+    const int bits = __builtin_ctz(static_cast<unsigned>(M));
+    for (int j = tid; j < M; j += kBlock) x[j] = make_float2(0.0F, 0.0F);
+    __syncthreads();
+    const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
+    for (int i = tid; i < ns; i += kBlock) {
+        float re, im;
+        RAW::get(row, i, re, im);
+        const float c = corr[i];
+        const unsigned j = static_cast<unsigned>((i - h) & mask);
+        x[__builtin_bitreverse32(j) >> (32 - bits)] = make_float2(re * c, im * c);
+    }
+    __syncthreads();
+    for (int lg = 0; (2 << lg) <= M; ++lg) {
+        // butterflies of span 2^(lg+1): (base, base + half), twiddle exp(+j 2 pi k / M)
+        const int half = 1 << lg;
+        const int stride = M >> (lg + 1);
+        for (int idx = tid; idx < M / 2; idx += kBlock) {
+            const int k = idx & (half - 1);
+            const int base = ((idx >> lg) << (lg + 1)) + k;
+            const float2 w = twiddle[k * stride];
+            const float2 a = x[base];
+            const float2 d = x[base + half];
+            const float cr = fmaf(d.x, w.x, -(d.y * w.y));
+            const float ci = fmaf(d.x, w.y, d.y * w.x);
+            x[base] = make_float2(a.x + cr, a.y + ci);
+            x[base + half] = make_float2(a.x - cr, a.y - ci);
+        }
+        __syncthreads();
+    }
+    float2 *dst = table + static_cast<size_t>(n) * Mp;
+    for (int j = tid; j < Mp; j += kBlock) dst[j] = x[(j - profile::kLead) & mask];
+}
+
 __global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restrict__ table,
                                                          const ChirpPos *__restrict__ cpos,
                                                          const double *__restrict__ grid,
@@ -879,45 +926,51 @@ hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
     return hipGetLastError();
 }
 
+namespace {
+
+template <class RAW>
+hipError_t stage_profile_raw(const void *staged, const float *corr, const float2 *twiddle,
+                             int npri, int ns, int M, int Mp, float2 *table, hipStream_t s)
+{
+    const bool inplace = M > kProfileStockhamMaxM;
+    const size_t lds = (inplace ? 1 : 2) * sizeof(float2) * static_cast<size_t>(M);
+    const void *kernel = inplace
+                             ? reinterpret_cast<const void *>(&stage_profile_inplace_kernel<RAW>)
+                             : reinterpret_cast<const void *>(&stage_profile_kernel<RAW>);
+    // more than 64 KiB of dynamic LDS is granted to a kernel on request
+    if (lds > (size_t(64) << 10)) {
+        const hipError_t e = hipFuncSetAttribute(
+            kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    const auto *src = static_cast<const typename RAW::T *>(staged);
+    if (inplace)
+        hipLaunchKernelGGL(stage_profile_inplace_kernel<RAW>, dim3(npri), dim3(kBlock), lds, s, src,
+                           corr, twiddle, ns, M, Mp, table);
+    else
+        hipLaunchKernelGGL(stage_profile_kernel<RAW>, dim3(npri), dim3(kBlock), lds, s, src, corr,
+                           twiddle, ns, M, Mp, table);
+    return hipGetLastError();
+}
+
+} // namespace
+
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp,
                                 float2 *table, hipStream_t s)
 {
     if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || Mp < M + kProfileTaps - 1)
         return hipErrorInvalidValue;
-    const size_t lds = 2 * sizeof(float2) * static_cast<size_t>(M);
-    // more than 64 KiB of dynamic LDS is granted to a kernel on request
-    auto allow = [lds](const void *kernel) {
-        if (lds <= (size_t(64) << 10)) return hipSuccess;
-        return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   static_cast<int>(lds));
-    };
-    hipError_t e = hipSuccess;
     switch (storage) {
     case 0:
-        if ((e = allow(reinterpret_cast<const void *>(&stage_profile_kernel<RawC64>))) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(stage_profile_kernel<RawC64>, dim3(npri), dim3(kBlock), lds, s,
-                           static_cast<const float2 *>(staged), corr, twiddle, ns, M, Mp, table);
-        break;
+        return stage_profile_raw<RawC64>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
     case 1:
-        if ((e = allow(reinterpret_cast<const void *>(&stage_profile_kernel<RawF16>))) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(stage_profile_kernel<RawF16>, dim3(npri), dim3(kBlock), lds, s,
-                           static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
-                           table);
-        break;
+        return stage_profile_raw<RawF16>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
     case 2:
-        if ((e = allow(reinterpret_cast<const void *>(&stage_profile_kernel<RawBF16>))) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(stage_profile_kernel<RawBF16>, dim3(npri), dim3(kBlock), lds, s,
-                           static_cast<const unsigned int *>(staged), corr, twiddle, ns, M, Mp,
-                           table);
-        break;
+        return stage_profile_raw<RawBF16>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,

@@ -208,7 +208,7 @@ struct Job {
     int64_t ntiles = 0, next_tile = 0;
     double terms = 0.0;
     double pair_cost = 0.0; // work units per (pixel, chirp)
-    // precision jobs: range-profile engine (chirps of at most 4096 samples) or
+    // precision jobs: range-profile engine (chirps of at most 8192 samples) or
     // matrix engine
     bool profile = false;
     tdc::ProfileParams pp{};

@@ -299,11 +299,12 @@ def test_matrix_engine_sample_counts(tdc, orc, ns):
 
 
 @pytest.mark.parametrize("ns", [1, 2, 7, 8, 9, 64, 65, 127, 300, 512, 513, 1000, 1024, 1025,
-                                1500, 2048, 2049, 3000, 4096])
+                                1500, 2048, 2049, 3000, 4096, 4097, 6000, 8192])
 def test_profile_engine_sample_counts(tdc, orc, ns):
-    """FFT lengths 16..8192 (M >= 2 ns, at least 16), odd and even re-centring
-    samples, one-sample chirps, FFTs whose two LDS buffers exceed 64 KiB
-    (M = 8192 from 2049 samples), and the 4096-sample limit."""
+    """FFT lengths 16..16384 (M >= 2 ns, at least 16), odd and even re-centring
+    samples, one-sample chirps, Stockham FFTs whose two LDS buffers exceed 64
+    KiB (M = 8192 from 2049 samples), the in-place FFT in one 128 KiB buffer
+    (M = 16384 from 4097 samples), and the 8192-sample limit."""
     s = ragged_scene(npri=40, ns=ns, rows=9, cols=11, seed=ns)
     img, st = form(tdc, f64=s.f64(), kind="precision", engine="profile")
     assert st["engine"] == "profile"
@@ -332,10 +333,10 @@ def test_profile_engine_window_and_gather_paths(tdc, orc, spacing):
 
 
 def test_profile_engine_sample_limit(tdc, orc):
-    """Beyond 4096 samples per chirp the range-profile engine declines with a
+    """Beyond 8192 samples per chirp the range-profile engine declines with a
     message and the automatic choice takes the matrix engine."""
-    s = ragged_scene(npri=12, ns=4097, rows=5, cols=6, seed=3)
-    with pytest.raises(Exception, match="at most 4096 samples"):
+    s = ragged_scene(npri=12, ns=8193, rows=5, cols=6, seed=3)
+    with pytest.raises(Exception, match="at most 8192 samples"):
         form(tdc, f64=s.f64(), kind="precision", engine="profile")
     img, st = form(tdc, f64=s.f64(), kind="precision")
     assert st["engine"] == "matrix"

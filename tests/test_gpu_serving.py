@@ -42,9 +42,9 @@ def test_single_job_parity(tdc, orc, form, storage, tol):
 
 
 def test_long_chirp_job_parity(tdc, orc):
-    """4100 samples per chirp: beyond the range-profile engine's 4096, the job
+    """8200 samples per chirp: beyond the range-profile engine's 8192, the job
     runs on the matrix engine's tiles (their own persistent kernel)."""
-    s = job_scene(2, 60, 4100, 40)
+    s = job_scene(2, 60, 8200, 40)
     srv = tdc.Server(0, 8e9)
     img, _ = srv.wait(submit(srv, s))
     srv.close()
@@ -57,7 +57,7 @@ def test_engines_bit_identical_alone_and_in_a_mix(tdc):
     """A range-profile job and a matrix-engine job, each alone and then in one
     mix with each other and a plain job: every image is the same bit for bit."""
     prof = job_scene(11, 150, 256, 80)
-    mat = job_scene(12, 40, 4100, 48)
+    mat = job_scene(12, 40, 8200, 48)
     plain = job_scene(13, 120, 128, 64)
     alone = []
     for s in (prof, mat):
