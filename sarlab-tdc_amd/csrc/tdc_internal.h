@@ -103,6 +103,8 @@ constexpr int kProfileTaps = 12;
 // of up to 8192 samples)
 constexpr int kProfileStockhamMaxM = 8192;
 constexpr int kProfileMaxM = 16384;
+// FFT length from which the per-chirp transform runs 1024 threads
+constexpr int kProfileWideM = 4096;
 constexpr int kProfilePad = 16;    // wrap-around entries after the M bins of a table row
 struct ProfileParams {
     double dk; // 2 K dt / c: cycles per sample per metre of range

@@ -492,8 +492,8 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
 // kProfilePad entries: a pixel whose bin position has integer part f reads
 // its taps B[f - kLead .. f - kLead + 11] as row[f .. f + 11], which never
 // wrap.
-template <class RAW>
-__global__ __launch_bounds__(kBlock) void stage_profile_kernel(
+template <class RAW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
     const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
 {
@@ -504,10 +504,10 @@ __global__ __launch_bounds__(kBlock) void stage_profile_kernel(
     const int mask = M - 1;
     float2 *x = buf;
     float2 *y = buf + M;
-    for (int j = tid; j < M; j += kBlock) x[j] = make_float2(0.0F, 0.0F);
+    for (int j = tid; j < M; j += BLOCK) x[j] = make_float2(0.0F, 0.0F);
     __syncthreads();
     const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
-    for (int i = tid; i < ns; i += kBlock) {
+    for (int i = tid; i < ns; i += BLOCK) {
         float re, im;
         RAW::get(row, i, re, im);
         const float c = corr[i];
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void stage_profile_kernel(
     }
     __syncthreads();
     for (int l = M / 2, s = 1, lg = 0; l >= 1; l >>= 1, s <<= 1, ++lg) {
-        for (int idx = tid; idx < M / 2; idx += kBlock) {
+        for (int idx = tid; idx < M / 2; idx += BLOCK) {
             const int j = idx >> lg;
             const int k = idx & (s - 1);
             const float2 w = twiddle[j * s];
@@ -531,15 +531,15 @@ __global__ __launch_bounds__(kBlock) void stage_profile_kernel(
         y = tmp;
     }
     float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += kBlock) dst[j] = x[(j - profile::kLead) & mask];
+    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - profile::kLead) & mask];
 }
 
 // The same transform for chirps whose two Stockham buffers would not fit the
 // LDS (M above kProfileStockhamMaxM): in place in one buffer, radix-2
 // decimation in time from a bit-reversed placement of b, then written like
 // the Stockham kernel's rows.
-template <class RAW>
-__global__ __launch_bounds__(kBlock) void stage_profile_inplace_kernel(
+template <class RAW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
     const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
     const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
 {
@@ -549,10 +549,10 @@ __global__ __launch_bounds__(kBlock) void stage_profile_inplace_kernel(
     const int h = ns / 2;
     const int mask = M - 1;
     const int bits = __builtin_ctz(static_cast<unsigned>(M));
-    for (int j = tid; j < M; j += kBlock) x[j] = make_float2(0.0F, 0.0F);
+    for (int j = tid; j < M; j += BLOCK) x[j] = make_float2(0.0F, 0.0F);
     __syncthreads();
     const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
-    for (int i = tid; i < ns; i += kBlock) {
+    for (int i = tid; i < ns; i += BLOCK) {
         float re, im;
         RAW::get(row, i, re, im);
         const float c = corr[i];
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void stage_profile_inplace_kernel(
         // butterflies of span 2^(lg+1): (base, base + half), twiddle exp(+j 2 pi k / M)
         const int half = 1 << lg;
         const int stride = M >> (lg + 1);
-        for (int idx = tid; idx < M / 2; idx += kBlock) {
+        for (int idx = tid; idx < M / 2; idx += BLOCK) {
             const int k = idx & (half - 1);
             const int base = ((idx >> lg) << (lg + 1)) + k;
             const float2 w = twiddle[k * stride];
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void stage_profile_inplace_kernel(
         __syncthreads();
     }
     float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += kBlock) dst[j] = x[(j - profile::kLead) & mask];
+    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - profile::kLead) & mask];
 }
 
 __global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restrict__ table,
@@ -928,15 +928,15 @@ hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
 
 namespace {
 
-template <class RAW>
-hipError_t stage_profile_raw(const void *staged, const float *corr, const float2 *twiddle,
-                             int npri, int ns, int M, int Mp, float2 *table, hipStream_t s)
+template <class RAW, int BLOCK>
+hipError_t stage_profile_launch(const void *staged, const float *corr, const float2 *twiddle,
+                                int npri, int ns, int M, int Mp, float2 *table, hipStream_t s)
 {
     const bool inplace = M > kProfileStockhamMaxM;
     const size_t lds = (inplace ? 1 : 2) * sizeof(float2) * static_cast<size_t>(M);
-    const void *kernel = inplace
-                             ? reinterpret_cast<const void *>(&stage_profile_inplace_kernel<RAW>)
-                             : reinterpret_cast<const void *>(&stage_profile_kernel<RAW>);
+    const void *kernel =
+        inplace ? reinterpret_cast<const void *>(&stage_profile_inplace_kernel<RAW, BLOCK>)
+                : reinterpret_cast<const void *>(&stage_profile_kernel<RAW, BLOCK>);
     // more than 64 KiB of dynamic LDS is granted to a kernel on request
     if (lds > (size_t(64) << 10)) {
         const hipError_t e = hipFuncSetAttribute(
@@ -945,12 +945,24 @@ hipError_t stage_profile_raw(const void *staged, const float *corr, const float2
     }
     const auto *src = static_cast<const typename RAW::T *>(staged);
     if (inplace)
-        hipLaunchKernelGGL(stage_profile_inplace_kernel<RAW>, dim3(npri), dim3(kBlock), lds, s, src,
-                           corr, twiddle, ns, M, Mp, table);
+        hipLaunchKernelGGL((stage_profile_inplace_kernel<RAW, BLOCK>), dim3(npri), dim3(BLOCK), lds,
+                           s, src, corr, twiddle, ns, M, Mp, table);
     else
-        hipLaunchKernelGGL(stage_profile_kernel<RAW>, dim3(npri), dim3(kBlock), lds, s, src, corr,
-                           twiddle, ns, M, Mp, table);
+        hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK>), dim3(npri), dim3(BLOCK), lds, s, src,
+                           corr, twiddle, ns, M, Mp, table);
     return hipGetLastError();
+}
+
+// One workgroup per chirp. Long transforms take most of a CU's LDS, so only
+// one workgroup fits per CU: they run 1024 threads (16 waves) to keep the
+// LDS busy across the passes' barriers; short ones run 256.
+template <class RAW>
+hipError_t stage_profile_raw(const void *staged, const float *corr, const float2 *twiddle,
+                             int npri, int ns, int M, int Mp, float2 *table, hipStream_t s)
+{
+    if (M >= kProfileWideM)
+        return stage_profile_launch<RAW, 1024>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
+    return stage_profile_launch<RAW, kBlock>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
 }
 
 } // namespace
