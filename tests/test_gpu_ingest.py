@@ -146,3 +146,23 @@ def test_small_image_runs_unstreamed(tdc):
     a, _ = streamed(p, WIN_OFF)
     b, _ = resident(p, WIN_OFF)
     assert_same(a, b)
+
+
+@pytest.mark.parametrize("kind,ppl,splits", [("plain", 2, 24), ("plain", 4, 40), ("plain", 1, 12),
+                                             ("precision", 2, 20), ("precision", 0, 12)])
+def test_start_streams_forced_launch_shapes(tdc, jitter, kind, ppl, splits):
+    """Forced pixels per lane (tiles of 512 or 1024 pixels) and chirp split
+    counts that are not powers of two (no XCD-aware mapping): the grid slabs
+    are cut at whole tiles and the partial images stay one full image apart,
+    so start() still forms the resident pass's image bit for bit."""
+    s = jitter
+    if kind == "precision":
+        p = processor(tdc, f=s.f64(), kind=kind, engine="vector" if ppl > 1 else "auto")
+    else:
+        p = processor(tdc, d=s.dropin(), kind=kind)
+    p.setLaunch(ppl, splits)
+    a, sa = streamed(p, WIN_OFF)
+    b, _ = resident(p, WIN_OFF)
+    assert sa["chirp_splits"] == splits
+    assert_streams(sa)
+    assert_same(a, b)
