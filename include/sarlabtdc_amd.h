@@ -198,6 +198,18 @@ int tdc_get_stats(const tdc_processor *h, tdc_stats *out);
  * form) and the chirp split count (0 = automatic). */
 int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits);
 
+/* Device memory the processor may use for one image's buffers, in bytes
+ * (0 = no limit, the default). No counterpart in the reference, whose
+ * start() stages 128-chirp chunks (tdcprocessor.cpp:245-345). When a
+ * precision-form image would exceed it, tdc_start streams the chirps through
+ * the device in slabs of whole chirp splits, two buffer sets in turn (one
+ * filled over PCIe while the other is formed), and adds each slab's splits in
+ * split order into an FP64 image: the image is the one a resident pass with
+ * the same split count forms, bit for bit (tdc_stats.chirp_splits reports
+ * that count). tdc_prepare refuses such an image. The plain form keeps its
+ * whole aperture resident and ignores the budget. */
+int tdc_set_device_budget(tdc_processor *h, unsigned long long bytes);
+
 /* ---- serving form (additive; no counterpart in the reference) ------------- */
 /* A server owns one GPU stream and a scheduler thread. Jobs of any shape are
  * submitted at any time; the scheduler admits them in arrival order while

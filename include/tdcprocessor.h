@@ -94,6 +94,7 @@ public:
     void setEngine(int engine);         /* TDC_ENGINE_* (precision form) */
     void setRangeWindow(int kind, double p0 = 0.0, double p1 = 0.0); /* TDC_WIN_* */
     void setLaunch(int pixelsPerLane, int chirpSplits);
+    void setDeviceBudget(unsigned long long bytes); /* 0: no limit */
     void prepare(float dopplerWinCenter, float dopplerBandwidth, bool dopCentroidWin,
                  bool applyRangeWin, bool dopplerWinTaper = true);
     void run(int iterations = 1);

@@ -199,6 +199,7 @@ public:
         proc_->setRangeWindow(k, p0, p1);
     }
     void setLaunch(int ppl, int splits) { proc_->setLaunch(ppl, splits); }
+    void setDeviceBudget(unsigned long long bytes) { proc_->setDeviceBudget(bytes); }
     void setLogging(bool on)
     {
         if (on) proc_->setLoggerSinks({std::make_shared<tdclog::stdout_sink>()});
@@ -406,6 +407,7 @@ PYBIND11_MODULE(_sarlabtdc, m)
              py::arg("p0") = 0.0, py::arg("p1") = 0.0)
         .def("setLaunch", &PyTdcProcessor::setLaunch, py::arg("pixelsPerLane"),
              py::arg("chirpSplits") = 0)
+        .def("setDeviceBudget", &PyTdcProcessor::setDeviceBudget, py::arg("bytes"))
         .def("setLogging", &PyTdcProcessor::setLogging, py::arg("enabled"))
         .def("prepare", &PyTdcProcessor::prepare, py::arg("dopplerWinCenter"),
              py::arg("dopplerBandwidth"), py::arg("dopCentroidWin"), py::arg("applyRangeWin"),

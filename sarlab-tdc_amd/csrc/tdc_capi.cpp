@@ -214,10 +214,15 @@ struct tdc_processor {
     int win_kind = TDC_WIN_HAMMING;
     double win_p0 = 25.0 / 46.0, win_p1 = 0.0; // tuning.h:31
     int force_ppl = 0, force_split = 0;
+    uint64_t budget = 0; // device bytes for one image's buffers (0: no limit)
 
     // device buffers
     DevBuf d_raw, d_pos, d_vel, d_att, d_kt, d_wr, d_rec, d_grid, d_part, d_img, d_cpos,
         d_kanchor, d_counters, d_atab, d_ascale, d_chirp, d_ptab, d_pcorr, d_ptw, d_pcpos;
+    // the two buffer sets of chirp slabs streamed through the device, and the
+    // FP64 image the slabs add into
+    DevBuf r_raw[2], r_rec[2], r_cp[2], r_tab[2], r_scale[2], d_acc;
+    hipEvent_t r_done[2] = {nullptr, nullptr};
     // host-to-device copies of the inputs (declared after the buffers it
     // fills, so it is destroyed, and its copies drained, before they are freed)
     tdc::Ingest ingest;
@@ -536,14 +541,26 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     return TDC_OK;
 }
 
-int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool centroid,
-                      bool apply_rw, bool taper)
+// The precision form's host side: input checks, the anchor and window
+// tables, the chirp records, the engine and its parameters (h->dk, h->pp,
+// h->ppl). Nothing touches the device.
+struct PrecisionInputs {
+    std::vector<double> kanchor;
+    std::vector<tdc::ChirpPos> cp; // FP64 chirp positions
+    std::vector<tdc::ChirpPos> pc; // range profile: positions from the scene origin, |.|^2 in pad
+    std::vector<float> wr;
+    const double *gridp = nullptr;
+    int engine = 0;
+    int plen = 0, mp = 0;
+    std::shared_ptr<const tdc::ProfileTables> tabs;
+};
+
+int precision_inputs(tdc_processor *h, float dop_center, float dop_bw, bool centroid,
+                     bool apply_rw, bool taper, PrecisionInputs &in)
 {
     const int npri = h->npri, ns = h->ns, npx = h->rows * h->cols;
-    const size_t nterm = static_cast<size_t>(npri) * ns;
     std::vector<double> t(ns), cpos(static_cast<size_t>(npri) * 3);
     double f0, krate;
-    const double *gridp = nullptr;
     if (h->have_raw64) {
         if (!h->grid64 && !h->grid)
             return fail(TDC_ERR_STATE, "setFocusGrid must be called before start()");
@@ -586,91 +603,139 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
         krate = h->mod_rate;
     }
     if (h->grid64) {
-        gridp = h->grid64;
+        in.gridp = h->grid64;
     } else {
         std::vector<double> &grid64 = h->grid64_host;
         grid64.resize(static_cast<size_t>(npx) * 3);
         tdc::parallel_for(grid64.size(), size_t(1) << 16, [&](size_t i0, size_t i1) {
             for (size_t i = i0; i < i1; ++i) grid64[i] = h->grid[i];
         });
-        gridp = grid64.data();
+        in.gridp = grid64.data();
     }
-    std::vector<double> kanchor;
     std::string why;
-    if (!tdc::precision_tables(t.data(), ns, f0, krate, kanchor, h->dk, why))
+    if (!tdc::precision_tables(t.data(), ns, f0, krate, in.kanchor, h->dk, why))
         return fail(TDC_ERR_UNSUPPORTED, why);
-    std::vector<tdc::ChirpPos> cp(npri);
-    for (int n = 0; n < npri; ++n) cp[n] = tdc::ChirpPos{cpos[3 * n], cpos[3 * n + 1], cpos[3 * n + 2], 0.0};
-    std::vector<float> wr;
-    range_window(h->win_kind, h->win_p0, h->win_p1, ns, apply_rw, wr);
+    in.cp.resize(npri);
+    for (int n = 0; n < npri; ++n)
+        in.cp[n] = tdc::ChirpPos{cpos[3 * n], cpos[3 * n + 1], cpos[3 * n + 2], 0.0};
+    range_window(h->win_kind, h->win_p0, h->win_p1, ns, apply_rw, in.wr);
 
-    hipStream_t s = h->stream;
-    int rc;
-    tdc::Range uploads("tdc_upload_inputs");
-    TDC_HIP(h->ingest.begin(s));
-    if ((rc = upload(h, h->d_raw, h->raw, nterm * 2)) != TDC_OK) return rc;
-    if ((rc = upload(h, h->d_wr, wr.data(), ns)) != TDC_OK) return rc;
-    if ((rc = upload(h, h->d_cpos, cp.data(), cp.size())) != TDC_OK) return rc;
-    if ((rc = upload(h, h->d_kanchor, kanchor.data(), kanchor.size())) != TDC_OK) return rc;
-    if ((rc = place_grid(h, gridp, npx)) != TDC_OK) return rc;
-    const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
-    TDC_HIP(h->d_rec.ensure(esz * nterm));
-    h->doppler = false;
     int engine = h->engine;
-    const int plen = tdc::profile_length(ns);
+    in.plen = tdc::profile_length(ns);
     if (engine == TDC_ENGINE_AUTO) {
         if (h->force_ppl > 1) engine = TDC_ENGINE_VECTOR;
-        else engine = plen <= tdc::kProfileMaxM ? TDC_ENGINE_PROFILE : TDC_ENGINE_MATRIX;
+        else engine = in.plen <= tdc::kProfileMaxM ? TDC_ENGINE_PROFILE : TDC_ENGINE_MATRIX;
     }
+    in.engine = engine;
     if (engine == TDC_ENGINE_PROFILE) {
         if (h->force_ppl > 1)
             return fail(TDC_ERR_INVALID_ARGUMENT, "the range-profile engine runs one pixel per lane");
-        if (plen > tdc::kProfileMaxM)
+        if (in.plen > tdc::kProfileMaxM)
             return fail(TDC_ERR_UNSUPPORTED,
                         "the range-profile engine takes at most " +
                             std::to_string(tdc::kProfileMaxM / 2) + " samples per chirp");
-        const auto tabs = tdc::profile_tables(ns, plen);
-        if ((rc = upload(h, h->d_pcorr, tabs->corr.data(), tabs->corr.size())) != TDC_OK) return rc;
-        if ((rc = upload(h, h->d_ptw, tabs->twiddle.data(), tabs->twiddle.size())) != TDC_OK)
-            return rc;
-        const int mp = plen + tdc::kProfilePad;
-        TDC_HIP(h->d_ptab.ensure(sizeof(float2) * static_cast<size_t>(npri) * mp));
+        in.tabs = tdc::profile_tables(ns, in.plen);
+        in.mp = in.plen + tdc::kProfilePad;
         // chirp positions from an origin inside the scene (the midpoint of the
         // first and last pixels), with |pos - o|^2 in the pad
+        const double *gridp = in.gridp;
         const size_t last = 3 * (static_cast<size_t>(npx) - 1);
         double o[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < 3; ++a) {
             o[a] = npx > 0 ? 0.5 * (gridp[a] + gridp[last + a]) : 0.0;
             if (!std::isfinite(o[a])) o[a] = 0.0;
         }
-        std::vector<tdc::ChirpPos> pc(npri);
+        in.pc.resize(npri);
         for (int n = 0; n < npri; ++n) {
-            const double x = cp[n].x - o[0], y = cp[n].y - o[1], z = cp[n].z - o[2];
-            pc[n] = tdc::ChirpPos{x, y, z, std::fma(z, z, std::fma(y, y, x * x))};
+            const tdc::ChirpPos &c = in.cp[n];
+            const double x = c.x - o[0], y = c.y - o[1], z = c.z - o[2];
+            in.pc[n] = tdc::ChirpPos{x, y, z, std::fma(z, z, std::fma(y, y, x * x))};
         }
-        if ((rc = upload(h, h->d_pcpos, pc.data(), pc.size())) != TDC_OK) return rc;
         h->pp.ox = o[0];
         h->pp.oy = o[1];
         h->pp.oz = o[2];
         const double dt = ns > 1 ? (t[ns - 1] - t[0]) / (ns - 1) : 0.0;
         h->pp.dk = h->dk;
         h->pp.kh = 2.0 * (f0 + krate * (t[0] + (ns / 2) * dt)) / kSpeedOfLight;
-        h->pp.M = plen;
-        h->pp.Mp = mp;
+        h->pp.M = in.plen;
+        h->pp.Mp = in.mp;
         h->ppl = 1;
     } else if (engine == TDC_ENGINE_MATRIX) {
         if (h->force_ppl > 1)
             return fail(TDC_ERR_INVALID_ARGUMENT, "the matrix engine runs one pixel per lane");
-        TDC_HIP(h->d_atab.ensure(tdc::matrix_table_bytes(npri, ns, h->storage)));
-        TDC_HIP(h->d_ascale.ensure(sizeof(float) * static_cast<size_t>(npri)));
         h->ppl = 1;
     } else {
         h->ppl = h->force_ppl ? h->force_ppl : 2;
     }
-    TDC_HIP(h->ingest.fence(s));
-    h->run_engine = engine;
+    return TDC_OK;
+}
+
+// Device bytes of one chirp of the precision form (raw, staged raw, chirp
+// record and the engine's per-chirp table) and of the buffers per pixel
+// (grid, image and one partial image per split).
+size_t precision_chirp_bytes(const tdc_processor *h, const PrecisionInputs &in)
+{
+    const size_t ns = static_cast<size_t>(h->ns);
+    const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
+    size_t b = 8 * ns + esz * ns + sizeof(tdc::ChirpPos);
+    if (in.engine == TDC_ENGINE_PROFILE)
+        b += sizeof(float2) * static_cast<size_t>(in.mp) + sizeof(tdc::ChirpPos);
+    else if (in.engine == TDC_ENGINE_MATRIX)
+        b += tdc::matrix_table_bytes(1, h->ns, h->storage) + sizeof(float);
+    return b;
+}
+
+size_t precision_pixel_bytes(int nsplit) { return 24 + 8 + sizeof(double2) * static_cast<size_t>(nsplit); }
+
+int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool centroid,
+                      bool apply_rw, bool taper, const PrecisionInputs *given)
+{
+    const int npri = h->npri, ns = h->ns, npx = h->rows * h->cols;
+    const size_t nterm = static_cast<size_t>(npri) * ns;
+    PrecisionInputs computed;
+    int rc = TDC_OK;
+    if (!given) {
+        rc = precision_inputs(h, dop_center, dop_bw, centroid, apply_rw, taper, computed);
+        if (rc != TDC_OK) return rc;
+    }
+    const PrecisionInputs &in = given ? *given : computed;
     const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
     const int nsplit = choose_split(ntiles, npri, h->force_split);
+    if (h->budget > 0) {
+        const double need = double(precision_chirp_bytes(h, in)) * npri +
+                            double(precision_pixel_bytes(nsplit)) * npx;
+        if (need > double(h->budget))
+            return fail(TDC_ERR_UNSUPPORTED,
+                        "the image needs " + std::to_string(static_cast<long long>(need)) +
+                            " device bytes, more than the device budget of " +
+                            std::to_string(static_cast<unsigned long long>(h->budget)) +
+                            ": tdc_start streams such an aperture in chirp slabs, tdc_prepare "
+                            "keeps it resident");
+    }
+    hipStream_t s = h->stream;
+    tdc::Range uploads("tdc_upload_inputs");
+    TDC_HIP(h->ingest.begin(s));
+    if ((rc = upload(h, h->d_raw, h->raw, nterm * 2)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_wr, in.wr.data(), ns)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_cpos, in.cp.data(), in.cp.size())) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_kanchor, in.kanchor.data(), in.kanchor.size())) != TDC_OK) return rc;
+    if ((rc = place_grid(h, in.gridp, npx)) != TDC_OK) return rc;
+    const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
+    TDC_HIP(h->d_rec.ensure(esz * nterm));
+    h->doppler = false;
+    if (in.engine == TDC_ENGINE_PROFILE) {
+        if ((rc = upload(h, h->d_pcorr, in.tabs->corr.data(), in.tabs->corr.size())) != TDC_OK)
+            return rc;
+        if ((rc = upload(h, h->d_ptw, in.tabs->twiddle.data(), in.tabs->twiddle.size())) != TDC_OK)
+            return rc;
+        TDC_HIP(h->d_ptab.ensure(sizeof(float2) * static_cast<size_t>(npri) * in.mp));
+        if ((rc = upload(h, h->d_pcpos, in.pc.data(), in.pc.size())) != TDC_OK) return rc;
+    } else if (in.engine == TDC_ENGINE_MATRIX) {
+        TDC_HIP(h->d_atab.ensure(tdc::matrix_table_bytes(npri, ns, h->storage)));
+        TDC_HIP(h->d_ascale.ensure(sizeof(float) * static_cast<size_t>(npri)));
+    }
+    TDC_HIP(h->ingest.fence(s));
+    h->run_engine = in.engine;
     make_geom(h->geom, npx, npri, ns, h->ppl, nsplit);
     TDC_HIP(h->d_img.ensure(sizeof(float2) * static_cast<size_t>(npx)));
     TDC_HIP(h->d_part.ensure(sizeof(double2) * static_cast<size_t>(npx) * nsplit));
@@ -839,6 +904,8 @@ int tdc_destroy(tdc_processor *h)
     for (auto &t : h->ev)
         for (hipEvent_t x : t)
             if (x) (void) hipEventDestroy(x);
+    for (hipEvent_t x : h->r_done)
+        if (x) (void) hipEventDestroy(x);
     if (h->stream) (void) hipStreamDestroy(h->stream);
     delete h;
     return ok();
@@ -1005,6 +1072,14 @@ int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int
     return ok();
 }
 
+int tdc_set_device_budget(tdc_processor *h, unsigned long long bytes)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    h->budget = bytes;
+    h->prepared = false;
+    return ok();
+}
+
 int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
@@ -1020,9 +1095,10 @@ int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits)
 }
 
 namespace {
+struct PrecisionInputs;
 int prepare_inputs(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
                    int dop_centroid_win, int apply_range_win, int doppler_win_taper,
-                   bool defer_grid);
+                   bool defer_grid, const PrecisionInputs *given = nullptr);
 } // namespace
 
 int tdc_prepare(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
@@ -1191,9 +1267,169 @@ int run_streamed(tdc_processor *h)
     return finish_stats(h, a, b, wgs);
 }
 
+// Slab plan of an aperture larger than the device budget: `sps` chirp splits
+// of `cps` chirps per slab, `nsplit` splits in all, with cps = ceil(npri /
+// nsplit) so that a resident pass with nsplit splits cuts the same splits.
+// Each of the two buffer sets holds one slab; the grid, the image, the FP64
+// sum and one partial image per split of a slab are held once.
+struct RingPlan {
+    int sps = 0, cps = 0, nsplit = 0;
+};
+
+int ring_plan(const tdc_processor *h, const PrecisionInputs &in, RingPlan &plan)
+{
+    const int npri = h->npri, npx = h->rows * h->cols;
+    const double per_chirp = double(precision_chirp_bytes(h, in));
+    const double tables = 8.0 * h->ns + 8.0 * in.kanchor.size() +
+                          (in.tabs ? 4.0 * in.tabs->corr.size() + 8.0 * in.tabs->twiddle.size() : 0.0);
+    const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
+    const int nsplit_full = choose_split(ntiles, npri, 0);
+    const int cps_full = (npri + nsplit_full - 1) / nsplit_full;
+    for (int sps = 16; sps >= 1; sps /= 2) {
+        // grid, image, FP64 sum and the slab's partial images
+        const double fixed = (24.0 + 8.0 + 16.0 + 16.0 * sps) * npx + tables;
+        const double chirps = (double(h->budget) - fixed) / (2.0 * per_chirp);
+        int nsplit;
+        if (h->force_split > 0) {
+            nsplit = h->force_split;
+        } else {
+            const double cps_fit = std::floor(chirps / sps);
+            if (cps_fit < 1.0) continue;
+            const int cps_target = static_cast<int>(std::min<double>(cps_full, cps_fit));
+            nsplit = (npri + cps_target - 1) / cps_target;
+        }
+        const int cps = (npri + nsplit - 1) / nsplit;
+        if (double(cps) * sps > chirps) continue;
+        plan.sps = sps;
+        plan.cps = cps;
+        plan.nsplit = nsplit;
+        return TDC_OK;
+    }
+    return fail(TDC_ERR_UNSUPPORTED,
+                "the device budget of " + std::to_string(static_cast<unsigned long long>(h->budget)) +
+                    " bytes holds no slab of chirps of this image (two slabs of one split each, "
+                    "with the grid, the image and its partial sums)");
+}
+
+// tdc_start of the precision form over an aperture larger than the device
+// budget. The chirps pass through the device in slabs of whole chirp splits,
+// alternating between two buffer sets: the copy stream fills one set while
+// the compute stream stages and forms the slab held in the other, and waits
+// for the set's previous slab to be formed before refilling it. Each slab's
+// partial images are added in split order into an FP64 image that the last
+// slab rounds into the image, so the result is bit for bit that of a
+// resident pass with the same number of splits.
+int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
+{
+    tdc::Range range("tdc_run_slabs");
+    const int npri = h->npri, ns = h->ns, npx = h->rows * h->cols;
+    const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
+    const int slab = plan.sps * plan.cps;
+    hipStream_t s = h->stream;
+    int rc = ensure_events(h, 1);
+    if (rc != TDC_OK) return rc;
+    TDC_HIP(hipStreamSynchronize(s));
+    // the budget holds the slabs: release the buffers of a resident aperture
+    for (DevBuf *b : {&h->d_raw, &h->d_rec, &h->d_cpos, &h->d_pcpos, &h->d_ptab, &h->d_atab,
+                      &h->d_ascale, &h->d_part})
+        b->release();
+    for (int k = 0; k < 2; ++k) {
+        TDC_HIP(h->r_raw[k].ensure(8 * static_cast<size_t>(ns) * slab));
+        TDC_HIP(h->r_rec[k].ensure(esz * static_cast<size_t>(ns) * slab));
+        TDC_HIP(h->r_cp[k].ensure(sizeof(tdc::ChirpPos) * static_cast<size_t>(slab)));
+        if (in.engine == TDC_ENGINE_PROFILE) {
+            TDC_HIP(h->r_tab[k].ensure(sizeof(float2) * static_cast<size_t>(in.mp) * slab));
+        } else if (in.engine == TDC_ENGINE_MATRIX) {
+            TDC_HIP(h->r_tab[k].ensure(tdc::matrix_table_bytes(slab, ns, h->storage)));
+            TDC_HIP(h->r_scale[k].ensure(sizeof(float) * static_cast<size_t>(slab)));
+        }
+        if (!h->r_done[k]) TDC_HIP(hipEventCreateWithFlags(&h->r_done[k], hipEventDisableTiming));
+    }
+    TDC_HIP(h->d_part.ensure(sizeof(double2) * static_cast<size_t>(npx) * plan.sps));
+    TDC_HIP(h->d_acc.ensure(sizeof(double2) * static_cast<size_t>(npx)));
+    TDC_HIP(h->d_img.ensure(sizeof(float2) * static_cast<size_t>(npx)));
+    h->run_form = TDC_FORM_PRECISION;
+    h->run_storage = h->storage;
+    h->run_engine = in.engine;
+    h->doppler = false;
+    make_geom(h->geom, npx, npri, ns, h->ppl, plan.nsplit);
+
+    const std::array<hipEvent_t, 4> &evt = h->ev[0];
+    {
+        tdc::Range uploads("tdc_upload_inputs");
+        TDC_HIP(h->ingest.begin(s));
+        if ((rc = upload(h, h->d_wr, in.wr.data(), ns)) != TDC_OK) return rc;
+        if ((rc = upload(h, h->d_kanchor, in.kanchor.data(), in.kanchor.size())) != TDC_OK)
+            return rc;
+        if ((rc = upload(h, h->d_grid, in.gridp, 3 * static_cast<size_t>(npx))) != TDC_OK)
+            return rc;
+        if (in.engine == TDC_ENGINE_PROFILE) {
+            if ((rc = upload(h, h->d_pcorr, in.tabs->corr.data(), in.tabs->corr.size())) != TDC_OK)
+                return rc;
+            if ((rc = upload(h, h->d_ptw, in.tabs->twiddle.data(), in.tabs->twiddle.size())) !=
+                TDC_OK)
+                return rc;
+        }
+    }
+    TDC_HIP(hipEventRecord(evt[0], s));
+    TDC_HIP(hipEventRecord(evt[1], s));
+    const std::vector<tdc::ChirpPos> &recs = in.engine == TDC_ENGINE_PROFILE ? in.pc : in.cp;
+    int wgs = 0;
+    for (int k = 0, n0 = 0; n0 < npri; ++k, n0 += slab) {
+        const int set = k & 1;
+        const int nk = std::min(slab, npri - n0);
+        if (k >= 2) TDC_HIP(h->ingest.wait(h->r_done[set]));
+        TDC_HIP(h->ingest.put(h->r_raw[set].p, h->raw + 2 * static_cast<size_t>(n0) * ns,
+                              8 * static_cast<size_t>(nk) * ns));
+        TDC_HIP(h->ingest.put(h->r_cp[set].p, recs.data() + n0, sizeof(tdc::ChirpPos) * nk));
+        TDC_HIP(h->ingest.fence(s));
+        TDC_HIP(tdc::launch_stage_precision(h->storage, h->r_raw[set].as<float>(),
+                                            h->d_wr.as<float>(), nk, ns, h->r_rec[set].p, s));
+        tdc::Geom gs{};
+        make_geom(gs, npx, nk, ns, h->ppl, (nk + plan.cps - 1) / plan.cps);
+        gs.cps = plan.cps;
+        int w = 0;
+        if (in.engine == TDC_ENGINE_PROFILE) {
+            TDC_HIP(tdc::launch_stage_profile(h->storage, h->r_rec[set].p, h->d_pcorr.as<float>(),
+                                              h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp,
+                                              h->r_tab[set].as<float2>(), s));
+            TDC_HIP(tdc::launch_profile(h->r_tab[set].as<float2>(),
+                                        h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),
+                                        h->pp, gs, h->d_part.as<double2>(), s, &w));
+        } else if (in.engine == TDC_ENGINE_MATRIX) {
+            const int parts = tdc::matrix_parts(h->storage);
+            TDC_HIP(tdc::launch_stage_matrix(h->storage, h->r_rec[set].p, nk, ns, parts,
+                                             h->r_tab[set].p, h->r_scale[set].as<float>(), s));
+            TDC_HIP(tdc::launch_precision_matrix(
+                parts, h->r_tab[set].p, h->r_scale[set].as<float>(),
+                h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),
+                h->d_kanchor.as<double>(), h->dk, gs, h->d_part.as<double2>(), s, &w));
+        } else {
+            TDC_HIP(tdc::launch_precision(h->ppl, h->storage, h->r_rec[set].p,
+                                          h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),
+                                          h->d_kanchor.as<double>(), h->dk, gs,
+                                          h->d_part.as<double2>(), s, &w));
+        }
+        wgs += w;
+        TDC_HIP(hipEventRecord(h->r_done[set], s));
+        const bool last = n0 + slab >= npri;
+        TDC_HIP(tdc::launch_accumulate_f64(h->d_part.as<double2>(), gs.nsplit, npx, k == 0 ? 1 : 0,
+                                           h->d_acc.as<double2>(),
+                                           last ? h->d_img.as<float2>() : nullptr, s));
+    }
+    TDC_HIP(hipEventRecord(evt[2], s));
+    TDC_HIP(hipEventRecord(evt[3], s));
+    TDC_HIP(hipEventSynchronize(evt[3]));
+    float a = 0.0F, b = 0.0F;
+    TDC_HIP(hipEventElapsedTime(&a, evt[1], evt[2]));
+    TDC_HIP(hipEventElapsedTime(&b, evt[0], evt[3]));
+    h->trivial_processed = static_cast<int64_t>(npx) * ((npri + tdc::kChunk - 1) / tdc::kChunk);
+    return finish_stats(h, a, b, wgs);
+}
+
 int prepare_inputs(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
                    int dop_centroid_win, int apply_range_win, int doppler_win_taper,
-                   bool defer_grid)
+                   bool defer_grid, const PrecisionInputs *given)
 {
     if (!h->have_raw32 && !h->have_raw64)
         return fail(TDC_ERR_STATE, "setRawData must be called before start()");
@@ -1203,7 +1439,7 @@ int prepare_inputs(tdc_processor *h, float doppler_win_center, float doppler_ban
     h->defer_grid = defer_grid;
     if (h->form == TDC_FORM_PRECISION || h->have_raw64) {
         rc = prepare_precision(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
-                               apply_range_win != 0, doppler_win_taper != 0);
+                               apply_range_win != 0, doppler_win_taper != 0, given);
     } else {
         rc = prepare_plain(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
                            apply_range_win != 0, doppler_win_taper != 0);
@@ -1266,13 +1502,11 @@ int tdc_run(tdc_processor *h, int iterations)
     return ok();
 }
 
-int tdc_fetch_image(tdc_processor *h)
+namespace {
+
+// The device image into the processor's page-locked image buffer (one DMA).
+int fetch_image_now(tdc_processor *h)
 {
-    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
-    tdc::Range range("tdc_fetch_image");
-    if (!h->prepared) return fail(TDC_ERR_STATE, "nothing has been run");
-    int rc = ensure_device(h);
-    if (rc != TDC_OK) return rc;
     const size_t npx = static_cast<size_t>(h->rows) * h->cols;
     h->has_image = false;
     h->image_px = npx;
@@ -1283,6 +1517,19 @@ int tdc_fetch_image(tdc_processor *h)
         TDC_HIP(hipStreamSynchronize(h->stream));
     }
     h->has_image = true;
+    return TDC_OK;
+}
+
+} // namespace
+
+int tdc_fetch_image(tdc_processor *h)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    tdc::Range range("tdc_fetch_image");
+    if (!h->prepared) return fail(TDC_ERR_STATE, "nothing has been run");
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    if ((rc = fetch_image_now(h)) != TDC_OK) return rc;
     return ok();
 }
 
@@ -1292,11 +1539,38 @@ int tdc_start(tdc_processor *h, float doppler_win_center, float doppler_bandwidt
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     tdc::Range range("tdc_start");
     const auto t0 = std::chrono::steady_clock::now();
-    int rc;
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    // with a device budget, a precision-form aperture that exceeds it streams
+    // through the device in chirp slabs
+    PrecisionInputs pin;
+    bool have_pin = false;
+    if (h->budget > 0 && (h->have_raw64 || (h->have_raw32 && h->form == TDC_FORM_PRECISION))) {
+        tdc::Range prep("tdc_prepare");
+        h->prepared = false;
+        rc = precision_inputs(h, doppler_win_center, doppler_bandwidth, dop_centroid_win != 0,
+                              apply_range_win != 0, doppler_win_taper != 0, pin);
+        if (rc != TDC_OK) return rc;
+        have_pin = true;
+        const int npx = h->rows * h->cols;
+        const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
+        const int nsplit = choose_split(ntiles, h->npri, h->force_split);
+        const double need = double(precision_chirp_bytes(h, pin)) * h->npri +
+                            double(precision_pixel_bytes(nsplit)) * npx;
+        if (need > double(h->budget) && npx > 0) {
+            RingPlan plan;
+            if ((rc = ring_plan(h, pin, plan)) != TDC_OK) return rc;
+            if ((rc = run_ring(h, pin, plan)) != TDC_OK) return rc;
+            if ((rc = fetch_image_now(h)) != TDC_OK) return rc;
+            const auto t1 = std::chrono::steady_clock::now();
+            h->stats.end_to_end_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            return ok();
+        }
+    }
     {
         tdc::Range prep("tdc_prepare");
         rc = prepare_inputs(h, doppler_win_center, doppler_bandwidth, dop_centroid_win,
-                            apply_range_win, doppler_win_taper, true);
+                            apply_range_win, doppler_win_taper, true, have_pin ? &pin : nullptr);
     }
     if (rc == TDC_OK) {
         if (h->geom.npx > 0) {

@@ -349,6 +349,13 @@ hipError_t Ingest::put(void *dst, const void *src, size_t bytes)
     return hipSuccess;
 }
 
+hipError_t Ingest::wait(hipEvent_t done)
+{
+    hipError_t e = init();
+    if (e != hipSuccess) return e;
+    return hipStreamWaitEvent(copy_, done, 0);
+}
+
 hipError_t Ingest::fence(hipStream_t compute)
 {
     hipError_t e = init();

@@ -177,6 +177,10 @@ inline size_t matrix_table_bytes(int npri, int ns, int storage)
 inline int anchor_entries(int ns) { return 4 * matrix_quads(ns); }
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
+// acc = (first ? 0 : acc) + the nsplit partial images in split order; with img
+// non-null the sum is rounded into img instead of stored in acc.
+hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, int first,
+                                 double2 *acc, float2 *img, hipStream_t s);
 // Range-profile tables [npri][Mp] from the staged raw samples (in `storage`
 // format, range window folded in); corr[i] = 1 / psi_hat((i - h) / M);
 // twiddle[k] = exp(+j 2 pi k / M), k < M / 2.
@@ -250,6 +254,8 @@ class Ingest {
     hipError_t begin(hipStream_t compute);
     hipError_t put(void *dst, const void *src, size_t bytes);
     hipError_t fence(hipStream_t compute);
+    // later copies wait for `done` (recorded on another stream)
+    hipError_t wait(hipEvent_t done);
 
   private:
     static constexpr size_t kSlotBytes = size_t(4) << 20;

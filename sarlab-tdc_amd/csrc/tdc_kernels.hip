@@ -1105,6 +1105,40 @@ hipError_t launch_precision_matrix(int parts, const void *atab, const float *inv
     return hipGetLastError();
 }
 
+// Slabs of an aperture streamed through the device: acc = (first ? 0 : acc)
+// plus the slab's partial images in split order, the order finalize_f64_kernel
+// adds a resident pass's splits in; the last slab rounds the sum into img.
+__global__ __launch_bounds__(kBlock) void accumulate_f64_kernel(const double2 *__restrict__ partial,
+                                                                int nsplit, int npx, int first,
+                                                                double2 *__restrict__ acc,
+                                                                float2 *__restrict__ img)
+{
+    const int p = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+    if (p >= npx) return;
+    double sr = 0.0, si = 0.0;
+    if (!first) {
+        const double2 a = acc[p];
+        sr = a.x;
+        si = a.y;
+    }
+    for (int s = 0; s < nsplit; ++s) {
+        const double2 v = partial[static_cast<size_t>(s) * npx + p];
+        sr += v.x;
+        si += v.y;
+    }
+    if (img) img[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+    else acc[p] = make_double2(sr, si);
+}
+
+hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, int first,
+                                 double2 *acc, float2 *img, hipStream_t s)
+{
+    if (npx <= 0) return hipSuccess;
+    hipLaunchKernelGGL(accumulate_f64_kernel, dim3((npx + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       partial, nsplit, npx, first, acc, img);
+    return hipGetLastError();
+}
+
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s)
 {

@@ -123,6 +123,10 @@ void TdcProcessor::setLaunch(int pixelsPerLane, int chirpSplits)
 {
     check(tdc_set_launch(h_, pixelsPerLane, chirpSplits));
 }
+void TdcProcessor::setDeviceBudget(unsigned long long bytes)
+{
+    check(tdc_set_device_budget(h_, bytes));
+}
 void TdcProcessor::prepare(float dopplerWinCenter, float dopplerBandwidth, bool dopCentroidWin,
                            bool applyRangeWin, bool dopplerWinTaper)
 {

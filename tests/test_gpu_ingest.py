@@ -166,3 +166,80 @@ def test_start_streams_forced_launch_shapes(tdc, jitter, kind, ppl, splits):
     assert sa["chirp_splits"] == splits
     assert_streams(sa)
     assert_same(a, b)
+
+
+# ---------------------------------------------------------------- device budget
+
+@pytest.fixture(scope="module")
+def long_aperture():
+    """2048 chirps x 512 samples on a 64 x 64 grid: ~34 MB of per-chirp data
+    (raw, staged raw, range-profile tables) against ~1 MB per pixel buffer."""
+    from sarlabtdc import scene
+
+    rng = np.random.default_rng(77)
+    npri, ns = 2048, 512
+    track = np.zeros((npri, 3))
+    track[:, 0] = np.linspace(-0.4, 0.4, npri)
+    track += rng.normal(scale=2e-4, size=track.shape)
+    grid = scene.plane_grid(64, 64, 3e-3, (0.0, 1.0, 0.0))
+    scat = scene._uniform_on_grid(rng, grid, 40)
+    return scene.synthesize("long", track, ns, grid, scat, scene._cn(rng, 40, 1.0), 25.0, rng)
+
+
+BUDGET = 12 << 20
+
+
+@pytest.mark.parametrize("engine,storage", [("profile", "c64"), ("profile", "f16"),
+                                            ("matrix", "c64"), ("vector", "c64")])
+def test_budget_streams_chirp_slabs(tdc, orc, long_aperture, engine, storage):
+    """An aperture larger than the device budget: start() streams chirp slabs
+    through the device and forms, bit for bit, the image of a resident pass
+    with the same chirp splits; the image passes the oracle."""
+    s = long_aperture
+    f = s.f64()
+    p = processor(tdc, f=f, kind="precision", storage=storage, engine=engine)
+    p.setDeviceBudget(BUDGET)
+    a, sa = streamed(p, WIN_OFF)
+    nsplit = sa["chirp_splits"]
+    print(engine, storage, "splits", nsplit, "engine", sa["engine"])
+    assert sa["engine"] == engine
+    # the per-chirp data alone exceed the budget: it cannot have been resident
+    per_chirp = 8 * 512 + (8 if storage == "c64" else 4) * 512
+    assert per_chirp * 2048 > BUDGET or engine != "vector"
+    # prepare() keeps an aperture resident and refuses this one
+    with pytest.raises(Exception, match="device budget"):
+        p.prepare(*WIN_OFF)
+    q = processor(tdc, f=f, kind="precision", storage=storage, engine=engine)
+    q.setLaunch(0, nsplit)
+    b, sb = resident(q, WIN_OFF)
+    assert sb["chirp_splits"] == nsplit
+    assert_same(a, b)
+    px = np.unique(np.random.default_rng(5).choice(64 * 64, 300, replace=False))
+    ref = orc.image_f64(s.raw, s.sample_times, s.track, s.grid, s.mod_rate, s.f0, pixels=px)
+    e = orc.rel_l2(a.reshape(-1)[px], ref)
+    assert e <= (1e-4 if storage == "c64" else 3e-3), e
+
+
+def test_budget_too_small_and_plain_form(tdc, long_aperture):
+    """A budget below the per-pixel buffers (grid, image, FP64 sum and one
+    partial image: 64 B a pixel, 256 KiB here) holds no slab and fails with a
+    message; 1 MiB already streams slabs of 8 chirps. The plain form keeps its
+    aperture resident whatever the budget."""
+    s = long_aperture
+    p = processor(tdc, f=s.f64(), kind="precision")
+    p.setDeviceBudget(200 << 10)
+    with pytest.raises(Exception, match="holds no slab"):
+        p.start(*WIN_OFF)
+    p.setDeviceBudget(1 << 20)
+    a, sa = streamed(p, WIN_OFF)
+    q = processor(tdc, f=s.f64(), kind="precision")
+    q.setLaunch(0, sa["chirp_splits"])
+    b, _ = resident(q, WIN_OFF)
+    assert_same(a, b)
+    d = s.dropin()
+    q = processor(tdc, d=d, kind="plain")
+    q.setDeviceBudget(1 << 20)
+    a, _ = streamed(q, WIN_OFF)
+    r = processor(tdc, d=d, kind="plain")
+    b, _ = resident(r, WIN_OFF)
+    assert_same(a, b)
