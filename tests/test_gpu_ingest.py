@@ -243,3 +243,17 @@ def test_budget_too_small_and_plain_form(tdc, long_aperture):
     r = processor(tdc, d=d, kind="plain")
     b, _ = resident(r, WIN_OFF)
     assert_same(a, b)
+
+
+def test_budget_streams_fp32_api(tdc, long_aperture):
+    """The reference's FP32 inputs in the precision form under a budget: the
+    chirp slabs form the resident pass's image bit for bit."""
+    d = long_aperture.dropin()
+    p = processor(tdc, d=d, kind="precision")
+    p.setDeviceBudget(BUDGET)
+    a, sa = streamed(p, WIN_OFF)
+    assert sa["form"] == "precision" and sa["engine"] == "profile"
+    q = processor(tdc, d=d, kind="precision")
+    q.setLaunch(0, sa["chirp_splits"])
+    b, _ = resident(q, WIN_OFF)
+    assert_same(a, b)
