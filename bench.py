@@ -468,7 +468,11 @@ def main():
                                   fp64_flop_per_pixel_chirp=PROFILE_FP64_PER_PAIR,
                                   fp32_equivalent_per_pixel_chirp=fp32eq,
                                   tflops=round(achieved, 3),
-                                  frac_of_fp32_peak=round(achieved / PEAK_FP32_TFLOPS, 4)),
+                                  frac_of_fp32_peak=round(achieved / PEAK_FP32_TFLOPS, 4),
+                                  # packed FP32 instructions are not used, so the
+                                  # VALU's attainable FP32 rate is the non-packed peak
+                                  frac_of_nonpacked_fp32_peak=round(
+                                      achieved / PEAK_FP32_NONPACKED, 4)),
                     simd_cycles=simd)
     elif head["engine"] == "matrix":
         # FP16 products on the matrix cores; the VALU part (phasors, nested
