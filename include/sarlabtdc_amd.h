@@ -80,8 +80,9 @@ typedef struct tdc_processor tdc_processor;
 #define TDC_PER_CHIRP_WINDOW 2
 
 typedef struct tdc_stats {
-    double kernel_ms;       /* device time of one pass of the last run: staging, correlate-sum and
-                               split reduction kernels (HIP events) */
+    double kernel_ms;       /* device time of the last run divided by its passes, each pass the
+                               staging, correlate-sum and split reduction kernels, back to back
+                               (HIP events) */
     double main_kernel_ms;  /* device time of the correlate-sum kernel alone (after tdc_start:
                                of its slabs, with the waits for their parts of the grid) */
     double end_to_end_ms;   /* host wall time of the last tdc_start (H2D + compute + D2H) */

@@ -178,8 +178,9 @@ PosKey key_of(float x, float y, float z)
 struct tdc_processor {
     int device = 0;
     hipStream_t stream = nullptr;
-    // per-iteration (start, staging end, main kernel end, step end) events of
-    // tdc_run: the iterations are enqueued back to back and timed after one
+    // timing events: ev[0] spans a whole run (and a streamed start); the
+    // rest, in slots 1 and 2, bracket each correlate-sum launch of tdc_run,
+    // whose iterations are enqueued back to back and read after one
     // synchronisation
     std::vector<std::array<hipEvent_t, 4>> ev;
 
@@ -1463,41 +1464,44 @@ int tdc_run(tdc_processor *h, int iterations)
     if (rc != TDC_OK) return rc;
     const tdc::Geom &g = h->geom;
     hipStream_t s = h->stream;
-    double main_ms = 0.0, total_ms = 0.0;
+    double main_ms = 0.0;
     int wgs = 0;
+    // events around each correlate-sum launch (a ring, read back every
+    // kEventRing iterations) and one pair around the whole run
     constexpr int kEventRing = 64;
-    if ((rc = ensure_events(h, std::min(iterations, kEventRing))) != TDC_OK) return rc;
+    if ((rc = ensure_events(h, std::min(iterations, kEventRing) + 1)) != TDC_OK) return rc;
+    const std::array<hipEvent_t, 4> &span = h->ev[0];
     int pending = 0;
     auto drain = [&]() -> int {
         if (pending == 0) return TDC_OK;
-        TDC_HIP(hipEventSynchronize(h->ev[pending - 1][3]));
-        for (int k = 0; k < pending; ++k) {
-            float a = 0.0F, b = 0.0F;
+        TDC_HIP(hipEventSynchronize(h->ev[pending][2]));
+        for (int k = 1; k <= pending; ++k) {
+            float a = 0.0F;
             TDC_HIP(hipEventElapsedTime(&a, h->ev[k][1], h->ev[k][2]));
-            TDC_HIP(hipEventElapsedTime(&b, h->ev[k][0], h->ev[k][3]));
             main_ms += a;
-            total_ms += b;
         }
         pending = 0;
         return TDC_OK;
     };
     const int ring = std::min(iterations, kEventRing);
+    TDC_HIP(hipEventRecord(span[0], s));
     for (int it = 0; it < iterations; ++it) {
         if (g.npx == 0) break;
-        const std::array<hipEvent_t, 4> &evt = h->ev[pending];
+        const std::array<hipEvent_t, 4> &evt = h->ev[pending + 1];
         if (h->doppler) TDC_HIP(hipMemsetAsync(h->d_counters.p, 0, 2 * sizeof(unsigned long long), s));
-        TDC_HIP(hipEventRecord(evt[0], s));
         if ((rc = stage_inputs(h)) != TDC_OK) return rc;
         TDC_HIP(hipEventRecord(evt[1], s));
         if ((rc = launch_main(h, g, 0, &wgs)) != TDC_OK) return rc;
         TDC_HIP(hipEventRecord(evt[2], s));
         if ((rc = launch_reduce(h)) != TDC_OK) return rc;
-        TDC_HIP(hipEventRecord(evt[3], s));
-        if (++pending == ring && (rc = drain()) != TDC_OK) return rc;
+        if (++pending == ring && it + 1 < iterations && (rc = drain()) != TDC_OK) return rc;
     }
+    TDC_HIP(hipEventRecord(span[3], s));
     if ((rc = drain()) != TDC_OK) return rc;
-    TDC_HIP(hipStreamSynchronize(s));
-    if ((rc = finish_stats(h, main_ms / iterations, total_ms / iterations, wgs)) != TDC_OK)
+    TDC_HIP(hipEventSynchronize(span[3]));
+    float total = 0.0F;
+    TDC_HIP(hipEventElapsedTime(&total, span[0], span[3]));
+    if ((rc = finish_stats(h, main_ms / iterations, double(total) / iterations, wgs)) != TDC_OK)
         return rc;
     return ok();
 }
