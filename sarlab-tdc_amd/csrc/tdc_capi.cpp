@@ -426,13 +426,16 @@ int stage_inputs(tdc_processor *h)
         }
         return TDC_OK;
     }
+    if (h->run_engine == TDC_ENGINE_PROFILE) {
+        // window and storage rounding folded into the per-chirp transform
+        TDC_HIP(tdc::launch_stage_profile_folded(
+            h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
+            h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp, h->d_ptab.as<float2>(), s));
+        return TDC_OK;
+    }
     TDC_HIP(tdc::launch_stage_precision(h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(),
                                         npri, ns, h->d_rec.p, s));
-    if (h->run_engine == TDC_ENGINE_PROFILE) {
-        TDC_HIP(tdc::launch_stage_profile(h->run_storage, h->d_rec.p, h->d_pcorr.as<float>(),
-                                          h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp,
-                                          h->d_ptab.as<float2>(), s));
-    } else if (h->run_engine == TDC_ENGINE_MATRIX) {
+    if (h->run_engine == TDC_ENGINE_MATRIX) {
         TDC_HIP(tdc::launch_stage_matrix(h->run_storage, h->d_rec.p, npri, ns,
                                          tdc::matrix_parts(h->run_storage), h->d_atab.p,
                                          h->d_ascale.as<float>(), s));
@@ -671,14 +674,16 @@ int precision_inputs(tdc_processor *h, float dop_center, float dop_bw, bool cent
     return TDC_OK;
 }
 
-// Device bytes of one chirp of the precision form (raw, staged raw, chirp
-// record and the engine's per-chirp table) and of the buffers per pixel
-// (grid, image and one partial image per split).
+// Device bytes of one chirp of the precision form (raw, staged raw but for
+// the range-profile engine, which stages from the raw input, chirp record and
+// the engine's per-chirp table) and of the buffers per pixel (grid, image and
+// one partial image per split).
 size_t precision_chirp_bytes(const tdc_processor *h, const PrecisionInputs &in)
 {
     const size_t ns = static_cast<size_t>(h->ns);
     const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
-    size_t b = 8 * ns + esz * ns + sizeof(tdc::ChirpPos);
+    size_t b = 8 * ns + sizeof(tdc::ChirpPos);
+    if (in.engine != TDC_ENGINE_PROFILE) b += esz * ns;
     if (in.engine == TDC_ENGINE_PROFILE)
         b += sizeof(float2) * static_cast<size_t>(in.mp) + sizeof(tdc::ChirpPos);
     else if (in.engine == TDC_ENGINE_MATRIX)
@@ -722,7 +727,8 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     if ((rc = upload(h, h->d_kanchor, in.kanchor.data(), in.kanchor.size())) != TDC_OK) return rc;
     if ((rc = place_grid(h, in.gridp, npx)) != TDC_OK) return rc;
     const size_t esz = h->storage == TDC_RAW_C64 ? 8 : 4;
-    TDC_HIP(h->d_rec.ensure(esz * nterm));
+    // the range-profile engine stages straight from the raw input
+    if (in.engine != TDC_ENGINE_PROFILE) TDC_HIP(h->d_rec.ensure(esz * nterm));
     h->doppler = false;
     if (in.engine == TDC_ENGINE_PROFILE) {
         if ((rc = upload(h, h->d_pcorr, in.tabs->corr.data(), in.tabs->corr.size())) != TDC_OK)
@@ -1336,7 +1342,8 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
         b->release();
     for (int k = 0; k < 2; ++k) {
         TDC_HIP(h->r_raw[k].ensure(8 * static_cast<size_t>(ns) * slab));
-        TDC_HIP(h->r_rec[k].ensure(esz * static_cast<size_t>(ns) * slab));
+        if (in.engine != TDC_ENGINE_PROFILE)
+            TDC_HIP(h->r_rec[k].ensure(esz * static_cast<size_t>(ns) * slab));
         TDC_HIP(h->r_cp[k].ensure(sizeof(tdc::ChirpPos) * static_cast<size_t>(slab)));
         if (in.engine == TDC_ENGINE_PROFILE) {
             TDC_HIP(h->r_tab[k].ensure(sizeof(float2) * static_cast<size_t>(in.mp) * slab));
@@ -1384,16 +1391,17 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
                               8 * static_cast<size_t>(nk) * ns));
         TDC_HIP(h->ingest.put(h->r_cp[set].p, recs.data() + n0, sizeof(tdc::ChirpPos) * nk));
         TDC_HIP(h->ingest.fence(s));
-        TDC_HIP(tdc::launch_stage_precision(h->storage, h->r_raw[set].as<float>(),
-                                            h->d_wr.as<float>(), nk, ns, h->r_rec[set].p, s));
+        if (in.engine != TDC_ENGINE_PROFILE)
+            TDC_HIP(tdc::launch_stage_precision(h->storage, h->r_raw[set].as<float>(),
+                                                h->d_wr.as<float>(), nk, ns, h->r_rec[set].p, s));
         tdc::Geom gs{};
         make_geom(gs, npx, nk, ns, h->ppl, (nk + plan.cps - 1) / plan.cps);
         gs.cps = plan.cps;
         int w = 0;
         if (in.engine == TDC_ENGINE_PROFILE) {
-            TDC_HIP(tdc::launch_stage_profile(h->storage, h->r_rec[set].p, h->d_pcorr.as<float>(),
-                                              h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp,
-                                              h->r_tab[set].as<float2>(), s));
+            TDC_HIP(tdc::launch_stage_profile_folded(
+                h->storage, h->r_raw[set].as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
+                h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp, h->r_tab[set].as<float2>(), s));
             TDC_HIP(tdc::launch_profile(h->r_tab[set].as<float2>(),
                                         h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),
                                         h->pp, gs, h->d_part.as<double2>(), s, &w));

@@ -187,6 +187,12 @@ hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, in
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp,
                                 float2 *table, hipStream_t s);
+// The same tables from the complex64 input: each sample times the range
+// window wr[i], rounded as `storage` stores it (bit-identical to staging it
+// with launch_stage_precision first).
+hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
+                                       const float *corr, const float2 *twiddle, int npri,
+                                       int ns, int M, int Mp, float2 *table, hipStream_t s);
 // `cpos`: chirp positions minus the origin (pp.ox, oy, oz), with |pos - o|^2 in pad.
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
                           const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,

@@ -485,6 +485,25 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
 // Range-profile engine: the per-chirp tables and the interpolation kernel.
 // ---------------------------------------------------------------------------
 
+// Sample i of chirp n for the per-chirp transform: from the staged raw in the
+// storage format, or (FOLD) from the complex64 input with the range window
+// folded in and the storage format's rounding applied, the same values.
+template <class RAW, bool FOLD>
+__device__ __forceinline__ void profile_sample(const void *src, const float *wr, int n, int ns,
+                                               int i, float &re, float &im)
+{
+    const size_t e = static_cast<size_t>(n) * ns + i;
+    if constexpr (FOLD) {
+        const float2 v = static_cast<const float2 *>(src)[e];
+        re = v.x * wr[i];
+        im = v.y * wr[i];
+        RAW::round(re, im);
+    } else {
+        RAW::get(as_const(static_cast<const typename RAW::T *>(src) + static_cast<size_t>(n) * ns),
+                 i, re, im);
+    }
+}
+
 // One workgroup per chirp: b[(i - h) mod M] = c_i * corr[i] (zero elsewhere),
 // then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-2 Stockham passes
 // between two LDS buffers (the output comes out in natural order), written
@@ -492,9 +511,9 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
 // kProfilePad entries: a pixel whose bin position has integer part f reads
 // its taps B[f - kLead .. f - kLead + 11] as row[f .. f + 11], which never
 // wrap.
-template <class RAW, int BLOCK>
+template <class RAW, int BLOCK, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
-    const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
+    const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
     const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
 {
     extern __shared__ float2 buf[]; // 2 M entries, sized at launch
@@ -506,10 +525,9 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     float2 *y = buf + M;
     for (int j = tid; j < M; j += BLOCK) x[j] = make_float2(0.0F, 0.0F);
     __syncthreads();
-    const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
     for (int i = tid; i < ns; i += BLOCK) {
         float re, im;
-        RAW::get(row, i, re, im);
+        profile_sample<RAW, FOLD>(src, wr, n, ns, i, re, im);
         const float c = corr[i];
         x[(i - h) & mask] = make_float2(re * c, im * c);
     }
@@ -538,9 +556,9 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
 // LDS (M above kProfileStockhamMaxM): in place in one buffer, radix-2
 // decimation in time from a bit-reversed placement of b, then written like
 // the Stockham kernel's rows.
-template <class RAW, int BLOCK>
+template <class RAW, int BLOCK, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
-    const typename RAW::T *__restrict__ raw, const float *__restrict__ corr,
+    const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
     const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
 {
     extern __shared__ float2 x[]; // M entries, sized at launch
@@ -551,10 +569,9 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
     const int bits = __builtin_ctz(static_cast<unsigned>(M));
     for (int j = tid; j < M; j += BLOCK) x[j] = make_float2(0.0F, 0.0F);
     __syncthreads();
-    const cptr<typename RAW::T> row = as_const(raw + static_cast<size_t>(n) * ns);
     for (int i = tid; i < ns; i += BLOCK) {
         float re, im;
-        RAW::get(row, i, re, im);
+        profile_sample<RAW, FOLD>(src, wr, n, ns, i, re, im);
         const float c = corr[i];
         const unsigned j = static_cast<unsigned>((i - h) & mask);
         x[__builtin_bitreverse32(j) >> (32 - bits)] = make_float2(re * c, im * c);
@@ -928,41 +945,63 @@ hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
 
 namespace {
 
-template <class RAW, int BLOCK>
-hipError_t stage_profile_launch(const void *staged, const float *corr, const float2 *twiddle,
-                                int npri, int ns, int M, int Mp, float2 *table, hipStream_t s)
+template <class RAW, int BLOCK, bool FOLD>
+hipError_t stage_profile_launch(const void *src, const float *wr, const float *corr,
+                                const float2 *twiddle, int npri, int ns, int M, int Mp,
+                                float2 *table, hipStream_t s)
 {
     const bool inplace = M > kProfileStockhamMaxM;
     const size_t lds = (inplace ? 1 : 2) * sizeof(float2) * static_cast<size_t>(M);
     const void *kernel =
-        inplace ? reinterpret_cast<const void *>(&stage_profile_inplace_kernel<RAW, BLOCK>)
-                : reinterpret_cast<const void *>(&stage_profile_kernel<RAW, BLOCK>);
+        inplace ? reinterpret_cast<const void *>(&stage_profile_inplace_kernel<RAW, BLOCK, FOLD>)
+                : reinterpret_cast<const void *>(&stage_profile_kernel<RAW, BLOCK, FOLD>);
     // more than 64 KiB of dynamic LDS is granted to a kernel on request
     if (lds > (size_t(64) << 10)) {
         const hipError_t e = hipFuncSetAttribute(
             kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    const auto *src = static_cast<const typename RAW::T *>(staged);
     if (inplace)
-        hipLaunchKernelGGL((stage_profile_inplace_kernel<RAW, BLOCK>), dim3(npri), dim3(BLOCK), lds,
-                           s, src, corr, twiddle, ns, M, Mp, table);
+        hipLaunchKernelGGL((stage_profile_inplace_kernel<RAW, BLOCK, FOLD>), dim3(npri),
+                           dim3(BLOCK), lds, s, src, wr, corr, twiddle, ns, M, Mp, table);
     else
-        hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK>), dim3(npri), dim3(BLOCK), lds, s, src,
-                           corr, twiddle, ns, M, Mp, table);
+        hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD>), dim3(npri), dim3(BLOCK), lds,
+                           s, src, wr, corr, twiddle, ns, M, Mp, table);
     return hipGetLastError();
 }
 
 // One workgroup per chirp. Long transforms take most of a CU's LDS, so only
 // one workgroup fits per CU: they run 1024 threads (16 waves) to keep the
 // LDS busy across the passes' barriers; short ones run 256.
-template <class RAW>
-hipError_t stage_profile_raw(const void *staged, const float *corr, const float2 *twiddle,
-                             int npri, int ns, int M, int Mp, float2 *table, hipStream_t s)
+template <class RAW, bool FOLD>
+hipError_t stage_profile_raw(const void *src, const float *wr, const float *corr,
+                             const float2 *twiddle, int npri, int ns, int M, int Mp,
+                             float2 *table, hipStream_t s)
 {
     if (M >= kProfileWideM)
-        return stage_profile_launch<RAW, 1024>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
-    return stage_profile_launch<RAW, kBlock>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
+        return stage_profile_launch<RAW, 1024, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp,
+                                                     table, s);
+    return stage_profile_launch<RAW, kBlock, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table,
+                                                   s);
+}
+
+template <bool FOLD>
+hipError_t stage_profile_any(int storage, const void *src, const float *wr, const float *corr,
+                             const float2 *twiddle, int npri, int ns, int M, int Mp,
+                             float2 *table, hipStream_t s)
+{
+    if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || Mp < M + kProfileTaps - 1)
+        return hipErrorInvalidValue;
+    switch (storage) {
+    case 0:
+        return stage_profile_raw<RawC64, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table, s);
+    case 1:
+        return stage_profile_raw<RawF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table, s);
+    case 2:
+        return stage_profile_raw<RawBF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table, s);
+    default:
+        return hipErrorInvalidValue;
+    }
 }
 
 } // namespace
@@ -971,18 +1010,15 @@ hipError_t launch_stage_profile(int storage, const void *staged, const float *co
                                 const float2 *twiddle, int npri, int ns, int M, int Mp,
                                 float2 *table, hipStream_t s)
 {
-    if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || Mp < M + kProfileTaps - 1)
-        return hipErrorInvalidValue;
-    switch (storage) {
-    case 0:
-        return stage_profile_raw<RawC64>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
-    case 1:
-        return stage_profile_raw<RawF16>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
-    case 2:
-        return stage_profile_raw<RawBF16>(staged, corr, twiddle, npri, ns, M, Mp, table, s);
-    default:
-        return hipErrorInvalidValue;
-    }
+    return stage_profile_any<false>(storage, staged, nullptr, corr, twiddle, npri, ns, M, Mp, table,
+                                    s);
+}
+
+hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
+                                       const float *corr, const float2 *twiddle, int npri,
+                                       int ns, int M, int Mp, float2 *table, hipStream_t s)
+{
+    return stage_profile_any<true>(storage, raw, wr, corr, twiddle, npri, ns, M, Mp, table, s);
 }
 
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,

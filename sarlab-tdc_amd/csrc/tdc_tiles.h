@@ -94,6 +94,8 @@ __device__ __forceinline__ void two_sum_acc(float &acc, float &comp, float x)
 }
 
 // Raw sample loaders for the three storage formats; the address is uniform.
+// round() gives an FP32 value the rounding of storage in the format, so that
+// staging kernels can take raw samples straight from complex64 input.
 struct RawC64 {
     using T = float2;
     static __device__ __forceinline__ void get(cptr<T> p, int i, float &re, float &im)
@@ -102,6 +104,7 @@ struct RawC64 {
         re = v.x;
         im = v.y;
     }
+    static __device__ __forceinline__ void round(float &, float &) {}
 };
 struct RawF16 {
     using T = unsigned int;
@@ -115,6 +118,11 @@ struct RawF16 {
         // FMA as v_fma_mix_f32, which measured 40% slower)
         asm volatile("" : "+v"(re), "+v"(im));
     }
+    static __device__ __forceinline__ void round(float &re, float &im)
+    {
+        re = static_cast<float>(static_cast<_Float16>(re));
+        im = static_cast<float>(static_cast<_Float16>(im));
+    }
 };
 struct RawBF16 {
     using T = unsigned int;
@@ -123,6 +131,15 @@ struct RawBF16 {
         const unsigned int v = p[i];
         re = __uint_as_float(v << 16);
         im = __uint_as_float(v & 0xffff0000U);
+    }
+    // round to nearest even (finite inputs)
+    static __device__ __forceinline__ void round(float &re, float &im)
+    {
+        unsigned int br = __float_as_uint(re), bi = __float_as_uint(im);
+        br += 0x7fffU + ((br >> 16) & 1U);
+        bi += 0x7fffU + ((bi >> 16) & 1U);
+        re = __uint_as_float(br & 0xffff0000U);
+        im = __uint_as_float(bi & 0xffff0000U);
     }
 };
 
