@@ -56,8 +56,10 @@ struct StepTable {
 };
 
 // `counter`: two counters, one per tile family.
+// `matrix`: also launch the matrix-engine family's kernel (a step with no
+// such tiles leaves it out).
 hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
-                             int workgroups, hipStream_t s);
+                             int workgroups, bool matrix, hipStream_t s);
 hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,
                                  hipStream_t s);
 hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,

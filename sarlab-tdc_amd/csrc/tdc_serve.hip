@@ -93,6 +93,9 @@ __global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const ServeJob *
         if (j.form == 0) {
             const float2 *part = static_cast<const float2 *>(j.partial);
             float sr = 0.0F, si = 0.0F;
+            // unrolled so that the loads of several splits are in flight at
+            // once; the adds stay in split order
+#pragma unroll 8
             for (int s = 0; s < j.nsplit; ++s) {
                 const float2 v = part[static_cast<size_t>(s) * j.npx + p];
                 sr += v.x;
@@ -102,6 +105,7 @@ __global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const ServeJob *
         } else {
             const double2 *part = static_cast<const double2 *>(j.partial);
             double sr = 0.0, si = 0.0;
+#pragma unroll 8
             for (int s = 0; s < j.nsplit; ++s) {
                 const double2 v = part[static_cast<size_t>(s) * j.npx + p];
                 sr += v.x;
@@ -136,11 +140,13 @@ __global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(
 } // namespace
 
 hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
-                             int workgroups, hipStream_t s)
+                             int workgroups, bool matrix, hipStream_t s)
 {
     hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(1), 0, s, counter);
     hipLaunchKernelGGL(serve_kernel<0>, dim3(workgroups), dim3(kBlock), 0, s, jobs, table, counter);
-    hipLaunchKernelGGL(serve_kernel<1>, dim3(workgroups), dim3(kBlock), 0, s, jobs, table, counter);
+    if (matrix)
+        hipLaunchKernelGGL(serve_kernel<1>, dim3(workgroups), dim3(kBlock), 0, s, jobs, table,
+                           counter);
     return hipGetLastError();
 }
 

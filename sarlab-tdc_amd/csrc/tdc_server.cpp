@@ -13,8 +13,8 @@
 // resident jobs -- an equal share of the step's work for every job first,
 // then the rest in arrival order -- plus the finalize blocks of the jobs
 // whose last tiles are in it. Steps alternate between two compute streams,
-// each replaying the step graph of its grid-size bucket (counter reset + two
-// persistent correlate-sum kernels, one per tile family), so the next step's
+// each replaying the step graph of its grid-size bucket (counter reset + the
+// persistent correlate-sum kernel of each tile family it holds), so the next step's
 // workgroups fill the CUs while the previous step drains. A job's finalize
 // follows the step holding its last tiles on that step's stream, after the
 // step before it (on the other stream) has ended too; that covers all of its
@@ -256,7 +256,8 @@ struct tdc_server {
     tdc::StepTable *h_table[kDepth] = {};   // pinned step tables
     tdc::StepTable *dev_table[kDepth] = {}; // their device addresses
     unsigned int *d_counter = nullptr; // kDepth counters, 256 bytes apart
-    hipGraphExec_t graph[kNumBuckets][kDepth] = {};
+    // [with the matrix-engine family][grid-size bucket][step table]
+    hipGraphExec_t graph[2][kNumBuckets][kDepth] = {};
     int ngraphs = 0;
     // staging for SDMA uploads, and coherent (uncached by the GPU) memory for
     // the images the finalize writes
@@ -546,20 +547,20 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
         reinterpret_cast<float4 *>(j.arena + j.off_cpos32), ss);
 }
 
-bool ensure_graph(tdc_server *s, int bucket, int buf, std::string &err)
+bool ensure_graph(tdc_server *s, int matrix, int bucket, int buf, std::string &err)
 {
-    if (s->graph[bucket][buf]) return true;
+    if (s->graph[matrix][bucket][buf]) return true;
     hipStream_t cs = nullptr;
     SRV_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         e = tdc::launch_serve_step(s->dev_jobs, s->dev_table[buf], s->d_counter + 64 * buf,
-                                   kBuckets[bucket], cs);
+                                   kBuckets[bucket], matrix != 0, cs);
         hipError_t e2 = hipStreamEndCapture(cs, &g);
         if (e == hipSuccess) e = e2;
     }
-    if (e == hipSuccess) e = hipGraphInstantiate(&s->graph[bucket][buf], g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphInstantiate(&s->graph[matrix][bucket][buf], g, nullptr, nullptr, 0);
     if (g) (void) hipGraphDestroy(g);
     (void) hipStreamDestroy(cs);
     if (e != hipSuccess) {
@@ -791,7 +792,10 @@ void scheduler(tdc_server *s)
                 break;
             }
         }
-        if (!ensure_graph(s, bucket, st.buf, err)) return die(err);
+        // a step without matrix-engine tiles replays the graph without their
+        // (otherwise empty) persistent kernel
+        const int matrix = mtiles.empty() ? 0 : 1;
+        if (!ensure_graph(s, matrix, bucket, st.buf, err)) return die(err);
         tdc::Range range("tdc_server_launch_step");
         const int p = st.buf;
         const int prev = (p + kDepth - 1) % kDepth; // the step before, on the other stream
@@ -804,7 +808,7 @@ void scheduler(tdc_server *s)
             if (e == hipSuccess) e = stage_job(*j, ss);
         if (e == hipSuccess) e = hipEventRecord(s->ev_staged[p], ss);
         if (e == hipSuccess) e = hipEventRecord(s->ev_start[p], ss);
-        if (e == hipSuccess) e = hipGraphLaunch(s->graph[bucket][p], ss);
+        if (e == hipSuccess) e = hipGraphLaunch(s->graph[matrix][bucket][p], ss);
         if (e == hipSuccess) e = hipEventRecord(s->ev_end[p], ss);
         // finalize on the same stream, right behind this step, once the previous
         // step (the other stream) has ended too: the two cover every earlier
@@ -1000,9 +1004,10 @@ int tdc_server_destroy(tdc_server *s)
         if (t.joinable()) t.join();
     (void) hipSetDevice(s->device);
     (void) hipDeviceSynchronize();
-    for (auto &row : s->graph)
-        for (auto &g : row)
-            if (g) (void) hipGraphExecDestroy(g);
+    for (auto &fam : s->graph)
+        for (auto &row : fam)
+            for (auto &g : row)
+                if (g) (void) hipGraphExecDestroy(g);
     for (int i = 0; i < kDepth; ++i) {
         if (s->ev_start[i]) (void) hipEventDestroy(s->ev_start[i]);
         if (s->ev_end[i]) (void) hipEventDestroy(s->ev_end[i]);
