@@ -216,9 +216,10 @@ int tdc_set_device_budget(tdc_processor *h, unsigned long long bytes);
  * submitted at any time; the scheduler admits them in arrival order while
  * their device footprint fits the HBM budget, keeps their inputs, partial and
  * final images resident, and advances all admitted jobs together in short
- * steps (continuous batching): each step is one persistent correlate-sum
- * kernel draining a table of (job, pixel tile, chirp range) tiles, replayed
- * as a hipGraph per grid-size bucket. A job's tile decomposition and
+ * steps (continuous batching): each step is a persistent correlate-sum
+ * kernel per tile family it holds, draining a table of (job, pixel tile,
+ * chirp range) tiles, replayed as a hipGraph per grid-size bucket and set of
+ * families (`graphs` counts those captured). A job's tile decomposition and
  * reduction order depend only on its own shape, so its image is the same
  * bit for bit alone or in any mix. */
 
