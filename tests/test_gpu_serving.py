@@ -93,6 +93,34 @@ def test_job_is_bit_identical_alone_and_in_a_mix(tdc):
     assert st["completed"] == 6 and st["failed"] == 0
 
 
+@pytest.mark.parametrize("nc,ns,side,tiles", [(2048, 128, 16, 128), (512, 256, 64, 512),
+                                              (600, 128, 256, 8192)])
+def test_jobs_with_few_pixel_tiles_are_cut_into_short_chirp_ranges(tdc, orc, nc, ns, side, tiles):
+    """A job with few 256-pixel tiles is cut into chirp ranges short enough
+    for 8192 tiles, down to 16 chirps: 16^2 pixels x 2048 chirps gives one
+    pixel tile x 128 ranges of 16 chirps, 64^2 x 512 gives 16 x 32, and
+    256^2 x 600 gives 256 x 32 ranges of 19 chirps. The image sums the many
+    partial images within the precision form's tolerance and is the same bit
+    for bit alone and beside other jobs."""
+    s = job_scene(21, nc, ns, side)
+    srv = tdc.Server(0, 16e9)
+    alone, info = srv.wait(submit(srv, s))
+    srv.close()
+    assert info["tiles"] == tiles, info
+    px = subset(alone.size, 600, extra=np.argsort(np.abs(alone.reshape(-1)))[-8:])
+    e = orc.rel_l2(alone.reshape(-1)[px], oracle_img(orc, s, px))
+    print(nc, ns, side, "tiles", info["tiles"], "rel-L2", e)
+    assert e < 1e-4
+    srv = tdc.Server(0, 16e9)
+    ids = [submit(srv, job_scene(200 + k, *shape)) for k, shape in
+           enumerate([(700, 512, 200), (64, 128, 32)])]
+    mixed, _ = srv.wait(submit(srv, s))
+    for i in ids:
+        srv.wait(i)
+    srv.close()
+    assert np.array_equal(alone, mixed)
+
+
 def test_plain_job_bit_identical_in_a_mix(tdc):
     target = job_scene(8, 260, 128, 60)
     srv = tdc.Server(0, 8e9)
