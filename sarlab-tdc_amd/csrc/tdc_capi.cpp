@@ -423,6 +423,10 @@ int stage_inputs(tdc_processor *h)
             TDC_HIP(tdc::launch_stage_plain(h->d_raw.as<float>(), h->d_pos.as<float>(),
                                             h->d_kt.as<float>(), h->d_wr.as<float>(), npri, ns,
                                             h->d_rec.as<tdc::PlainRec>(), s));
+            // the position of each chirp's first sample, the origin of the records' offsets
+            TDC_HIP(tdc::launch_stage_chirps(h->d_pos.as<float>(), nullptr, nullptr, npri,
+                                             h->ant_stride, 0.0F, nullptr, 0, 0.0F,
+                                             h->d_chirp.as<float4>(), nullptr, s));
         }
         return TDC_OK;
     }
@@ -451,12 +455,14 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     if (!h->grid) return fail(TDC_ERR_STATE, "setFocusGrid must be called before start()");
     const int npri = h->npri, ns = h->ns, npx = h->rows * h->cols;
     const size_t nterm = static_cast<size_t>(npri) * ns;
-    std::vector<float> wr, kt(ns);
+    std::vector<float> wr, dk;
+    std::vector<double> k(ns), kanchor;
     range_window(h->win_kind, h->win_p0, h->win_p1, ns, apply_rw, wr);
     for (int m = 0; m < ns; ++m) {
         const double f = double(h->start_freq) + double(h->mod_rate) * double(h->sample_times[m]);
-        kt[m] = static_cast<float>(2.0 * f / kSpeedOfLight);
+        k[m] = 2.0 * f / kSpeedOfLight;
     }
+    tdc::plain_phase_tables(k, dk, kanchor);
     tdc::AntennaScan scan;
     {
         tdc::Range checks("tdc_check_inputs");
@@ -474,7 +480,8 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     tdc::Range uploads("tdc_upload_inputs");
     TDC_HIP(h->ingest.begin(s));
     if ((rc = upload(h, h->d_raw, h->raw, nterm * 2)) != TDC_OK) return rc;
-    if ((rc = upload(h, h->d_kt, kt.data(), ns)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_kt, dk.data(), ns)) != TDC_OK) return rc;
+    if ((rc = upload(h, h->d_kanchor, kanchor.data(), kanchor.size())) != TDC_OK) return rc;
     if ((rc = upload(h, h->d_wr, wr.data(), ns)) != TDC_OK) return rc;
     if ((rc = place_grid(h, h->grid, npx)) != TDC_OK) return rc;
     if (h->chirp_path) {
@@ -527,6 +534,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
         TDC_HIP(h->d_chirp.ensure(sizeof(float4) * static_cast<size_t>(npri)));
     } else {
         TDC_HIP(h->d_rec.ensure(sizeof(tdc::PlainRec) * nterm));
+        TDC_HIP(h->d_chirp.ensure(sizeof(float4) * static_cast<size_t>(npri)));
     }
     // measured best on 1024 chirps x 512 samples x 512^2 pixels: two pixels per
     // lane for the per-term plain records, one for the per-chirp range
@@ -864,6 +872,26 @@ bool precision_tables(const double *t, int ns, double f0, double krate,
     return true;
 }
 
+void plain_phase_tables(const std::vector<double> &k, std::vector<float> &dk,
+                        std::vector<double> &kanchor)
+{
+    const int ns = static_cast<int>(k.size());
+    const int nblk = (ns + kAnchorBlock - 1) / kAnchorBlock;
+    dk.resize(ns);
+    kanchor.resize(nblk);
+    for (int b = 0; b < nblk; ++b) {
+        const int m0 = b * kAnchorBlock, m1 = std::min(ns, m0 + kAnchorBlock);
+        double lo = k[m0], hi = k[m0];
+        for (int m = m0; m < m1; ++m) {
+            lo = std::min(lo, k[m]);
+            hi = std::max(hi, k[m]);
+        }
+        const double kb = 0.5 * (lo + hi);
+        kanchor[b] = std::isfinite(kb) ? kb : 0.0;
+        for (int m = m0; m < m1; ++m) dk[m] = static_cast<float>(k[m] - kanchor[b]);
+    }
+}
+
 } // namespace tdc
 
 extern "C" {
@@ -958,8 +986,8 @@ int tdc_set_focus_grid(tdc_processor *h, const float *grid, int n_rows, int n_co
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     if (n_rows < 0 || n_cols < 0)
         return fail(TDC_ERR_INVALID_ARGUMENT, "grid dimensions must not be negative");
-    if (static_cast<int64_t>(n_rows) * n_cols > INT_MAX)
-        return fail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^31 - 1 pixels");
+    if (static_cast<int64_t>(n_rows) * n_cols > tdc::kMaxPixels)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^30 pixels");
     if (!grid && n_rows > 0 && n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
     h->grid = grid;
     h->grid64 = nullptr;
@@ -1068,8 +1096,8 @@ int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
     if (n_rows < 0 || n_cols < 0)
         return fail(TDC_ERR_INVALID_ARGUMENT, "grid dimensions must not be negative");
-    if (static_cast<int64_t>(n_rows) * n_cols > INT_MAX)
-        return fail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^31 - 1 pixels");
+    if (static_cast<int64_t>(n_rows) * n_cols > tdc::kMaxPixels)
+        return fail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^30 pixels");
     if (!grid && n_rows > 0 && n_cols > 0) return fail(TDC_ERR_INVALID_ARGUMENT, "grid is NULL");
     h->grid64 = grid;
     h->grid = nullptr;
@@ -1130,17 +1158,20 @@ int launch_main(tdc_processor *h, const tdc::Geom &g, size_t p0, int *wgs)
         float2 *dst = (g.nsplit > 1 ? h->d_part.as<float2>() : h->d_img.as<float2>()) + p0;
         const float *grid = h->d_grid.as<float>() + 3 * p0;
         const float4 *cpos = h->d_chirp.as<float4>();
+        const double *ka = h->d_kanchor.as<double>();
         if (h->doppler && h->chirp_path) {
             TDC_HIP(tdc::launch_doppler_chirp(h->ppl, h->d_rec.as<float4>(), cpos, cpos + g.npri,
-                                              grid, g, h->dp, dst,
+                                              grid, ka, g, h->dp, dst,
                                               h->d_counters.as<unsigned long long>(), s, wgs));
         } else if (h->doppler) {
-            TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(), grid, g, h->dp, dst,
-                                        h->d_counters.as<unsigned long long>(), s, wgs));
+            TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(), grid, ka, g, h->dp,
+                                        dst, h->d_counters.as<unsigned long long>(), s, wgs));
         } else if (h->chirp_path) {
-            TDC_HIP(tdc::launch_range(h->ppl, h->d_rec.as<float4>(), cpos, grid, g, dst, s, wgs));
+            TDC_HIP(tdc::launch_range(h->ppl, h->d_rec.as<float4>(), cpos, grid, ka, g, dst, s,
+                                      wgs));
         } else {
-            TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), grid, g, dst, s, wgs));
+            TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), cpos, grid, ka, g, dst,
+                                      s, wgs));
         }
         return TDC_OK;
     }

@@ -31,13 +31,17 @@ constexpr int kQuadSamples = 2 * kMatrixBlock;
 constexpr int kTileBytes = 64 * 16;
 
 
-// One (chirp, sample) term of the plain form, staged once per job.
-// Every field is wave-uniform in the correlate-sum kernel, so a record is
-// read with one scalar load (s_load_dwordx8) into SGPRs.
+// Largest focus grid the setters accept (pixel indices, and the padded tile
+// counts above them, stay inside 32-bit arithmetic).
+constexpr int64_t kMaxPixels = int64_t(1) << 30;
+
+// One (chirp, sample) term of the plain form with the antenna moving within
+// the chirp, staged once per job and read by the workgroup through LDS.
 struct alignas(32) PlainRec {
     float rr, ri; // raw sample times the range window
-    float x, y, z; // antenna phase centre
-    float k;      // 2 (f0 + K t_m) / c, cycles per metre of range
+    float x, y, z; // antenna phase centre minus that of the chirp's first sample
+    float k;      // dk_m = k_m - k_b, k_m = 2 (f0 + K t_m) / c in cycles per metre of range
+                  // and k_b the anchor wavenumber of the sample's 64-sample block
     float pad0, pad1;
 };
 
@@ -46,18 +50,20 @@ struct alignas(32) PlainRec {
 // 2 / lambda_c and the Doppler window centre of the term.
 struct alignas(16) DopRec {
     float rr, ri;
-    float x, y, z;
-    float k;
+    float x, y, z; // antenna phase centre (absolute: the skip test reads it)
+    float k;       // dk_m, as in PlainRec
     float vx, vy, vz; // (2 / lambda_c) * velocity
     float fc;         // Doppler window centre (centroid or fixed), Hz
     float pad0, pad1;
 };
 
 // Records of the plain form when the antenna position is constant within
-// every chirp (stop-and-hop): one float4 (rr, ri, k, 0) per term, the raw
-// sample times the range window and k_m as in PlainRec, plus one float4 per
+// every chirp (stop-and-hop): one float4 (rr, ri, dk_m, 0) per term, the raw
+// sample times the range window and dk_m as in PlainRec, plus one float4 per
 // chirp: its position (x, y, z, 0) and, with the Doppler window on, its
 // velocity pre-scaled by 2 / lambda_c and window centre (vx, vy, vz, fc).
+// Every plain kernel also reads the FP64 anchor wavenumbers k_b, one per
+// kAnchorBlock samples (plain_phase_tables).
 
 // Chirp phase centre of the precision form (FP64, padded to 32 bytes).
 struct alignas(32) ChirpPos {
@@ -121,34 +127,38 @@ inline int profile_length(int ns)
 }
 
 // Kernel launch wrappers (defined in tdc_kernels.hip). All enqueue on `s`.
-hipError_t launch_stage_plain(const float *raw, const float *pos, const float *kt, const float *wr,
+// `dk`: per-sample dk_m; `kanchor`: the FP64 anchor wavenumbers; `cpos`: the
+// position of each chirp's first sample (launch_stage_chirps).
+hipError_t launch_stage_plain(const float *raw, const float *pos, const float *dk, const float *wr,
                               int npri, int ns, PlainRec *out, hipStream_t s);
 hipError_t launch_stage_doppler(const float *raw, const float *pos, const float *vel,
-                                const float *att, const float *kt, const float *wr, int npri,
+                                const float *att, const float *dk, const float *wr, int npri,
                                 int ns, float two_over_lambda, const float *bore, int centroid,
                                 float fc_fixed, DopRec *out, hipStream_t s);
-hipError_t launch_plain(int ppl, const PlainRec *rec, const float *grid, const Geom &g,
-                        float2 *out, hipStream_t s, int *workgroups);
-hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const Geom &g,
-                          const DopParams &dp, float2 *out, unsigned long long *counters,
-                          hipStream_t s, int *workgroups);
-// Stop-and-hop records: terms (rr, ri, k, 0) and per chirp the position of
+hipError_t launch_plain(int ppl, const PlainRec *rec, const float4 *cpos, const float *grid,
+                        const double *kanchor, const Geom &g, float2 *out, hipStream_t s,
+                        int *workgroups);
+hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const double *kanchor,
+                          const Geom &g, const DopParams &dp, float2 *out,
+                          unsigned long long *counters, hipStream_t s, int *workgroups);
+// Stop-and-hop records: terms (rr, ri, dk_m, 0) and per chirp the position of
 // its first sample; with `vel` non-null also (2/lambda_c) v and the window
 // centre of its first sample (the values stage_doppler writes for it).
-hipError_t launch_stage_terms(const float *raw, const float *kt, const float *wr, int npri, int ns,
+hipError_t launch_stage_terms(const float *raw, const float *dk, const float *wr, int npri, int ns,
                               float4 *out, hipStream_t s);
 hipError_t launch_stage_chirps(const float *pos, const float *vel, const float *att, int npri,
                                int ns, float two_over_lambda, const float *bore, int centroid,
                                float fc_fixed, float4 *cpos, float4 *cvel, hipStream_t s);
 // Range evaluated once per (pixel, chirp): position constant within each chirp.
 hipError_t launch_range(int ppl, const float4 *rec, const float4 *cpos, const float *grid,
-                        const Geom &g, float2 *out, hipStream_t s, int *workgroups);
+                        const double *kanchor, const Geom &g, float2 *out, hipStream_t s,
+                        int *workgroups);
 // Range and Doppler window evaluated once per (pixel, chirp): position,
 // velocity and attitude constant within each chirp.
 hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
-                                const float4 *cvel, const float *grid, const Geom &g,
-                                const DopParams &dp, float2 *out, unsigned long long *counters,
-                                hipStream_t s, int *workgroups);
+                                const float4 *cvel, const float *grid, const double *kanchor,
+                                const Geom &g, const DopParams &dp, float2 *out,
+                                unsigned long long *counters, hipStream_t s, int *workgroups);
 hipError_t launch_finalize_f32(const float2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
 
@@ -225,6 +235,12 @@ struct ProfileTables {
 std::shared_ptr<const ProfileTables> profile_tables(int ns, int M);
 bool precision_tables(const double *t, int ns, double f0, double krate,
                       std::vector<double> &kanchor, double &dk, std::string &why);
+// Phase tables of the plain form from the per-sample wavenumbers
+// k[m] = 2 (f0 + K t_m) / c (FP64): per kAnchorBlock samples the anchor k_b,
+// the midpoint of the block's smallest and largest k (FP64), and per sample
+// dk[m] = k[m] - k_b rounded once to FP32.
+void plain_phase_tables(const std::vector<double> &k, std::vector<float> &dk,
+                        std::vector<double> &kanchor);
 
 // Host ingest (tdc_host.cpp). Host threads for passes over one image's inputs.
 constexpr unsigned kHostThreads = 8;

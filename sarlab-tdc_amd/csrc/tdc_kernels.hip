@@ -16,8 +16,9 @@
 //
 // Plain form (the drop-in): per term the direct reference phase
 //   theta = k_m * |pos - p|  (cycles, k_m = 2 (f0 + K t_m) / c)
-// with hardware sqrt / fract / sin / cos (inputs in revolutions), then a
-// complex multiply-accumulate of raw * exp(+j 2 pi theta) -- the conjugate
+// from an FP64 range and an FP64 anchor phase every 64 samples plus one FP32
+// FMA per term, with hardware fract / sin / cos (inputs in revolutions), then
+// a complex multiply-accumulate of raw * exp(+j 2 pi theta) -- the conjugate
 // product of tdckernels.cu:270-272 with ref = w exp(-j 4 pi f R / c).
 //
 // Precision form: per (pixel, chirp) the range R is computed in FP64; per
@@ -88,25 +89,27 @@ __device__ __forceinline__ float window_centre(float3 vv, float4 att, float3 bor
 }
 
 // ---------------------------------------------------------------------------
-// Staging: raw * range window, positions and k_m packed per term.
+// Staging: raw * range window, antenna offsets and dk_m packed per term.
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBlock) void stage_plain_kernel(
-    const float2 *__restrict__ raw, const float *__restrict__ pos, const float *__restrict__ kt,
+    const float2 *__restrict__ raw, const float *__restrict__ pos, const float *__restrict__ dk,
     const float *__restrict__ wr, int npri, int ns, PlainRec *__restrict__ out)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
     for (size_t e = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; e < total;
          e += static_cast<size_t>(gridDim.x) * kBlock) {
         const int m = static_cast<int>(e % ns);
+        const size_t e0 = e - static_cast<size_t>(m); // the chirp's first sample
         const float2 v = raw[e];
         PlainRec r;
         r.rr = v.x * wr[m];
         r.ri = v.y * wr[m];
-        r.x = pos[3 * e];
-        r.y = pos[3 * e + 1];
-        r.z = pos[3 * e + 2];
-        r.k = kt[m];
+        // offset from the chirp's first position (exactly zero on stop-and-hop input)
+        r.x = pos[3 * e] - pos[3 * e0];
+        r.y = pos[3 * e + 1] - pos[3 * e0 + 1];
+        r.z = pos[3 * e + 2] - pos[3 * e0 + 2];
+        r.k = dk[m];
         r.pad0 = 0.0F;
         r.pad1 = 0.0F;
         out[e] = r;
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void stage_plain_kernel(
 
 __global__ __launch_bounds__(kBlock) void stage_doppler_kernel(
     const float2 *__restrict__ raw, const float *__restrict__ pos, const float *__restrict__ vel,
-    const float4 *__restrict__ att, const float *__restrict__ kt, const float *__restrict__ wr,
+    const float4 *__restrict__ att, const float *__restrict__ dk, const float *__restrict__ wr,
     int npri, int ns, float two_over_lambda, float3 bore, int centroid, float fc_fixed,
     DopRec *__restrict__ out)
 {
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void stage_doppler_kernel(
         r.x = pos[3 * e];
         r.y = pos[3 * e + 1];
         r.z = pos[3 * e + 2];
-        r.k = kt[m];
+        r.k = dk[m];
         r.vx = two_over_lambda * vv.x;
         r.vy = two_over_lambda * vv.y;
         r.vz = two_over_lambda * vv.z;
@@ -142,10 +145,10 @@ __global__ __launch_bounds__(kBlock) void stage_doppler_kernel(
     }
 }
 
-// Stop-and-hop terms: (raw * range window, k_m), the same products as
+// Stop-and-hop terms: (raw * range window, dk_m), the same products as
 // stage_plain_kernel.
 __global__ __launch_bounds__(kBlock) void stage_terms_kernel(const float2 *__restrict__ raw,
-                                                             const float *__restrict__ kt,
+                                                             const float *__restrict__ dk,
                                                              const float *__restrict__ wr,
                                                              int npri, int ns,
                                                              float4 *__restrict__ out)
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void stage_terms_kernel(const float2 *__res
          e += static_cast<size_t>(gridDim.x) * kBlock) {
         const int m = static_cast<int>(e % ns);
         const float2 v = raw[e];
-        out[e] = make_float4(v.x * wr[m], v.y * wr[m], kt[m], 0.0F);
+        out[e] = make_float4(v.x * wr[m], v.y * wr[m], dk[m], 0.0F);
     }
 }
 
@@ -186,7 +189,9 @@ __global__ __launch_bounds__(kBlock) void stage_chirps_kernel(
 
 template <int P>
 __global__ __launch_bounds__(kBlock) void plain_kernel(const PlainRec *__restrict__ rec,
-                                                       const float *__restrict__ grid, Geom g,
+                                                       const float4 *__restrict__ cpos,
+                                                       const float *__restrict__ grid,
+                                                       const double *__restrict__ kanchor, Geom g,
                                                        float2 *__restrict__ out)
 {
     int tile, split;
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void plain_kernel(const PlainRec *__restric
     if (tile >= g.ntiles) return;
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
-    plain_tile<P>(rec, grid, g.npx, g.ns, n0, n1, tile * P * kBlock,
+    plain_tile<P>(rec, cpos, grid, kanchor, g.npx, g.ns, n0, n1, tile * P * kBlock,
                   out + static_cast<size_t>(split) * g.pstride);
 }
 
@@ -249,8 +254,10 @@ __device__ __forceinline__ bool chunk_processed(const DopRec *__restrict__ rec, 
 
 template <int P>
 __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restrict__ rec,
-                                                         const float *__restrict__ grid, Geom g,
-                                                         DopParams dp, float2 *__restrict__ out,
+                                                         const float *__restrict__ grid,
+                                                         const double *__restrict__ kanchor,
+                                                         Geom g, DopParams dp,
+                                                         float2 *__restrict__ out,
                                                          unsigned long long *__restrict__ counters)
 {
     int tile, split;
@@ -266,13 +273,14 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
     for (int j = 0; j < P; ++j) {
         pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
         live[j] = pix[j] < g.npx;
-        const int q = min(pix[j], g.npx - 1);
+        const size_t q = static_cast<size_t>(min(pix[j], g.npx - 1));
         px[j] = grid[3 * q];
         py[j] = grid[3 * q + 1];
         pz[j] = grid[3 * q + 2];
         ar[j] = 0.0F;
         ai[j] = 0.0F;
     }
+    const cptr<double> ka = as_const(kanchor);
     unsigned long long processed = 0, skipped = 0;
 
     int n = n0;
@@ -292,32 +300,62 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
             any |= on[j];
         }
         if (__ballot(any) != 0ULL) {
-            const DopRec *__restrict__ r = rec + static_cast<size_t>(n) * g.ns;
-            const int nterms = (nend - n) * g.ns;
-#pragma unroll 2
-            for (int e = 0; e < nterms; ++e) {
-                const DopRec t = r[e];
+            for (int c = n; c < nend; ++c) {
+                const DopRec *__restrict__ r = rec + static_cast<size_t>(c) * g.ns;
+                // the phase's FP64 range from the chirp's first position (plain_tile)
+                const float cx = r[0].x, cy = r[0].y, cz = r[0].z;
+                double rd[P];
+                float r32[P], d0x[P], d0y[P], d0z[P];
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
-                    const float dx = px[j] - t.x;
-                    const float dy = py[j] - t.y;
-                    const float dz = pz[j] - t.z;
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float rs = __builtin_amdgcn_rsqf(r2);
-                    // windowing.h:29-37: (2/lambda) v . (p - pos) / |p - pos|
-                    const float fd = ((t.vx * dx) + (t.vy * dy) + (t.vz * dz)) * rs;
-                    const float delta = fd - t.fc;
-                    // windowing.h:43-58; a NaN difference fails the test
-                    const bool inband = on[j] && (fabsf(delta) <= dp.half_bw);
-                    const float wa =
-                        dp.taper ? fmaf(1.0F - dp.az_a, cos_rev(delta * dp.inv_bw), dp.az_a) : 1.0F;
-                    const float th = __builtin_amdgcn_fractf((r2 * rs) * t.k);
-                    const float cs = inband ? cos_rev(th) * wa : 0.0F;
-                    const float sn = inband ? sin_rev(th) * wa : 0.0F;
-                    ar[j] = fmaf(t.rr, cs, ar[j]);
-                    ar[j] = fmaf(-t.ri, sn, ar[j]);
-                    ai[j] = fmaf(t.rr, sn, ai[j]);
-                    ai[j] = fmaf(t.ri, cs, ai[j]);
+                    rd[j] = range64(px[j], py[j], pz[j], cx, cy, cz);
+                    r32[j] = static_cast<float>(rd[j]);
+                    d0x[j] = px[j] - cx;
+                    d0y[j] = py[j] - cy;
+                    d0z[j] = pz[j] - cz;
+                }
+                for (int b0 = 0; b0 < g.ns; b0 += kAnchorBlock) {
+                    const double kb = ka[b0 / kAnchorBlock];
+                    const float kb32 = static_cast<float>(kb);
+                    float thb[P];
+#pragma unroll
+                    for (int j = 0; j < P; ++j) thb[j] = anchor_phase(kb, rd[j]);
+                    const int b1 = min(g.ns, b0 + kAnchorBlock);
+#pragma unroll 2
+                    for (int m = b0; m < b1; ++m) {
+                        const DopRec t = r[m];
+                        // offset of this sample's position from the chirp's first
+                        const float ex = t.x - cx, ey = t.y - cy, ez = t.z - cz;
+#pragma unroll
+                        for (int j = 0; j < P; ++j) {
+                            const float dx = px[j] - t.x;
+                            const float dy = py[j] - t.y;
+                            const float dz = pz[j] - t.z;
+                            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            const float rs = __builtin_amdgcn_rsqf(r2);
+                            // windowing.h:29-37: (2/lambda) v . (p - pos) / |p - pos|
+                            const float fd = ((t.vx * dx) + (t.vy * dy) + (t.vz * dz)) * rs;
+                            const float delta = fd - t.fc;
+                            // windowing.h:43-58; a NaN difference fails the test
+                            const bool inband = on[j] && (fabsf(delta) <= dp.half_bw);
+                            const float wa = dp.taper
+                                                 ? fmaf(1.0F - dp.az_a, cos_rev(delta * dp.inv_bw),
+                                                        dp.az_a)
+                                                 : 1.0F;
+                            // |d|^2 - R0^2 = -e . (d + d0), with d = d0 - e
+                            const float num = -fmaf(ez, dz + d0z[j],
+                                                    fmaf(ey, dy + d0y[j], ex * (dx + d0x[j])));
+                            const float dr = num * __builtin_amdgcn_rcpf(r2 * rs + r32[j]);
+                            const float th = __builtin_amdgcn_fractf(
+                                fmaf(t.k, r32[j] + dr, fmaf(kb32, dr, thb[j])));
+                            const float cs = inband ? cos_rev(th) * wa : 0.0F;
+                            const float sn = inband ? sin_rev(th) * wa : 0.0F;
+                            ar[j] = fmaf(t.rr, cs, ar[j]);
+                            ar[j] = fmaf(-t.ri, sn, ar[j]);
+                            ai[j] = fmaf(t.rr, sn, ai[j]);
+                            ai[j] = fmaf(t.ri, cs, ai[j]);
+                        }
+                    }
                 }
             }
         }
@@ -338,7 +376,8 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
 template <int P>
 __global__ __launch_bounds__(kBlock) void range_kernel(const float4 *__restrict__ rec,
                                                        const float4 *__restrict__ cpos,
-                                                       const float *__restrict__ grid, Geom g,
+                                                       const float *__restrict__ grid,
+                                                       const double *__restrict__ kanchor, Geom g,
                                                        float2 *__restrict__ out)
 {
     int tile, split;
@@ -346,8 +385,8 @@ __global__ __launch_bounds__(kBlock) void range_kernel(const float4 *__restrict_
     if (tile >= g.ntiles) return;
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
-    range_tile<P>(rec, cpos, grid, g.npx, g.ns, n0, n1, tile * P * kBlock,
-                  out + static_cast<size_t>(split) * g.pstride);
+    range_tile<P, float, float4>(rec, cpos, grid, kanchor, g.npx, g.ns, n0, n1, tile * P * kBlock,
+                                 out + static_cast<size_t>(split) * g.pstride);
 }
 
 // chunk_processed on stop-and-hop input: the three probe samples of a row
@@ -392,8 +431,9 @@ __device__ __forceinline__ bool chunk_processed_chirp(const float4 *__restrict__
 template <int P>
 __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
     const float4 *__restrict__ rec, const float4 *__restrict__ cpos,
-    const float4 *__restrict__ cvel, const float *__restrict__ grid, Geom g, DopParams dp,
-    float2 *__restrict__ out, unsigned long long *__restrict__ counters)
+    const float4 *__restrict__ cvel, const float *__restrict__ grid,
+    const double *__restrict__ kanchor, Geom g, DopParams dp, float2 *__restrict__ out,
+    unsigned long long *__restrict__ counters)
 {
     __shared__ float4 s_term[kTermStage];
     int tile, split;
@@ -409,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
     for (int j = 0; j < P; ++j) {
         pix[j] = (tile * P + j) * kBlock + static_cast<int>(threadIdx.x);
         live[j] = pix[j] < g.npx;
-        const int q = min(pix[j], g.npx - 1);
+        const size_t q = static_cast<size_t>(min(pix[j], g.npx - 1));
         px[j] = grid[3 * q];
         py[j] = grid[3 * q + 1];
         pz[j] = grid[3 * q + 2];
@@ -434,7 +474,8 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
         for (int c = n; c < nend; ++c) {
             const float4 cp = as_const(cpos)[c];
             const float4 cv = as_const(cvel)[c];
-            float rng[P], fac[P];
+            double rd[P];
+            float r32[P], fac[P];
             bool any = false;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
@@ -450,7 +491,9 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
                 const float wa =
                     dp.taper ? fmaf(1.0F - dp.az_a, cos_rev(delta * dp.inv_bw), dp.az_a) : 1.0F;
                 fac[j] = inband ? wa : 0.0F;
-                rng[j] = __builtin_amdgcn_sqrtf(r2);
+                // the phase's FP64 range (range_tile)
+                rd[j] = range64(px[j], py[j], pz[j], cp.x, cp.y, cp.z);
+                r32[j] = static_cast<float>(rd[j]);
                 any |= inband;
             }
             const bool run = __ballot(any) != 0ULL;
@@ -461,7 +504,8 @@ __global__ __launch_bounds__(kBlock) void doppler_chirp_kernel(
                 tr[j] = 0.0F;
                 ti[j] = 0.0F;
             }
-            chirp_terms<P>(rec + static_cast<size_t>(c) * g.ns, g.ns, s_term, rng, tr, ti, run);
+            chirp_terms<P>(rec + static_cast<size_t>(c) * g.ns, g.ns, s_term, kanchor, rd, r32, tr,
+                           ti, run);
             if (!run) continue;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
@@ -808,42 +852,46 @@ int correlate_grid(const Geom &g) { return g.tiles_padded * g.nsplit; }
 
 } // namespace
 
-hipError_t launch_stage_plain(const float *raw, const float *pos, const float *kt, const float *wr,
+hipError_t launch_stage_plain(const float *raw, const float *pos, const float *dk, const float *wr,
                               int npri, int ns, PlainRec *out, hipStream_t s)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
     hipLaunchKernelGGL(stage_plain_kernel, dim3(stage_grid(total)), dim3(kBlock), 0, s,
-                       reinterpret_cast<const float2 *>(raw), pos, kt, wr, npri, ns, out);
+                       reinterpret_cast<const float2 *>(raw), pos, dk, wr, npri, ns, out);
     return hipGetLastError();
 }
 
 hipError_t launch_stage_doppler(const float *raw, const float *pos, const float *vel,
-                                const float *att, const float *kt, const float *wr, int npri,
+                                const float *att, const float *dk, const float *wr, int npri,
                                 int ns, float two_over_lambda, const float *bore, int centroid,
                                 float fc_fixed, DopRec *out, hipStream_t s)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
     hipLaunchKernelGGL(stage_doppler_kernel, dim3(stage_grid(total)), dim3(kBlock), 0, s,
                        reinterpret_cast<const float2 *>(raw), pos, vel,
-                       reinterpret_cast<const float4 *>(att), kt, wr, npri, ns, two_over_lambda,
+                       reinterpret_cast<const float4 *>(att), dk, wr, npri, ns, two_over_lambda,
                        make_float3(bore[0], bore[1], bore[2]), centroid, fc_fixed, out);
     return hipGetLastError();
 }
 
-hipError_t launch_plain(int ppl, const PlainRec *rec, const float *grid, const Geom &g,
-                        float2 *out, hipStream_t s, int *workgroups)
+hipError_t launch_plain(int ppl, const PlainRec *rec, const float4 *cpos, const float *grid,
+                        const double *kanchor, const Geom &g, float2 *out, hipStream_t s,
+                        int *workgroups)
 {
     const int nb = correlate_grid(g);
     *workgroups = nb;
     switch (ppl) {
     case 1:
-        hipLaunchKernelGGL(plain_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, out);
+        hipLaunchKernelGGL(plain_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
+                           g, out);
         break;
     case 2:
-        hipLaunchKernelGGL(plain_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, out);
+        hipLaunchKernelGGL(plain_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
+                           g, out);
         break;
     case 4:
-        hipLaunchKernelGGL(plain_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, out);
+        hipLaunchKernelGGL(plain_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
+                           g, out);
         break;
     default:
         return hipErrorInvalidValue;
@@ -851,24 +899,24 @@ hipError_t launch_plain(int ppl, const PlainRec *rec, const float *grid, const G
     return hipGetLastError();
 }
 
-hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const Geom &g,
-                          const DopParams &dp, float2 *out, unsigned long long *counters,
-                          hipStream_t s, int *workgroups)
+hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const double *kanchor,
+                          const Geom &g, const DopParams &dp, float2 *out,
+                          unsigned long long *counters, hipStream_t s, int *workgroups)
 {
     const int nb = correlate_grid(g);
     *workgroups = nb;
     switch (ppl) {
     case 1:
-        hipLaunchKernelGGL(doppler_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp, out,
-                           counters);
+        hipLaunchKernelGGL(doppler_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, kanchor, g,
+                           dp, out, counters);
         break;
     case 2:
-        hipLaunchKernelGGL(doppler_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp, out,
-                           counters);
+        hipLaunchKernelGGL(doppler_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, kanchor, g,
+                           dp, out, counters);
         break;
     case 4:
-        hipLaunchKernelGGL(doppler_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, g, dp, out,
-                           counters);
+        hipLaunchKernelGGL(doppler_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, kanchor, g,
+                           dp, out, counters);
         break;
     default:
         return hipErrorInvalidValue;
@@ -876,12 +924,12 @@ hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const G
     return hipGetLastError();
 }
 
-hipError_t launch_stage_terms(const float *raw, const float *kt, const float *wr, int npri, int ns,
+hipError_t launch_stage_terms(const float *raw, const float *dk, const float *wr, int npri, int ns,
                               float4 *out, hipStream_t s)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
     hipLaunchKernelGGL(stage_terms_kernel, dim3(stage_grid(total)), dim3(kBlock), 0, s,
-                       reinterpret_cast<const float2 *>(raw), kt, wr, npri, ns, out);
+                       reinterpret_cast<const float2 *>(raw), dk, wr, npri, ns, out);
     return hipGetLastError();
 }
 
@@ -897,19 +945,23 @@ hipError_t launch_stage_chirps(const float *pos, const float *vel, const float *
 }
 
 hipError_t launch_range(int ppl, const float4 *rec, const float4 *cpos, const float *grid,
-                        const Geom &g, float2 *out, hipStream_t s, int *workgroups)
+                        const double *kanchor, const Geom &g, float2 *out, hipStream_t s,
+                        int *workgroups)
 {
     const int nb = correlate_grid(g);
     *workgroups = nb;
     switch (ppl) {
     case 1:
-        hipLaunchKernelGGL(range_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, g, out);
+        hipLaunchKernelGGL(range_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
+                           g, out);
         break;
     case 2:
-        hipLaunchKernelGGL(range_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, g, out);
+        hipLaunchKernelGGL(range_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
+                           g, out);
         break;
     case 4:
-        hipLaunchKernelGGL(range_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, g, out);
+        hipLaunchKernelGGL(range_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
+                           g, out);
         break;
     default:
         return hipErrorInvalidValue;
@@ -918,24 +970,24 @@ hipError_t launch_range(int ppl, const float4 *rec, const float4 *cpos, const fl
 }
 
 hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
-                                const float4 *cvel, const float *grid, const Geom &g,
-                                const DopParams &dp, float2 *out, unsigned long long *counters,
-                                hipStream_t s, int *workgroups)
+                                const float4 *cvel, const float *grid, const double *kanchor,
+                                const Geom &g, const DopParams &dp, float2 *out,
+                                unsigned long long *counters, hipStream_t s, int *workgroups)
 {
     const int nb = correlate_grid(g);
     *workgroups = nb;
     switch (ppl) {
     case 1:
         hipLaunchKernelGGL(doppler_chirp_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, cvel,
-                           grid, g, dp, out, counters);
+                           grid, kanchor, g, dp, out, counters);
         break;
     case 2:
         hipLaunchKernelGGL(doppler_chirp_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, cvel,
-                           grid, g, dp, out, counters);
+                           grid, kanchor, g, dp, out, counters);
         break;
     case 4:
         hipLaunchKernelGGL(doppler_chirp_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, cvel,
-                           grid, g, dp, out, counters);
+                           grid, kanchor, g, dp, out, counters);
         break;
     default:
         return hipErrorInvalidValue;

@@ -19,14 +19,13 @@ constexpr int kMaxJobSlots = 8192;
 // One admitted job as the kernels see it.
 struct alignas(16) ServeJob {
     const void *raw;          // precision: range-profile table or matrix engine A tiles;
-                              // plain: (rr, ri, k, 0) terms
+                              // plain: (rr, ri, dk_m, 0) terms
     const float *inv_scale;   // precision: per-chirp scale of the A tiles
-    const ChirpPos *cpos;     // precision: FP64 phase centre per chirp (range profile: from
-                              // the origin pp.o, with its squared norm in pad)
-    const float4 *cpos32;     // plain: FP32 phase centre per chirp (x, y, z, 0)
-    const double *grid64;     // precision: FP64 pixel coordinates
-    const float *grid32;      // plain: FP32 pixel coordinates
-    const double *kanchor;    // precision: 2 (f0 + K t_a) / c per 64-sample block
+    const ChirpPos *cpos;     // FP64 phase centre per chirp (range profile: from the origin
+                              // pp.o, with its squared norm in pad)
+    const double *grid64;     // FP64 pixel coordinates
+    const double *kanchor;    // matrix engine: 2 (f0 + K t_a) / c per 64-sample block;
+                              // plain: the anchor wavenumbers k_b per 64-sample block
     void *partial;            // [nsplit][npx]: double2 (precision) or float2 (plain)
     float2 *image;            // [npx]
     double dk;                // precision, matrix engine: 2 K dt / c
@@ -62,8 +61,9 @@ hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsig
                              int workgroups, bool matrix, hipStream_t s);
 hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,
                                  hipStream_t s);
-hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,
-                                    int npri, int ns, float4 *terms, float4 *cpos32,
-                                    hipStream_t s);
+// Plain-form terms (rr, ri, dk_m, 0) of a served job from its complex64 raw
+// samples (windows applied) and the per-sample dk_m.
+hipError_t launch_stage_plain_chirp(const float *raw, const float *dk, int npri, int ns,
+                                    float4 *terms, hipStream_t s);
 
 } // namespace tdc

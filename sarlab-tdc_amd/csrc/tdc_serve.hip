@@ -55,10 +55,11 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restric
         const int n1 = min(j.npri, n0 + j.cps);
         if (FAMILY == 0) {
             if (j.form == 0) {
-                range_tile<1>(static_cast<const float4 *>(j.raw), j.cpos32, j.grid32, j.npx, j.ns,
-                              n0, n1, tl.pix0,
-                              static_cast<float2 *>(j.partial) +
-                                  static_cast<size_t>(tl.split) * j.npx);
+                range_tile<1, double, ChirpPos>(static_cast<const float4 *>(j.raw), j.cpos,
+                                                j.grid64, j.kanchor, j.npx, j.ns, n0, n1,
+                                                tl.pix0,
+                                                static_cast<float2 *>(j.partial) +
+                                                    static_cast<size_t>(tl.split) * j.npx);
             } else {
                 profile_tile(static_cast<const float2 *>(j.raw), j.cpos, j.grid64, j.pp, j.npx, n0,
                              n1, tl.pix0,
@@ -116,11 +117,12 @@ __global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const ServeJob *
     }
 }
 
-// Plain-form records of a served job (stop-and-hop: one phase centre per
-// chirp): terms (rr, ri, k_m, 0) and the FP32 phase centre of every chirp.
-__global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(
-    const float2 *__restrict__ raw, const ChirpPos *__restrict__ cpos, const float *__restrict__ kt,
-    int npri, int ns, float4 *__restrict__ terms, float4 *__restrict__ cpos32)
+// Plain-form records of a served job (stop-and-hop: the FP64 phase centre of
+// each chirp is read from the job's chirp table): terms (rr, ri, dk_m, 0).
+__global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(const float2 *__restrict__ raw,
+                                                                   const float *__restrict__ dk,
+                                                                   int npri, int ns,
+                                                                   float4 *__restrict__ terms)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
     const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
@@ -128,12 +130,7 @@ __global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(
     for (size_t e = first; e < total; e += stride) {
         const int m = static_cast<int>(e % ns);
         const float2 v = raw[e];
-        terms[e] = make_float4(v.x, v.y, kt[m], 0.0F);
-    }
-    for (size_t n = first; n < static_cast<size_t>(npri); n += stride) {
-        const ChirpPos c = cpos[n];
-        cpos32[n] = make_float4(static_cast<float>(c.x), static_cast<float>(c.y),
-                                static_cast<float>(c.z), 0.0F);
+        terms[e] = make_float4(v.x, v.y, dk[m], 0.0F);
     }
 }
 
@@ -157,15 +154,14 @@ hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, i
     return hipGetLastError();
 }
 
-hipError_t launch_stage_plain_chirp(const float *raw, const ChirpPos *cpos, const float *kt,
-                                    int npri, int ns, float4 *terms, float4 *cpos32,
-                                    hipStream_t s)
+hipError_t launch_stage_plain_chirp(const float *raw, const float *dk, int npri, int ns,
+                                    float4 *terms, hipStream_t s)
 {
     const size_t total = static_cast<size_t>(npri) * ns;
     const size_t want = (total + kBlock - 1) / kBlock;
     const int grid = static_cast<int>(want < 4096 ? (want > 0 ? want : 1) : 4096);
     hipLaunchKernelGGL(stage_plain_chirp_kernel, dim3(grid), dim3(kBlock), 0, s,
-                       reinterpret_cast<const float2 *>(raw), cpos, kt, npri, ns, terms, cpos32);
+                       reinterpret_cast<const float2 *>(raw), dk, npri, ns, terms);
     return hipGetLastError();
 }
 

@@ -220,7 +220,7 @@ struct Job {
     tdc::ProfileParams pp{};
     // device arena layout: [upload region | staged records or raw | partials]
     size_t off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
-           off_cpos32 = 0, off_pcorr = 0, off_ptw = 0, off_ptab = 0,
+           off_pcorr = 0, off_ptw = 0, off_ptab = 0,
            off_atab = 0, off_ascale = 0, off_part = 0, up_bytes = 0, bytes = 0;
     bool needs_stage = false;
     char *arena = nullptr;
@@ -341,16 +341,17 @@ void plan(Job &j)
     } else {
         j.off_kt = o;
         o += align256(4 * static_cast<size_t>(d.n_samples));
+        j.off_ka = o;
+        o += align256(8 * static_cast<size_t>((d.n_samples + tdc::kAnchorBlock - 1) /
+                                              tdc::kAnchorBlock));
         j.off_grid = o;
-        o += align256(12 * npx);
+        o += align256(24 * npx);
     }
     j.up_bytes = o;
     j.needs_stage = true;
     j.off_stage = o;
     if (!prec) {
         o += align256(sizeof(float4) * nterm);
-        j.off_cpos32 = o;
-        o += align256(sizeof(float4) * static_cast<size_t>(d.n_pri));
     } else if (d.raw_storage != TDC_RAW_C64) {
         o += align256(4 * nterm);
     }
@@ -377,8 +378,8 @@ int validate(const tdc_job_desc *d)
         return sfail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not be empty");
     if (static_cast<int64_t>(d->n_pri) * d->n_samples > INT_MAX)
         return sfail(TDC_ERR_INVALID_ARGUMENT, "nPri * nSamples must not exceed 2^31 - 1");
-    if (static_cast<int64_t>(d->n_rows) * d->n_cols > INT_MAX)
-        return sfail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^31 - 1 pixels");
+    if (static_cast<int64_t>(d->n_rows) * d->n_cols > tdc::kMaxPixels)
+        return sfail(TDC_ERR_INVALID_ARGUMENT, "the focus grid must not exceed 2^30 pixels");
     if (!d->raw_iq || !d->sample_times || !d->chirp_position || !d->grid)
         return sfail(TDC_ERR_INVALID_ARGUMENT, "job arrays must not be NULL");
     if (d->form != TDC_FORM_PLAIN && d->form != TDC_FORM_PRECISION)
@@ -457,11 +458,16 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
             j.pp.Mp = plen + tdc::kProfilePad;
         }
     } else {
-        auto *kt = reinterpret_cast<float *>(h + j.off_kt);
+        // the plain form's phase tables (the processor's, from FP64 inputs) and
+        // the FP64 grid: the range is computed in FP64 from FP64 coordinates
+        std::vector<double> k(d.n_samples), kb;
+        std::vector<float> dkm;
         for (int m = 0; m < d.n_samples; ++m)
-            kt[m] = static_cast<float>(2.0 * (d.start_freq + d.mod_rate * d.sample_times[m]) / kC);
-        auto *g = reinterpret_cast<float *>(h + j.off_grid);
-        for (size_t i = 0; i < 3 * static_cast<size_t>(j.npx); ++i) g[i] = static_cast<float>(d.grid[i]);
+            k[m] = 2.0 * (d.start_freq + d.mod_rate * d.sample_times[m]) / kC;
+        tdc::plain_phase_tables(k, dkm, kb);
+        std::memcpy(h + j.off_kt, dkm.data(), 4 * dkm.size());
+        std::memcpy(h + j.off_ka, kb.data(), 8 * kb.size());
+        std::memcpy(h + j.off_grid, d.grid, 24 * static_cast<size_t>(j.npx));
     }
     char *arena = nullptr;
     hipError_t e = s->arenas.get(j.bytes, s->budget, &arena);
@@ -505,8 +511,8 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
         sj.dk = dk;
     } else {
         sj.raw = j.arena + j.off_stage;
-        sj.cpos32 = reinterpret_cast<const float4 *>(j.arena + j.off_cpos32);
-        sj.grid32 = reinterpret_cast<const float *>(j.arena + j.off_grid);
+        sj.grid64 = reinterpret_cast<const double *>(j.arena + j.off_grid);
+        sj.kanchor = reinterpret_cast<const double *>(j.arena + j.off_ka);
     }
     // the kernels read the descriptor from pinned memory; it is written before
     // any step that names the job is launched
@@ -539,12 +545,10 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
                                         tdc::matrix_parts(d.raw_storage), j.arena + j.off_atab,
                                         reinterpret_cast<float *>(j.arena + j.off_ascale), ss);
     }
-    return tdc::launch_stage_plain_chirp(
-        reinterpret_cast<const float *>(j.arena),
-        reinterpret_cast<const tdc::ChirpPos *>(j.arena + j.off_cpos),
-        reinterpret_cast<const float *>(j.arena + j.off_kt), d.n_pri, d.n_samples,
-        reinterpret_cast<float4 *>(j.arena + j.off_stage),
-        reinterpret_cast<float4 *>(j.arena + j.off_cpos32), ss);
+    return tdc::launch_stage_plain_chirp(reinterpret_cast<const float *>(j.arena),
+                                         reinterpret_cast<const float *>(j.arena + j.off_kt),
+                                         d.n_pri, d.n_samples,
+                                         reinterpret_cast<float4 *>(j.arena + j.off_stage), ss);
 }
 
 bool ensure_graph(tdc_server *s, int matrix, int bucket, int buf, std::string &err)

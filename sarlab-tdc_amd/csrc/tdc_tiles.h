@@ -57,6 +57,9 @@ __device__ __forceinline__ double range_newton(double r2)
     return fma(fma(-r0, r0, r2), static_cast<double>(0.5F * y0f), r0);
 }
 
+// x - floor(x) in FP64 (one instruction)
+__device__ __forceinline__ double fract64(double x) { return __builtin_amdgcn_fract(x); }
+
 // sin and cos of 2 pi x for x in [0, 1): reduction to the nearest quarter
 // turn, |2 pi f| <= pi/4, then the Taylor series through a^9 (sin) and a^10
 // (cos), whose truncation error there is below 2e-9.
@@ -143,58 +146,123 @@ struct RawBF16 {
     }
 };
 
-// Plain form over one tile, Doppler window identically one: per term the
-// direct phase theta = fract(k_m |pos - p|) in cycles and a complex MAC of
-// raw * exp(+j 2 pi theta) (reference tdckernels.cu:206-215, :270-272). The
-// workgroup stages the tile's term records in LDS, kStageRecords at a time,
-// and every lane reads each record (a broadcast ds_read): the records then
-// reach the VALU as VGPR operands, which issue at twice the rate of SGPR
-// operands on gfx950.
+// ---------------------------------------------------------------------------
+// Plain form (the drop-in): per term the direct reference phase
+// theta = fract(k_m |pos - p|) in cycles, k_m = 2 (f0 + K t_m) / c, and a
+// complex MAC of raw * exp(+j 2 pi theta) (reference tdckernels.cu:206-215,
+// :270-272).
+//
+// Phase evaluation. k_m R reaches ~10^3 cycles, so forming it in FP32 leaves
+// ~1e-4 cycles of rounding in every term. Instead the range R0 of a (pixel,
+// chirp) pair is computed in FP64 (exact differences of the coordinates and
+// an FP64 Newton square root), and every kAnchorBlock samples the phase of
+// the block's anchor wavenumber k_b is reduced modulo one cycle in FP64,
+// theta_b = fract(k_b R0). A term then adds dk_m R0 in FP32, dk_m = k_m - k_b
+// tabulated per sample: theta_m = fract(fma(dk_m, R0, theta_b)), one FMA and
+// a fract per term -- the operation count of fract(k_m R) -- with |dk_m R0|
+// a few cycles, so ~1e-7 cycles of rounding.
+//
+// When the antenna moves within a chirp, the range of sample m is
+// R0 + dR_m from the chirp's first position c: with delta = pos_m - c and
+// d0 = p - c, |p - pos_m|^2 - R0^2 = delta . (delta - 2 d0) = num and
+// dR = num / (|p - pos_m| + R0), formed from the small delta without
+// cancellation; then theta_m = fract(theta_b + k_b dR + dk_m (R0 + dR)). For
+// delta = 0 (stop-and-hop input) every extra term is an exact zero and the
+// phase is bit-identical to the per-chirp form above.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double range64(double px, double py, double pz, double cx, double cy,
+                                          double cz)
+{
+    const double dx = px - cx, dy = py - cy, dz = pz - cz;
+    return range_newton(fma(dz, dz, fma(dy, dy, dx * dx)));
+}
+
+// theta_b = fract(k_b R) in FP64, rounded to FP32
+__device__ __forceinline__ float anchor_phase(double kb, double r)
+{
+    return static_cast<float>(fract64(kb * r));
+}
+
+// Per-term records of the plain form, motion within a chirp: the workgroup
+// stages a chirp's records in LDS, kStageRecords at a time, and every lane
+// reads each record (a broadcast ds_read): the records then reach the VALU as
+// VGPR operands, which issue at twice the rate of SGPR operands on gfx950.
+// Records: `rec` [npri][ns] PlainRec (raw * window, delta, dk_m), `cpos`
+// [npri] float4 (x, y, z, 0) of each chirp's first sample, `kanchor`
+// [ceil(ns / kAnchorBlock)] FP64 anchor wavenumbers.
 constexpr int kStageRecords = 256;
 
 template <int P>
 __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
-                                           const float *__restrict__ grid, int npx, int ns,
+                                           const float4 *__restrict__ cpos,
+                                           const float *__restrict__ grid,
+                                           const double *__restrict__ kanchor, int npx, int ns,
                                            int n0, int n1, int pix0, float2 *__restrict__ out)
 {
     __shared__ float4 s_rec[2 * kStageRecords];
-    float px[P], py[P], pz[P], ar[P], ai[P];
+    double px[P], py[P], pz[P];
+    float ar[P], ai[P];
     int pix[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         pix[j] = pix0 + j * kBlock + static_cast<int>(threadIdx.x);
-        const int q = min(pix[j], npx - 1);
+        const size_t q = static_cast<size_t>(min(pix[j], npx - 1));
         px[j] = grid[3 * q];
         py[j] = grid[3 * q + 1];
         pz[j] = grid[3 * q + 2];
         ar[j] = 0.0F;
         ai[j] = 0.0F;
     }
-    const float4 *src = reinterpret_cast<const float4 *>(rec + static_cast<size_t>(max(n0, 0)) * ns);
-    const int nterms = (n1 > n0) ? (n1 - n0) * ns : 0;
-    for (int e0 = 0; e0 < nterms; e0 += kStageRecords) {
-        const int cnt = min(kStageRecords, nterms - e0);
-        __syncthreads();
-        for (int i = static_cast<int>(threadIdx.x); i < 2 * cnt; i += kBlock)
-            s_rec[i] = src[2 * static_cast<size_t>(e0) + i];
-        __syncthreads();
-#pragma unroll 4
-        for (int e = 0; e < cnt; ++e) {
-            // record fields: a = (rr, ri, x, y), b = (z, k, pad, pad)
-            const float4 a = s_rec[2 * e], b = s_rec[2 * e + 1];
+    const cptr<double> ka = as_const(kanchor);
+    for (int n = n0; n < n1; ++n) {
+        const float4 c = as_const(cpos)[n];
+        double rd[P];
+        float r32[P], r0sq[P], mx[P], my[P], mz[P];
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const float dx = px[j] - a.z;
-                const float dy = py[j] - a.w;
-                const float dz = pz[j] - b.x;
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                const float th = __builtin_amdgcn_fractf(__builtin_amdgcn_sqrtf(r2) * b.y);
-                const float sn = sin_rev(th);
-                const float cs = cos_rev(th);
-                ar[j] = fmaf(a.x, cs, ar[j]);
-                ar[j] = fmaf(-a.y, sn, ar[j]);
-                ai[j] = fmaf(a.x, sn, ai[j]);
-                ai[j] = fmaf(a.y, cs, ai[j]);
+        for (int j = 0; j < P; ++j) {
+            rd[j] = range64(px[j], py[j], pz[j], c.x, c.y, c.z);
+            r32[j] = static_cast<float>(rd[j]);
+            r0sq[j] = static_cast<float>(rd[j] * rd[j]);
+            // -2 d0, d0 = p - c
+            mx[j] = static_cast<float>(-2.0 * (px[j] - c.x));
+            my[j] = static_cast<float>(-2.0 * (py[j] - c.y));
+            mz[j] = static_cast<float>(-2.0 * (pz[j] - c.z));
+        }
+        const float4 *src = reinterpret_cast<const float4 *>(rec + static_cast<size_t>(n) * ns);
+        for (int m0 = 0; m0 < ns; m0 += kStageRecords) {
+            const int cnt = min(kStageRecords, ns - m0);
+            __syncthreads();
+            for (int i = static_cast<int>(threadIdx.x); i < 2 * cnt; i += kBlock)
+                s_rec[i] = src[2 * static_cast<size_t>(m0) + i];
+            __syncthreads();
+            for (int b0 = 0; b0 < cnt; b0 += kAnchorBlock) {
+                const double kb = ka[(m0 + b0) / kAnchorBlock];
+                const float kb32 = static_cast<float>(kb);
+                float thb[P];
+#pragma unroll
+                for (int j = 0; j < P; ++j) thb[j] = anchor_phase(kb, rd[j]);
+                const int b1 = min(cnt, b0 + kAnchorBlock);
+#pragma unroll 4
+                for (int e = b0; e < b1; ++e) {
+                    // record fields: a = (rr, ri, delta x, delta y), b = (delta z, dk, pad, pad)
+                    const float4 a = s_rec[2 * e], b = s_rec[2 * e + 1];
+#pragma unroll
+                    for (int j = 0; j < P; ++j) {
+                        const float num =
+                            fmaf(b.x, b.x + mz[j], fmaf(a.w, a.w + my[j], a.z * (a.z + mx[j])));
+                        const float rm = __builtin_amdgcn_sqrtf(r0sq[j] + num);
+                        const float dr = num * __builtin_amdgcn_rcpf(rm + r32[j]);
+                        const float th = __builtin_amdgcn_fractf(
+                            fmaf(b.y, r32[j] + dr, fmaf(kb32, dr, thb[j])));
+                        const float sn = sin_rev(th);
+                        const float cs = cos_rev(th);
+                        ar[j] = fmaf(a.x, cs, ar[j]);
+                        ar[j] = fmaf(-a.y, sn, ar[j]);
+                        ai[j] = fmaf(a.x, sn, ai[j]);
+                        ai[j] = fmaf(a.y, cs, ai[j]);
+                    }
+                }
             }
         }
     }
@@ -205,67 +273,71 @@ __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
 }
 
 // Stop-and-hop input: when the antenna position is the same for every sample
-// of a chirp, the range |pos - p| of plain_tile is one FP32 value per
-// (pixel, chirp). It is computed once with the same operations and each term
-// keeps the direct phase fract(k_m * range), so the image is bit-identical to
-// plain_tile's while the three subtractions, the squared norm and v_sqrt_f32
-// leave the per-term work. Records: `rec` [npri][ns] float4 (rr, ri, k, 0),
-// `cpos` [npri] float4 (x, y, z, 0).
+// of a chirp, the range is one value per (pixel, chirp) and a term's phase is
+// fract(fma(dk_m, R0, theta_b)): the same operations plain_tile performs on
+// such input, so the image is bit-identical to plain_tile's while the per-term
+// range arithmetic leaves the loop. Records: `rec` [npri][ns] float4
+// (rr, ri, dk_m, 0), `cpos` [npri] chirp positions (float4 (x, y, z, 0) or
+// ChirpPos), `grid` FP32 or FP64 pixel coordinates.
 constexpr int kTermStage = 512;
 
-__device__ __forceinline__ float range32(float px, float py, float pz, float4 c)
-{
-    const float dx = px - c.x;
-    const float dy = py - c.y;
-    const float dz = pz - c.z;
-    return __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-}
-
-// Adds sum_m (rr + j ri) exp(+j 2 pi fract(rng * k_m)) over one chirp's
-// records `row` to (sr, si), term by term in sample order. Every wave of the
-// workgroup takes part in the staging; a wave with run == false (the same for
-// all its lanes) only stages.
+// Adds sum_m (rr + j ri) exp(+j 2 pi theta_m) over one chirp's records `row`
+// to (sr, si), term by term in sample order, with the pair's FP64 range `rd`
+// and its FP32 rounding `r32`. Every wave of the workgroup takes part in the
+// staging; a wave with run == false (the same for all its lanes) only stages.
 template <int P>
 __device__ __forceinline__ void chirp_terms(const float4 *__restrict__ row, int ns,
-                                            float4 *__restrict__ s_term, const float (&rng)[P],
+                                            float4 *__restrict__ s_term,
+                                            const double *__restrict__ kanchor,
+                                            const double (&rd)[P], const float (&r32)[P],
                                             float (&sr)[P], float (&si)[P], bool run)
 {
+    const cptr<double> ka = as_const(kanchor);
     for (int m0 = 0; m0 < ns; m0 += kTermStage) {
         const int cnt = min(kTermStage, ns - m0);
         __syncthreads();
         for (int i = static_cast<int>(threadIdx.x); i < cnt; i += kBlock) s_term[i] = row[m0 + i];
         __syncthreads();
         if (!run) continue;
-#pragma unroll 4
-        for (int e = 0; e < cnt; ++e) {
-            const float4 a = s_term[e];
+        for (int b0 = 0; b0 < cnt; b0 += kAnchorBlock) {
+            const double kb = ka[(m0 + b0) / kAnchorBlock];
+            float thb[P];
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-                const float th = __builtin_amdgcn_fractf(rng[j] * a.z);
-                const float sn = sin_rev(th);
-                const float cs = cos_rev(th);
-                sr[j] = fmaf(a.x, cs, sr[j]);
-                sr[j] = fmaf(-a.y, sn, sr[j]);
-                si[j] = fmaf(a.x, sn, si[j]);
-                si[j] = fmaf(a.y, cs, si[j]);
+            for (int j = 0; j < P; ++j) thb[j] = anchor_phase(kb, rd[j]);
+            const int b1 = min(cnt, b0 + kAnchorBlock);
+#pragma unroll 4
+            for (int e = b0; e < b1; ++e) {
+                const float4 a = s_term[e];
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const float th = __builtin_amdgcn_fractf(fmaf(a.z, r32[j], thb[j]));
+                    const float sn = sin_rev(th);
+                    const float cs = cos_rev(th);
+                    sr[j] = fmaf(a.x, cs, sr[j]);
+                    sr[j] = fmaf(-a.y, sn, sr[j]);
+                    si[j] = fmaf(a.x, sn, si[j]);
+                    si[j] = fmaf(a.y, cs, si[j]);
+                }
             }
         }
     }
 }
 
-template <int P>
+template <int P, class GT, class CT>
 __device__ __forceinline__ void range_tile(const float4 *__restrict__ rec,
-                                           const float4 *__restrict__ cpos,
-                                           const float *__restrict__ grid, int npx, int ns,
+                                           const CT *__restrict__ cpos,
+                                           const GT *__restrict__ grid,
+                                           const double *__restrict__ kanchor, int npx, int ns,
                                            int n0, int n1, int pix0, float2 *__restrict__ out)
 {
     __shared__ float4 s_term[kTermStage];
-    float px[P], py[P], pz[P], ar[P], ai[P];
+    double px[P], py[P], pz[P];
+    float ar[P], ai[P];
     int pix[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         pix[j] = pix0 + j * kBlock + static_cast<int>(threadIdx.x);
-        const int q = min(pix[j], npx - 1);
+        const size_t q = static_cast<size_t>(min(pix[j], npx - 1));
         px[j] = grid[3 * q];
         py[j] = grid[3 * q + 1];
         pz[j] = grid[3 * q + 2];
@@ -273,11 +345,16 @@ __device__ __forceinline__ void range_tile(const float4 *__restrict__ rec,
         ai[j] = 0.0F;
     }
     for (int n = n0; n < n1; ++n) {
-        const float4 c = as_const(cpos)[n];
-        float rng[P];
+        const CT c = as_const(cpos)[n];
+        double rd[P];
+        float r32[P];
 #pragma unroll
-        for (int j = 0; j < P; ++j) rng[j] = range32(px[j], py[j], pz[j], c);
-        chirp_terms<P>(rec + static_cast<size_t>(n) * ns, ns, s_term, rng, ar, ai, true);
+        for (int j = 0; j < P; ++j) {
+            rd[j] = range64(px[j], py[j], pz[j], c.x, c.y, c.z);
+            r32[j] = static_cast<float>(rd[j]);
+        }
+        chirp_terms<P>(rec + static_cast<size_t>(n) * ns, ns, s_term, kanchor, rd, r32, ar, ai,
+                       true);
     }
 #pragma unroll
     for (int j = 0; j < P; ++j) {
@@ -303,7 +380,7 @@ __device__ __forceinline__ void precision_tile(
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         pix[j] = pix0 + j * kBlock + static_cast<int>(threadIdx.x);
-        const int q = min(pix[j], npx - 1);
+        const size_t q = static_cast<size_t>(min(pix[j], npx - 1));
         px[j] = grid[3 * q];
         py[j] = grid[3 * q + 1];
         pz[j] = grid[3 * q + 2];
@@ -478,9 +555,6 @@ __device__ __forceinline__ void split_f16x2(float v0, float v1, unsigned int &hi
         : "v"(hi), "v"(v1));
 }
 
-// x - floor(x) in FP64 (one instruction)
-__device__ __forceinline__ double fract64(double x) { return __builtin_amdgcn_fract(x); }
-
 __device__ __forceinline__ void cmul(float ar, float ai, float br, float bi, float &cr, float &ci)
 {
     cr = fmaf(ar, br, -(ai * bi));
@@ -529,7 +603,7 @@ __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
     const int lane = static_cast<int>(threadIdx.x) & 63;
     const bool upper = lane >= 32;
     const int pix = pix0 + static_cast<int>(threadIdx.x);
-    const int q = min(pix, npx - 1);
+    const size_t q = static_cast<size_t>(min(pix, npx - 1));
     const double px = grid[3 * q], py = grid[3 * q + 1], pz = grid[3 * q + 2];
     const double dk16 = 16.0 * dk;
     double accr[2] = {0.0, 0.0}, acci[2] = {0.0, 0.0};
@@ -691,7 +765,7 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     const int lane = static_cast<int>(threadIdx.x) & 63;
     float2 *win = s_win[threadIdx.x >> 6];
     const int pix = pix0 + static_cast<int>(threadIdx.x);
-    const int q = min(pix, npx - 1);
+    const size_t q = static_cast<size_t>(min(pix, npx - 1));
     // |pos - p|^2 = |pos'|^2 - 2 pos'.p' + |p'|^2 with both taken from an
     // origin inside the scene (no cancellation beyond the scene's own size):
     // four FP64 operations per chirp instead of six

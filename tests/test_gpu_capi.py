@@ -129,8 +129,8 @@ def test_capi_precision_golden_ragged(lib, orc, engine):
 
 
 def test_capi_rejects_sizes_beyond_int_range(lib):
-    """Pixel and per-pixel term counts are 32-bit: larger inputs fail cleanly
-    at the setter, before anything reads the (here too small) arrays."""
+    """Pixel counts above 2^30 and per-pixel term counts above 2^31 - 1 fail
+    cleanly at the setter, before anything reads the (here too small) arrays."""
     codes = status_codes()
     bad = codes["TDC_ERR_INVALID_ARGUMENT"]
     buf, p_buf = f32(np.zeros(16))
@@ -139,7 +139,9 @@ def test_capi_rejects_sizes_beyond_int_range(lib):
     assert lib.tdc_create(0, ctypes.byref(h)) == 0, lib.tdc_last_error()
     try:
         assert lib.tdc_set_focus_grid(h, p_buf, 65536, 65536) == bad
-        assert b"2^31" in lib.tdc_last_error()
+        assert b"2^30" in lib.tdc_last_error()
+        # one pixel beyond 2^30 (3 * index would pass 2^31 in 32-bit arithmetic)
+        assert lib.tdc_set_focus_grid(h, p_buf, 32769, 32768) == bad
         assert lib.tdc_set_focus_grid_f64(h, p_dbuf, 46341, 46341) == bad
         assert lib.tdc_set_raw_data(h, p_buf, p_buf, p_buf, p_buf, p_buf, p_buf, 65536, 32768,
                                     1e14, 77e9, p_buf) == bad
@@ -147,6 +149,7 @@ def test_capi_rejects_sizes_beyond_int_range(lib):
                                         1e14, 77e9) == bad
         assert b"2^31" in lib.tdc_last_error()
         # at the limit the setters accept the sizes (nothing is read before a start)
-        assert lib.tdc_set_focus_grid(h, p_buf, 46340, 46340) == 0
+        assert lib.tdc_set_focus_grid(h, p_buf, 32768, 32768) == 0
+        assert lib.tdc_set_focus_grid_f64(h, p_dbuf, 32768, 32768) == 0
     finally:
         lib.tdc_destroy(h)

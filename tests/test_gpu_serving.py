@@ -26,7 +26,7 @@ def oracle_img(orc, s, px=None):
 
 
 @pytest.mark.parametrize("form,storage,tol", [("precision", "c64", 1e-4), ("precision", "bf16", 2e-2),
-                                              ("precision", "f16", 3e-3), ("plain", "c64", 2e-3)])
+                                              ("precision", "f16", 3e-3), ("plain", "c64", 1e-4)])
 def test_single_job_parity(tdc, orc, form, storage, tol):
     s = job_scene(1, 200, 256, 70)
     srv = tdc.Server(0, 8e9)

@@ -137,7 +137,7 @@ skipped_any = []
 def test_random_dropin(tdc, orc, case):
     """The plain form (the drop-in) on random inputs, without and with the
     Doppler window: the image within the plain form's tolerances of the FP64
-    oracle (2e-3; 5e-3 with the window, whose band edge FP32 and FP64 may put
+    oracle (1e-4; 5e-3 with the window, whose band edge FP32 and FP64 may put
     a term on either side of), the skip counts equal to the oracle's skip
     test, the path taken as the inputs dictate, and start() bitwise equal to
     the resident pass."""
@@ -171,7 +171,7 @@ def test_random_dropin(tdc, orc, case):
             e = float(orc.rel_l2(img, ref))
         print(f"  window {'on ' if window else 'off'}: rel-L2 {e:.2e}, doppler "
               f"{st['doppler_active']}, skipped {st['skipped_chunks']}", flush=True)
-        assert e <= (5e-3 if window else 2e-3)
+        assert e <= (5e-3 if window else 1e-4)
         if window and st["doppler_active"]:
             processed = sum(
                 orc.chunk_processed(d["rawData"], d["sampleTimes"], d["position"],
