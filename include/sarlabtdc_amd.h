@@ -211,6 +211,30 @@ int tdc_set_launch(tdc_processor *h, int pixels_per_lane, int chirp_splits);
  * whole aperture resident and ignores the budget. */
 int tdc_set_device_budget(tdc_processor *h, unsigned long long bytes);
 
+/* ---- allocation-free entry point ------------------------------------------ */
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory at the point of the call (serving loops, harnesses that forbid
+ * hipMalloc): forms the image tdc_start forms for the precision form with
+ * the range-profile engine (FP64 inputs or tdc_set_form(TDC_FORM_PRECISION),
+ * at most 8192 samples per chirp, any raw storage) and writes it into the
+ * caller's host buffer image_iq [n_rows][n_cols][2] (image_floats floats).
+ * It calls no device allocator: every device buffer lies in one static
+ * device array of TDC_NOALLOC_SCRATCH_BYTES per GPU, held by each GPU the
+ * library runs on until the process ends. The chirp split count is the one
+ * tdc_start would choose, halved until the buffers fit (tdc_stats.chirp_splits
+ * reports it); with that count forced on tdc_start (tdc_set_launch) the two
+ * images are bit-identical. Inputs too large for the array, even with one
+ * split, give TDC_ERR_INVALID_ARGUMENT and a message with the bytes needed
+ * (config 3, 1024 chirps x 512 samples x 512^2 pixels, needs ~55 MB with 8
+ * splits). Calls on one GPU share the array and are serialised inside the
+ * process; the array keeps the last call's contents. Blocks until the image
+ * is in image_iq. The processor's prepared state (tdc_prepare) must be
+ * prepared again afterwards. */
+#define TDC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int tdc_start_noalloc(tdc_processor *h, float doppler_win_center, float doppler_bandwidth,
+                      int dop_centroid_win, int apply_range_win, int doppler_win_taper,
+                      float *image_iq, size_t image_floats);
+
 /* ---- serving form (additive; no counterpart in the reference) ------------- */
 /* A server owns one GPU stream and a scheduler thread. Jobs of any shape are
  * submitted at any time; the scheduler admits them in arrival order while

@@ -894,6 +894,69 @@ void plain_phase_tables(const std::vector<double> &k, std::vector<float> &dk,
 
 } // namespace tdc
 
+namespace tdc {
+
+void make_launch_geom(Geom &g, int npx, int npri, int ns, int ppl, int nsplit)
+{
+    make_geom(g, npx, npri, ns, ppl, nsplit);
+}
+
+int noalloc_inputs(tdc_processor *h, float dop_center, float dop_bw, bool centroid, bool apply_rw,
+                   bool taper, NoallocInputs &out)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (!h->have_raw32 && !h->have_raw64)
+        return fail(TDC_ERR_STATE, "setRawData must be called before start()");
+    if (!(h->form == TDC_FORM_PRECISION || h->have_raw64))
+        return fail(TDC_ERR_UNSUPPORTED,
+                    "the allocation-free entry point forms the precision form (FP64 inputs, or "
+                    "tdc_set_form(TDC_FORM_PRECISION))");
+    int rc = ensure_device(h);
+    if (rc != TDC_OK) return rc;
+    PrecisionInputs in;
+    rc = precision_inputs(h, dop_center, dop_bw, centroid, apply_rw, taper, in);
+    if (rc != TDC_OK) return rc;
+    if (in.engine != TDC_ENGINE_PROFILE)
+        return fail(TDC_ERR_UNSUPPORTED,
+                    "the allocation-free entry point runs the range-profile engine (at most " +
+                        std::to_string(kProfileMaxM / 2) + " samples per chirp)");
+    out.device = h->device;
+    out.stream = h->stream;
+    out.npri = h->npri;
+    out.ns = h->ns;
+    out.npx = h->rows * h->cols;
+    out.storage = h->storage;
+    const int ntiles = (out.npx + kBlock - 1) / kBlock;
+    out.nsplit = choose_split(ntiles, h->npri, h->force_split);
+    out.forced = h->force_split > 0;
+    out.raw = h->raw;
+    out.grid = in.gridp;
+    out.wr = std::move(in.wr);
+    out.pc = std::move(in.pc);
+    out.tabs = in.tabs;
+    out.pp = h->pp;
+    return TDC_OK;
+}
+
+void noalloc_finish(tdc_processor *h, const NoallocInputs &in, int nsplit, int workgroups,
+                    double end_to_end_ms)
+{
+    h->prepared = false;
+    tdc_stats st{};
+    st.end_to_end_ms = end_to_end_ms;
+    st.terms = static_cast<double>(in.npx) * in.npri * in.ns;
+    st.form = TDC_FORM_PRECISION;
+    st.raw_storage = in.storage;
+    st.chirp_splits = nsplit;
+    st.pixels_per_lane = 1;
+    st.workgroups = workgroups;
+    st.engine = TDC_ENGINE_PROFILE;
+    h->stats = st;
+    clear_last_error();
+}
+
+} // namespace tdc
+
 extern "C" {
 
 const char *tdc_last_error(void) { return t_err.c_str(); }

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <rocprofiler-sdk-roctx/roctx.h>
 
+struct tdc_processor;
+
 namespace tdc {
 
 // Chirps per skip-test chunk (reference tdc-lib/include/tuning.h:18).
@@ -241,6 +243,35 @@ bool precision_tables(const double *t, int ns, double f0, double krate,
 // dk[m] = k[m] - k_b rounded once to FP32.
 void plain_phase_tables(const std::vector<double> &k, std::vector<float> &dk,
                         std::vector<double> &kanchor);
+
+// Launch geometry of the one-job kernels (tdc_capi.cpp): pixels, chirps,
+// samples, pixels per lane, chirp splits.
+void make_launch_geom(Geom &g, int npx, int npri, int ns, int ppl, int nsplit);
+
+// Host side of the allocation-free entry point (tdc_noalloc.hip): the
+// precision form's range-profile inputs of a processor's current data, as
+// tdc_start computes them (tdc_capi.cpp).
+struct NoallocInputs {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int npri = 0, ns = 0, npx = 0, storage = 0;
+    int nsplit = 1;        // the chirp split count tdc_start chooses
+    bool forced = false;   // that count was set with tdc_set_launch
+    const float *raw = nullptr;   // host complex64 [npri][ns]
+    const double *grid = nullptr; // host FP64 [npx][3]
+    std::vector<float> wr;        // range window [ns]
+    std::vector<ChirpPos> pc;     // chirp positions from the scene origin, |.|^2 in pad
+    std::shared_ptr<const ProfileTables> tabs;
+    ProfileParams pp{};
+};
+// Validates the processor's inputs for the allocation-free entry point and
+// fills `in`; on failure returns the status and sets the message.
+int noalloc_inputs(tdc_processor *h, float dop_center, float dop_bw, bool centroid, bool apply_rw,
+                   bool taper, NoallocInputs &in);
+// Records the statistics of an allocation-free pass; the processor's
+// prepared state (which that pass did not use) must be prepared again.
+void noalloc_finish(tdc_processor *h, const NoallocInputs &in, int nsplit, int workgroups,
+                    double end_to_end_ms);
 
 // Host ingest (tdc_host.cpp). Host threads for passes over one image's inputs.
 constexpr unsigned kHostThreads = 8;

@@ -53,6 +53,8 @@ def test_null_handle_is_invalid_argument(lib):
     assert b"NULL" in lib.tdc_last_error()
     assert lib.tdc_start(None, ctypes.c_float(0), ctypes.c_float(1), 0, 0, 0) == bad
     assert lib.tdc_run(None, 1) == bad
+    assert lib.tdc_start_noalloc(None, ctypes.c_float(0), ctypes.c_float(1), 0, 0, 0, None,
+                                 ctypes.c_size_t(0)) == bad
     assert lib.tdc_destroy(None) == codes["TDC_OK"]
     assert lib.tdc_last_error() == b""
 
