@@ -99,6 +99,8 @@ typedef struct tdc_stats {
     int32_t per_chirp;        /* plain form, TDC_PER_CHIRP_* bits: what ran once per (pixel,
                                  chirp) because the inputs it depends on were constant
                                  within every chirp (stop-and-hop) */
+    int32_t profile_taps;         /* range-profile engine: taps of its kernel, else 0 */
+    int32_t profile_oversampling; /* range-profile engine: M / samples (2 or 4), else 0 */
 } tdc_stats;
 
 /* Message for the last failing call on this thread ("" when none). */
@@ -193,6 +195,14 @@ int tdc_fetch_image(tdc_processor *h);
 /* Statistics of the last run (timings from HIP events on the processor's
  * stream). */
 int tdc_get_stats(const tdc_processor *h, tdc_stats *out);
+
+/* Interpolation kernel of the range-profile engine: (taps, oversampling) =
+ * (8, 2), (10, 2), (12, 2) or (8, 4); (0, 0) = automatic, (8, 4) while the
+ * fourfold transform fits (chirps of up to 4096 samples), else (10, 2). The
+ * kernel's error against the direct sum is ~1e-5, ~1e-6, ~1e-7 and ~2e-6 of
+ * a chirp's profile (rel-L2); each two taps fewer save ~8 of ~90 instructions
+ * per (pixel, chirp), fourfold oversampling doubles the per-chirp table. */
+int tdc_set_profile_kernel(tdc_processor *h, int taps, int oversampling);
 
 /* Force a launch configuration for tuning and tests: pixels per lane
  * (0 = automatic; 1, 2 or 4 for the plain form, up to 8 for the precision

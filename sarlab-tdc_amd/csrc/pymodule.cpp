@@ -192,6 +192,7 @@ public:
     void setForm(const py::object &form) { proc_->setForm(parse_form(form)); }
     void setRawStorage(const py::object &s) { proc_->setRawStorage(parse_storage(s)); }
     void setEngine(const py::object &e) { proc_->setEngine(parse_engine(e)); }
+    void setProfileKernel(int taps, int over) { proc_->setProfileKernel(taps, over); }
     void setRangeWindow(const py::object &kind, double p0, double p1)
     {
         int k = parse_window(kind);
@@ -243,6 +244,8 @@ public:
                       : st.engine == TDC_ENGINE_VECTOR  ? "vector"
                       : st.engine == TDC_ENGINE_PROFILE ? "profile"
                                                         : "none";
+        d["taps"] = st.profile_taps;
+        d["oversampling"] = st.profile_oversampling;
         return d;
     }
 
@@ -403,6 +406,8 @@ PYBIND11_MODULE(_sarlabtdc, m)
         .def("setForm", &PyTdcProcessor::setForm, py::arg("form"))
         .def("setRawStorage", &PyTdcProcessor::setRawStorage, py::arg("storage"))
         .def("setEngine", &PyTdcProcessor::setEngine, py::arg("engine"))
+        .def("setProfileKernel", &PyTdcProcessor::setProfileKernel, py::arg("taps") = 0,
+             py::arg("oversampling") = 0)
         .def("setRangeWindow", &PyTdcProcessor::setRangeWindow, py::arg("kind"),
              py::arg("p0") = 0.0, py::arg("p1") = 0.0)
         .def("setLaunch", &PyTdcProcessor::setLaunch, py::arg("pixelsPerLane"),

@@ -215,6 +215,7 @@ struct tdc_processor {
     int win_kind = TDC_WIN_HAMMING;
     double win_p0 = 25.0 / 46.0, win_p1 = 0.0; // tuning.h:31
     int force_ppl = 0, force_split = 0;
+    int profile_taps = 0, profile_over = 0; // range-profile kernel (0, 0: automatic)
     uint64_t budget = 0; // device bytes for one image's buffers (0: no limit)
 
     // device buffers
@@ -434,7 +435,8 @@ int stage_inputs(tdc_processor *h)
         // window and storage rounding folded into the per-chirp transform
         TDC_HIP(tdc::launch_stage_profile_folded(
             h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
-            h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp, h->d_ptab.as<float2>(), s));
+            h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp, h->pp.taps,
+            h->d_ptab.as<float2>(), s));
         return TDC_OK;
     }
     TDC_HIP(tdc::launch_stage_precision(h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(),
@@ -642,11 +644,13 @@ int precision_inputs(tdc_processor *h, float dop_center, float dop_bw, bool cent
     if (engine == TDC_ENGINE_PROFILE) {
         if (h->force_ppl > 1)
             return fail(TDC_ERR_INVALID_ARGUMENT, "the range-profile engine runs one pixel per lane");
-        if (in.plen > tdc::kProfileMaxM)
-            return fail(TDC_ERR_UNSUPPORTED,
-                        "the range-profile engine takes at most " +
-                            std::to_string(tdc::kProfileMaxM / 2) + " samples per chirp");
-        in.tabs = tdc::profile_tables(ns, in.plen);
+        tdc::ProfileChoice pc;
+        if (!tdc::profile_choice(ns, h->profile_taps, h->profile_over, pc, why))
+            return fail(TDC_ERR_UNSUPPORTED, why);
+        in.plen = pc.M;
+        in.tabs = tdc::profile_tables(ns, in.plen, pc.taps, pc.over);
+        h->pp.taps = pc.taps;
+        h->pp.over = pc.over;
         in.mp = in.plen + tdc::kProfilePad;
         // chirp positions from an origin inside the scene (the midpoint of the
         // first and last pixels), with |pos - o|^2 in the pad
@@ -804,20 +808,21 @@ struct GaussLegendre64 {
 };
 } // namespace
 
-std::shared_ptr<const ProfileTables> profile_tables(int ns, int M)
+std::shared_ptr<const ProfileTables> profile_tables(int ns, int M, int taps, int over)
 {
-    // computed once per (ns, M) in the process: the tables depend on nothing else
+    // computed once per (ns, M, taps, over) in the process: the tables depend on nothing else
     static std::mutex mu;
-    static std::map<std::pair<int, int>, std::shared_ptr<const ProfileTables>> cache;
+    static std::map<std::array<int, 4>, std::shared_ptr<const ProfileTables>> cache;
+    const std::array<int, 4> key{ns, M, taps, over};
     {
         std::lock_guard<std::mutex> lk(mu);
-        auto it = cache.find({ns, M});
+        auto it = cache.find(key);
         if (it != cache.end()) return it->second;
     }
     static const GaussLegendre64 gl;
     constexpr int kNodes = GaussLegendre64::kNodes;
-    const double half = kProfileTaps / 2;
-    const double b = 0.75 * kPi / half;
+    const double half = taps / 2;
+    const double b = profile_bfactor(taps, over) * kPi / half;
     // psi_hat is even in nu and the integrand even in u: sum the nodes of one
     // half twice, once per distance |i - h|
     double wexp[kNodes / 2], unode[kNodes / 2];
@@ -844,7 +849,7 @@ std::shared_ptr<const ProfileTables> profile_tables(int ns, int M)
         tab->twiddle[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
     }
     std::lock_guard<std::mutex> lk(mu);
-    return cache.emplace(std::make_pair(ns, M), std::move(tab)).first->second;
+    return cache.emplace(key, std::move(tab)).first->second;
 }
 
 bool precision_tables(const double *t, int ns, double f0, double krate,
@@ -951,6 +956,8 @@ void noalloc_finish(tdc_processor *h, const NoallocInputs &in, int nsplit, int w
     st.pixels_per_lane = 1;
     st.workgroups = workgroups;
     st.engine = TDC_ENGINE_PROFILE;
+    st.profile_taps = in.pp.taps;
+    st.profile_oversampling = in.pp.over;
     h->stats = st;
     clear_last_error();
 }
@@ -1170,6 +1177,19 @@ int tdc_set_focus_grid_f64(tdc_processor *h, const double *grid, int n_rows, int
     return ok();
 }
 
+int tdc_set_profile_kernel(tdc_processor *h, int taps, int oversampling)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    if (!((taps == 0 && oversampling == 0) || tdc::profile_kernel_valid(taps, oversampling)))
+        return fail(TDC_ERR_INVALID_ARGUMENT,
+                    "the range-profile kernels are (taps, oversampling) = (8, 2), (10, 2), "
+                    "(12, 2) and (8, 4), or (0, 0) for the automatic choice");
+    h->profile_taps = taps;
+    h->profile_over = oversampling;
+    h->prepared = false;
+    return ok();
+}
+
 int tdc_set_device_budget(tdc_processor *h, unsigned long long bytes)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
@@ -1302,6 +1322,8 @@ int finish_stats(tdc_processor *h, double main_ms, double total_ms, int wgs)
     st.pixels_per_lane = h->ppl;
     st.workgroups = wgs;
     st.engine = h->run_form == TDC_FORM_PRECISION ? h->run_engine : 0;
+    st.profile_taps = st.engine == TDC_ENGINE_PROFILE ? h->pp.taps : 0;
+    st.profile_oversampling = st.engine == TDC_ENGINE_PROFILE ? h->pp.over : 0;
     st.per_chirp = 0;
     if (h->run_form == TDC_FORM_PLAIN && h->chirp_path)
         st.per_chirp = TDC_PER_CHIRP_RANGE | (h->doppler ? TDC_PER_CHIRP_WINDOW : 0);
@@ -1495,7 +1517,8 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
         if (in.engine == TDC_ENGINE_PROFILE) {
             TDC_HIP(tdc::launch_stage_profile_folded(
                 h->storage, h->r_raw[set].as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
-                h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp, h->r_tab[set].as<float2>(), s));
+                h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp, h->pp.taps,
+                h->r_tab[set].as<float2>(), s));
             TDC_HIP(tdc::launch_profile(h->r_tab[set].as<float2>(),
                                         h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),
                                         h->pp, gs, h->d_part.as<double2>(), s, &w));

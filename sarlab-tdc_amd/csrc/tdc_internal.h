@@ -101,10 +101,26 @@ struct DopParams {
 // (re-centred on sample h = ns / 2) at the frequency phi: the chirp's range
 // profile. It is evaluated as a type-2 non-uniform FFT: once per chirp the
 // samples, divided by the kernel's Fourier transform, go through an
-// M-point FFT (M >= 2 ns, a power of two); each pixel then sums
-// kProfileTaps table entries around phi * M weighted by the Gaussian kernel
-// exp(-b u^2), b = 0.75 pi / (kProfileTaps / 2).
-constexpr int kProfileTaps = 12;
+// M-point FFT (M >= over * ns, a power of two, `over` the oversampling);
+// each pixel then sums `taps` table entries around phi * M weighted by the
+// Gaussian kernel exp(-b u^2), |u| <= taps / 2, b = k pi / (taps / 2) with k
+// chosen per kernel for the smallest error. Built kernels (taps, over) and
+// their error against the direct sum (rel-L2 of one chirp's profile at
+// random frequencies, 512 Hann-windowed samples): (12, 2) 6.3e-8,
+// (10, 2) 8.0e-7, (8, 2) 1.1e-5, (8, 4) 2.3e-6 -- fourfold oversampling buys
+// the accuracy of about two more taps for twice the table.
+constexpr int kProfileTaps = 12; // the widest kernel
+inline bool profile_kernel_valid(int taps, int over)
+{
+    return (over == 2 && (taps == 8 || taps == 10 || taps == 12)) || (over == 4 && taps == 8);
+}
+// k of the kernel (taps, over); tdc_tiles.h holds the device constants of
+// exp(-b u^2) generated from the same values
+inline double profile_bfactor(int taps, int over)
+{
+    if (over == 4) return 0.88;
+    return taps == 12 ? 0.79 : 0.80;
+}
 // The FFT of one chirp runs in the LDS, of which a gfx950 workgroup may
 // address 160 KiB: radix-2 Stockham between two buffers (2 M entries) up to
 // M = 8192, in place in one buffer (M entries, 128 KiB) for M = 16384 (chirps
@@ -120,12 +136,48 @@ struct ProfileParams {
     double ox, oy, oz; // origin the chirp positions are given from (a pixel of the grid)
     int M;     // FFT length
     int Mp;    // table row length, M + kProfilePad
+    int taps;  // kernel width
+    int over;  // oversampling, M >= over * ns
 };
-inline int profile_length(int ns)
+inline int profile_length(int ns, int over = 2)
 {
     int m = 16;
-    while (m < 2 * ns) m *= 2;
+    while (m < over * static_cast<int64_t>(ns)) m *= 2;
     return m;
+}
+// The range-profile kernel for chirps of ns samples: `taps` and `over` as
+// requested (0, 0: automatic -- 8 taps with fourfold oversampling while the
+// transform fits, M <= kProfileMaxM, else 10 taps with twofold); false with a
+// reason when the request is not a built kernel or its transform is too long.
+struct ProfileChoice {
+    int taps = 0, over = 0, M = 0;
+};
+inline bool profile_choice(int ns, int taps, int over, ProfileChoice &c, std::string &why)
+{
+    if (taps == 0 && over == 0) {
+        if (profile_length(ns, 4) <= kProfileMaxM) {
+            taps = 8;
+            over = 4;
+        } else {
+            taps = 10;
+            over = 2;
+        }
+    }
+    if (!profile_kernel_valid(taps, over)) {
+        why = "the range-profile engine's kernels are (taps, oversampling) = (8, 2), (10, 2), "
+              "(12, 2) and (8, 4)";
+        return false;
+    }
+    const int M = profile_length(ns, over);
+    if (M > kProfileMaxM) {
+        why = "the range-profile engine takes at most " + std::to_string(kProfileMaxM / over) +
+              " samples per chirp with " + std::to_string(over) + "-fold oversampling";
+        return false;
+    }
+    c.taps = taps;
+    c.over = over;
+    c.M = M;
+    return true;
 }
 
 // Kernel launch wrappers (defined in tdc_kernels.hip). All enqueue on `s`.
@@ -195,16 +247,18 @@ hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, in
                                  double2 *acc, float2 *img, hipStream_t s);
 // Range-profile tables [npri][Mp] from the staged raw samples (in `storage`
 // format, range window folded in); corr[i] = 1 / psi_hat((i - h) / M);
-// twiddle[k] = exp(+j 2 pi k / M), k < M / 2.
+// twiddle[k] = exp(+j 2 pi k / M), k < M / 2; rows rotated by taps / 2 - 1
+// bins for a kernel of `taps` taps.
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
-                                const float2 *twiddle, int npri, int ns, int M, int Mp,
+                                const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
                                 float2 *table, hipStream_t s);
 // The same tables from the complex64 input: each sample times the range
 // window wr[i], rounded as `storage` stores it (bit-identical to staging it
 // with launch_stage_precision first).
 hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
                                        const float *corr, const float2 *twiddle, int npri,
-                                       int ns, int M, int Mp, float2 *table, hipStream_t s);
+                                       int ns, int M, int Mp, int taps, float2 *table,
+                                       hipStream_t s);
 // `cpos`: chirp positions minus the origin (pp.ox, oy, oz), with |pos - o|^2 in pad.
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
                           const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,
@@ -225,16 +279,16 @@ class Range {
 // The thread-local message returned by tdc_last_error().
 int set_last_error(int code, const std::string &msg);
 void clear_last_error();
-// Host tables of the range-profile engine for ns samples and an M-point FFT:
-// corr[i] = 1 / psi_hat((i - ns/2) / M), psi_hat the Fourier transform of the
-// truncated Gaussian exp(-b u^2), |u| <= taps/2 (64-point Gauss-Legendre
-// quadrature in FP64); twiddle[k] = exp(+j 2 pi k / M), k < M / 2. Shared,
-// computed once per (ns, M).
+// Host tables of the range-profile engine for ns samples, an M-point FFT and
+// the kernel (taps, over): corr[i] = 1 / psi_hat((i - ns/2) / M), psi_hat the
+// Fourier transform of the truncated Gaussian exp(-b u^2), |u| <= taps/2
+// (64-point Gauss-Legendre quadrature in FP64); twiddle[k] = exp(+j 2 pi k /
+// M), k < M / 2. Shared, computed once per (ns, M, taps, over).
 struct ProfileTables {
     std::vector<float> corr;
     std::vector<float2> twiddle;
 };
-std::shared_ptr<const ProfileTables> profile_tables(int ns, int M);
+std::shared_ptr<const ProfileTables> profile_tables(int ns, int M, int taps, int over);
 bool precision_tables(const double *t, int ns, double f0, double krate,
                       std::vector<double> &kanchor, double &dk, std::string &why);
 // Phase tables of the plain form from the per-sample wavenumbers

@@ -551,14 +551,15 @@ __device__ __forceinline__ void profile_sample(const void *src, const float *wr,
 // One workgroup per chirp: b[(i - h) mod M] = c_i * corr[i] (zero elsewhere),
 // then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-2 Stockham passes
 // between two LDS buffers (the output comes out in natural order), written
-// shifted by profile::kLead bins, row[j] = B[(j - kLead) mod M], over M +
+// shifted by lead = taps / 2 - 1 bins, row[j] = B[(j - lead) mod M], over M +
 // kProfilePad entries: a pixel whose bin position has integer part f reads
-// its taps B[f - kLead .. f - kLead + 11] as row[f .. f + 11], which never
-// wrap.
+// its taps B[f - lead .. f - lead + taps - 1] as row[f .. f + taps - 1],
+// which never wrap.
 template <class RAW, int BLOCK, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
-    const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
+    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead,
+    float2 *__restrict__ table)
 {
     extern __shared__ float2 buf[]; // 2 M entries, sized at launch
     const int n = static_cast<int>(blockIdx.x);
@@ -593,7 +594,7 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
         y = tmp;
     }
     float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - profile::kLead) & mask];
+    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - lead) & mask];
 }
 
 // The same transform for chirps whose two Stockham buffers would not fit the
@@ -603,7 +604,8 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
 template <class RAW, int BLOCK, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
     const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
-    const float2 *__restrict__ twiddle, int ns, int M, int Mp, float2 *__restrict__ table)
+    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead,
+    float2 *__restrict__ table)
 {
     extern __shared__ float2 x[]; // M entries, sized at launch
     const int n = static_cast<int>(blockIdx.x);
@@ -639,9 +641,10 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
         __syncthreads();
     }
     float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - profile::kLead) & mask];
+    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - lead) & mask];
 }
 
+template <int TAPS, int OVER>
 __global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restrict__ table,
                                                          const ChirpPos *__restrict__ cpos,
                                                          const double *__restrict__ grid,
@@ -653,8 +656,8 @@ __global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restric
     if (tile >= g.ntiles) return;
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
-    profile_tile(table, cpos, grid, pp, g.npx, n0, n1, tile * kBlock,
-                 out + static_cast<size_t>(split) * g.pstride);
+    profile_tile<TAPS, OVER>(table, cpos, grid, pp, g.npx, n0, n1, tile * kBlock,
+                             out + static_cast<size_t>(split) * g.pstride);
 }
 
 // Sum of the per-split partial images in split order.
@@ -1000,7 +1003,7 @@ namespace {
 template <class RAW, int BLOCK, bool FOLD>
 hipError_t stage_profile_launch(const void *src, const float *wr, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp,
-                                float2 *table, hipStream_t s)
+                                int lead, float2 *table, hipStream_t s)
 {
     const bool inplace = M > kProfileStockhamMaxM;
     const size_t lds = (inplace ? 1 : 2) * sizeof(float2) * static_cast<size_t>(M);
@@ -1015,10 +1018,10 @@ hipError_t stage_profile_launch(const void *src, const float *wr, const float *c
     }
     if (inplace)
         hipLaunchKernelGGL((stage_profile_inplace_kernel<RAW, BLOCK, FOLD>), dim3(npri),
-                           dim3(BLOCK), lds, s, src, wr, corr, twiddle, ns, M, Mp, table);
+                           dim3(BLOCK), lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, table);
     else
         hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD>), dim3(npri), dim3(BLOCK), lds,
-                           s, src, wr, corr, twiddle, ns, M, Mp, table);
+                           s, src, wr, corr, twiddle, ns, M, Mp, lead, table);
     return hipGetLastError();
 }
 
@@ -1027,30 +1030,35 @@ hipError_t stage_profile_launch(const void *src, const float *wr, const float *c
 // LDS busy across the passes' barriers; short ones run 256.
 template <class RAW, bool FOLD>
 hipError_t stage_profile_raw(const void *src, const float *wr, const float *corr,
-                             const float2 *twiddle, int npri, int ns, int M, int Mp,
+                             const float2 *twiddle, int npri, int ns, int M, int Mp, int lead,
                              float2 *table, hipStream_t s)
 {
     if (M >= kProfileWideM)
         return stage_profile_launch<RAW, 1024, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp,
-                                                     table, s);
-    return stage_profile_launch<RAW, kBlock, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table,
-                                                   s);
+                                                     lead, table, s);
+    return stage_profile_launch<RAW, kBlock, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
+                                                   table, s);
 }
 
 template <bool FOLD>
 hipError_t stage_profile_any(int storage, const void *src, const float *wr, const float *corr,
-                             const float2 *twiddle, int npri, int ns, int M, int Mp,
+                             const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
                              float2 *table, hipStream_t s)
 {
-    if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || Mp < M + kProfileTaps - 1)
+    if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || taps < 2 || taps > kProfileTaps ||
+        Mp < M + taps - 1)
         return hipErrorInvalidValue;
+    const int lead = taps / 2 - 1;
     switch (storage) {
     case 0:
-        return stage_profile_raw<RawC64, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table, s);
+        return stage_profile_raw<RawC64, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead, table,
+                                               s);
     case 1:
-        return stage_profile_raw<RawF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table, s);
+        return stage_profile_raw<RawF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead, table,
+                                               s);
     case 2:
-        return stage_profile_raw<RawBF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, table, s);
+        return stage_profile_raw<RawBF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
+                                                table, s);
     default:
         return hipErrorInvalidValue;
     }
@@ -1059,18 +1067,20 @@ hipError_t stage_profile_any(int storage, const void *src, const float *wr, cons
 } // namespace
 
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
-                                const float2 *twiddle, int npri, int ns, int M, int Mp,
+                                const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
                                 float2 *table, hipStream_t s)
 {
-    return stage_profile_any<false>(storage, staged, nullptr, corr, twiddle, npri, ns, M, Mp, table,
-                                    s);
+    return stage_profile_any<false>(storage, staged, nullptr, corr, twiddle, npri, ns, M, Mp, taps,
+                                    table, s);
 }
 
 hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
                                        const float *corr, const float2 *twiddle, int npri,
-                                       int ns, int M, int Mp, float2 *table, hipStream_t s)
+                                       int ns, int M, int Mp, int taps, float2 *table,
+                                       hipStream_t s)
 {
-    return stage_profile_any<true>(storage, raw, wr, corr, twiddle, npri, ns, M, Mp, table, s);
+    return stage_profile_any<true>(storage, raw, wr, corr, twiddle, npri, ns, M, Mp, taps, table,
+                                   s);
 }
 
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
@@ -1079,8 +1089,20 @@ hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const doubl
 {
     const int nb = correlate_grid(g);
     *workgroups = nb;
-    hipLaunchKernelGGL(profile_kernel, dim3(nb), dim3(kBlock), 0, s, table, cpos, grid, pp, g,
-                       out);
+    if (pp.over == 4 && pp.taps == 8)
+        hipLaunchKernelGGL((profile_kernel<8, 4>), dim3(nb), dim3(kBlock), 0, s, table, cpos, grid,
+                           pp, g, out);
+    else if (pp.over == 2 && pp.taps == 8)
+        hipLaunchKernelGGL((profile_kernel<8, 2>), dim3(nb), dim3(kBlock), 0, s, table, cpos, grid,
+                           pp, g, out);
+    else if (pp.over == 2 && pp.taps == 10)
+        hipLaunchKernelGGL((profile_kernel<10, 2>), dim3(nb), dim3(kBlock), 0, s, table, cpos, grid,
+                           pp, g, out);
+    else if (pp.over == 2 && pp.taps == 12)
+        hipLaunchKernelGGL((profile_kernel<12, 2>), dim3(nb), dim3(kBlock), 0, s, table, cpos, grid,
+                           pp, g, out);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -217,6 +217,7 @@ struct Job {
     // precision jobs: range-profile engine (chirps of at most 8192 samples) or
     // matrix engine
     bool profile = false;
+    tdc::ProfileChoice pk; // its kernel and FFT length
     tdc::ProfileParams pp{};
     // device arena layout: [upload region | staged records or raw | partials]
     size_t off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
@@ -300,8 +301,9 @@ void plan(Job &j)
     const size_t npx = static_cast<size_t>(d.n_rows) * d.n_cols;
     j.npx = static_cast<int>(npx);
     const bool prec = d.form == TDC_FORM_PRECISION;
-    const int plen = tdc::profile_length(d.n_samples);
-    j.profile = prec && plen <= tdc::kProfileMaxM;
+    std::string why;
+    j.profile = prec && tdc::profile_choice(d.n_samples, 0, 0, j.pk, why);
+    const int plen = j.pk.M;
     j.pair_cost = j.profile ? kProfilePairCost : (prec ? 1.0 : kPlainTermCost) * d.n_samples;
     j.tile_px = tdc::kServeTilePixels;
     j.npt = (j.npx + j.tile_px - 1) / j.tile_px;
@@ -434,7 +436,7 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
             // origin inside the scene with their squared norms, the kernel's
             // correction and the FFT's twiddles
             const int ns = d.n_samples;
-            const int plen = tdc::profile_length(ns);
+            const int plen = j.pk.M;
             const size_t last = 3 * (static_cast<size_t>(j.npx) - 1);
             double o[3];
             for (int a = 0; a < 3; ++a) {
@@ -445,7 +447,7 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
                 const double x = hc[n].x - o[0], y = hc[n].y - o[1], z = hc[n].z - o[2];
                 hc[n] = tdc::ChirpPos{x, y, z, std::fma(z, z, std::fma(y, y, x * x))};
             }
-            const auto tabs = tdc::profile_tables(ns, plen);
+            const auto tabs = tdc::profile_tables(ns, plen, j.pk.taps, j.pk.over);
             std::memcpy(h + j.off_pcorr, tabs->corr.data(), 4 * tabs->corr.size());
             std::memcpy(h + j.off_ptw, tabs->twiddle.data(), sizeof(float2) * tabs->twiddle.size());
             const double dt = ns > 1 ? (d.sample_times[ns - 1] - d.sample_times[0]) / (ns - 1) : 0.0;
@@ -456,6 +458,8 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
             j.pp.oz = o[2];
             j.pp.M = plen;
             j.pp.Mp = plen + tdc::kProfilePad;
+            j.pp.taps = j.pk.taps;
+            j.pp.over = j.pk.over;
         }
     } else {
         // the plain form's phase tables (the processor's, from FP64 inputs) and
@@ -540,7 +544,7 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
             return tdc::launch_stage_profile(
                 d.raw_storage, staged, reinterpret_cast<const float *>(j.arena + j.off_pcorr),
                 reinterpret_cast<const float2 *>(j.arena + j.off_ptw), d.n_pri, d.n_samples,
-                j.pp.M, j.pp.Mp, reinterpret_cast<float2 *>(j.arena + j.off_ptab), ss);
+                j.pp.M, j.pp.Mp, j.pp.taps, reinterpret_cast<float2 *>(j.arena + j.off_ptab), ss);
         return tdc::launch_stage_matrix(d.raw_storage, staged, d.n_pri, d.n_samples,
                                         tdc::matrix_parts(d.raw_storage), j.arena + j.off_atab,
                                         reinterpret_cast<float *>(j.arena + j.off_ascale), ss);

@@ -714,40 +714,68 @@ __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
 }
 
 // Range-profile engine over one tile (one pixel per thread, chirps n0..n1):
-// per (pixel, chirp) the FP64 range R, the kernel position phi * M - taps/2
-// split into an integer bin m0 and a fraction g in (-1, 0], the twelve table
-// entries m0 .. m0 + 11 weighted by exp(-b (g - s)^2), s = t - 6 (computed as
-// exp(-b g^2) exp(2 b g)^s exp(-b s^2): three exp2 per pair, two multiplies
-// per tap), and the FP64 anchor exp(j 2 pi k_h R); chirps accumulate in FP64.
+// per (pixel, chirp) the FP64 range R, the bin position u = frac(dk R) M
+// split into an integer bin f and a fraction g, the TAPS table entries around
+// u weighted by the Gaussian kernel (two exp2 per pair, two multiplies per
+// tap), and the FP64 anchor exp(j 2 pi k_h R); chirps accumulate in FP32
+// blocks of kProfileBlock chirps and the blocks in FP64.
 namespace profile {
 // A pixel's bin position u = frac(dk R) M has integer part f and fraction g;
-// tap t (0 .. 11) is bin f - kLead + t, at distance g + kLead - t from u, and
-// weighs exp(-b (g + kLead - t)^2) = up0 e1^t kC[t] with
-// up0 = exp(-b g^2 - 2 kLead b g), e1 = exp(2 b g), kC[t] = exp(-b (kLead - t)^2)
+// tap t (0 .. TAPS - 1) is bin f - lead + t, lead = TAPS / 2 - 1, at distance
+// g + lead - t from u, and weighs exp(-b (g + lead - t)^2) = up0 e1^t c[t] with
+// up0 = exp(-b g^2 - 2 lead b g), e1 = exp(2 b g), c[t] = exp(-b (lead - t)^2)
 // (fast Gaussian gridding: two exponentials per pixel and chirp, then one
-// product per tap).
-constexpr int kLead = kProfileTaps / 2 - 1;
+// product per tap); b = k pi / (TAPS / 2).
 constexpr int kProfileBlock = 8;
-constexpr float kE0 = -0.5665450177283992F; // -b log2(e), b = 0.75 pi / 6
-constexpr float kE1 = 1.1330900354567983F;  // 2 b log2(e)
-constexpr float kC[kProfileTaps] = {5.449088911519817e-05F, 0.0018674427317079893F,
-                                    0.029179416423313852F,  0.20787957635076193F,
-                                    0.6752319066557773F,    1.0F,
-                                    0.6752319066557773F,    0.20787957635076193F,
-                                    0.029179416423313852F,  0.0018674427317079893F,
-                                    5.449088911519817e-05F, 7.249472515987942e-07F};
+// Constants of the built kernels (taps, oversampling), generated from
+// b = k pi / (taps / 2) with the k of profile_bfactor: kE0 = -b log2(e),
+// kE1 = 2 b log2(e), kC[t] = exp(-b (taps / 2 - 1 - t)^2).
+template <int TAPS, int OVER> struct Kernel;
+template <> struct Kernel<8, 2> { // k = 0.80
+    static constexpr float kE0 = -0.9064720283654387F;
+    static constexpr float kE1 = 1.8129440567308774F;
+    static constexpr float kC[8] = {0.003500439396667034F, 0.08100259215794314F,
+                                    0.5334880910911033F,   1.0F,
+                                    0.5334880910911033F,   0.08100259215794314F,
+                                    0.003500439396667034F, 4.305223158055476e-05F};
+};
+template <> struct Kernel<10, 2> { // k = 0.80
+    static constexpr float kE0 = -0.7251776226923511F;
+    static constexpr float kE1 = 1.4503552453847022F;
+    static constexpr float kC[10] = {0.00032151151668886733F, 0.010846710538160161F,
+                                     0.133905721399763F,      0.6049225627642709F,
+                                     1.0F,                    0.6049225627642709F,
+                                     0.133905721399763F,      0.010846710538160161F,
+                                     0.00032151151668886733F, 3.487342356208991e-06F};
+};
+template <> struct Kernel<12, 2> { // k = 0.79
+    static constexpr float kE0 = -0.5967607520072472F;
+    static constexpr float kE1 = 1.1935215040144944F;
+    static constexpr float kC[12] = {3.227957702166608e-05F, 0.0013357150352831012F,
+                                     0.02416651468988259F,   0.19117383558744308F,
+                                     0.6612369490950528F,    1.0F,
+                                     0.6612369490950528F,    0.19117383558744308F,
+                                     0.02416651468988259F,   0.0013357150352831012F,
+                                     3.227957702166608e-05F, 3.410798652928483e-07F};
+};
+template <> struct Kernel<8, 4> { // k = 0.88
+    static constexpr float kE0 = -0.9971192312019825F;
+    static constexpr float kE1 = 1.994238462403965F;
+    static constexpr float kC[8] = {0.0019885422346150906F, 0.0630011981095824F,
+                                    0.5009993958482307F,    1.0F,
+                                    0.5009993958482307F,    0.0630011981095824F,
+                                    0.0019885422346150906F, 1.5754159369011635e-05F};
+};
 } // namespace profile
 
 // Taps from a window of the table row staged by the wave itself: a wave's
-// 64 pixels are close together, so their first taps m0 usually lie within
-// kWinReach bins of lane 0's. Each lane then loads one entry of the 64-entry
-// window starting kWinReach bins below lane 0's m0 (one coalesced load) into
-// the wave's own LDS slot, and reads its twelve taps from there (LDS delivers
-// twice the bytes per clock of the vector L1, which the twelve gathered taps
-// per lane would otherwise saturate). A wave with a lane outside the window
+// 64 pixels are close together, so their first taps usually lie within
+// (64 - TAPS) / 2 bins of lane 0's. Each lane then loads one entry of the
+// 64-entry window starting that far below lane 0's first tap (one coalesced
+// load) into the wave's own LDS slot, and reads its taps from there (LDS
+// delivers twice the bytes per clock of the vector L1, which the gathered
+// taps would otherwise saturate). A wave with a lane outside the window
 // gathers from the table directly.
-constexpr int kWinReach = (64 - kProfileTaps) / 2;
-
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -755,6 +783,7 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <int TAPS, int OVER>
 __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                                              const ChirpPos *__restrict__ cpos,
                                              const double *__restrict__ grid,
@@ -790,30 +819,32 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
             // in the row; window indices are masked to the row
             const unsigned f = static_cast<unsigned>(static_cast<int>(u));
             const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
-            const unsigned lo = __builtin_amdgcn_readfirstlane(f) - kWinReach;
+            constexpr unsigned kReach = (64 - TAPS) / 2;
+            const unsigned lo = __builtin_amdgcn_readfirstlane(f) - kReach;
             const unsigned off = f - lo;
-            float2 v[kProfileTaps];
-            if (__all(off <= 64U - kProfileTaps)) {
+            float2 v[TAPS];
+            if (__all(off <= 64U - TAPS)) {
                 const float2 e = row[(lo + lane) & mask];
                 wave_lds_sync(); // the previous chirp's reads of the slot are done
                 win[lane] = e;
                 wave_lds_sync();
 #pragma unroll
-                for (int t = 0; t < kProfileTaps; ++t) v[t] = win[off + t];
+                for (int t = 0; t < TAPS; ++t) v[t] = win[off + t];
             } else {
                 const float2 *r = row + f;
 #pragma unroll
-                for (int t = 0; t < kProfileTaps; ++t) v[t] = r[t];
+                for (int t = 0; t < TAPS; ++t) v[t] = r[t];
             }
-            // up0 = exp2(g (kE0 g - kLead kE1)), e1 = exp2(kE1 g): for g in
-            // [0, 1) up0 lies in (0.013, 1] and up0 e1^11 below 6e3
-            float up = __builtin_amdgcn_exp2f(
-                g * fmaf(profile::kE0, g, -profile::kLead * profile::kE1));
-            const float e1 = __builtin_amdgcn_exp2f(profile::kE1 * g);
+            // up0 = exp2(g (kE0 g - lead kE1)), e1 = exp2(kE1 g): for g in
+            // [0, 1) up0 lies in (0.013, 1] and up0 e1^(TAPS - 1) below 6e3
+            using K = profile::Kernel<TAPS, OVER>;
+            constexpr int kLead = TAPS / 2 - 1;
+            float up = __builtin_amdgcn_exp2f(g * fmaf(K::kE0, g, -kLead * K::kE1));
+            const float e1 = __builtin_amdgcn_exp2f(K::kE1 * g);
             float ar = 0.0F, ai = 0.0F;
 #pragma unroll
-            for (int t = 0; t < kProfileTaps; ++t) {
-                const float w = up * profile::kC[t];
+            for (int t = 0; t < TAPS; ++t) {
+                const float w = up * K::kC[t];
                 ar = fmaf(w, v[t].x, ar);
                 ai = fmaf(w, v[t].y, ai);
                 up *= e1;

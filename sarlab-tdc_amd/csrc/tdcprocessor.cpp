@@ -115,6 +115,10 @@ void TdcProcessor::setFocusGridF64(double const *focusGrid, int nRows, int nCols
 void TdcProcessor::setForm(int form) { check(tdc_set_form(h_, form)); }
 void TdcProcessor::setRawStorage(int storage) { check(tdc_set_raw_storage(h_, storage)); }
 void TdcProcessor::setEngine(int engine) { check(tdc_set_engine(h_, engine)); }
+void TdcProcessor::setProfileKernel(int taps, int oversampling)
+{
+    check(tdc_set_profile_kernel(h_, taps, oversampling));
+}
 void TdcProcessor::setRangeWindow(int kind, double p0, double p1)
 {
     check(tdc_set_range_window(h_, kind, p0, p1));

@@ -332,6 +332,52 @@ def test_profile_engine_window_and_gather_paths(tdc, orc, spacing):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
+@pytest.mark.parametrize("taps,over,ns", [(8, 2, 300), (10, 2, 300), (12, 2, 300), (8, 4, 300),
+                                          (8, 4, 4096), (10, 2, 8192)])
+def test_profile_engine_kernels(tdc, orc, taps, over, ns):
+    """Every built interpolation kernel of the range-profile engine (taps,
+    oversampling) against the oracle, each at the longest chirps its transform
+    takes for the last two; the statistics name the kernel that ran."""
+    s = ragged_scene(npri=40, ns=ns, rows=9, cols=11, seed=taps * over + ns)
+    f = s.f64()
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"], f["modRate"],
+                    f["startFreq"])
+    p.setFocusGridF64(f["focusGrid"])
+    p.setEngine("profile")
+    p.setProfileKernel(taps, over)
+    p.start(0.0, 1e30, False, False, False)
+    st = p.stats()
+    assert (st["engine"], st["taps"], st["oversampling"]) == ("profile", taps, over)
+    check(orc, p.getFocusedImage(), oracle_f64(orc, s), PREC_TOL)
+
+
+def test_profile_kernel_choice(tdc):
+    """The automatic kernel: (8, 4) up to 4096 samples per chirp, (10, 2) above;
+    a kernel that is not built, or fourfold oversampling beyond 4096 samples,
+    is refused."""
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    with pytest.raises(ValueError):
+        p.setProfileKernel(6, 4)
+    with pytest.raises(ValueError):
+        p.setProfileKernel(10, 4)
+    for ns, want in ((4096, (8, 4)), (4097, (10, 2))):
+        s = ragged_scene(npri=4, ns=ns, rows=2, cols=3, seed=ns)
+        f = s.f64()
+        p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"],
+                        f["modRate"], f["startFreq"])
+        p.setFocusGridF64(f["focusGrid"])
+        p.setProfileKernel(0, 0)
+        p.start(0.0, 1e30, False, False, False)
+        st = p.stats()
+        assert (st["taps"], st["oversampling"]) == want
+    p.setProfileKernel(8, 4)
+    with pytest.raises(Exception, match="at most 4096 samples"):
+        p.start(0.0, 1e30, False, False, False)
+
+
 def test_profile_engine_sample_limit(tdc, orc):
     """Beyond 8192 samples per chirp the range-profile engine declines with a
     message and the automatic choice takes the matrix engine."""
