@@ -356,6 +356,15 @@ def serve_leg(tdc, args):
             f"p99 {rec_c['p99_ms']:.2f} ms, {rec_c['gterms']:.1f} G terms/s over "
             f"{rec_c['makespan_s']:.2f} s (device busy {rec_c['device_busy_s']:.2f} s)")
         out["precision_capacity"] = rec_c
+    if args.serve_saturation_rate > 0:
+        # the same jobs offered faster than the device forms them: the makespan
+        # is the server's, so the executed fraction is the device-bound one
+        rec_s, _ = serve_stream(tdc, scenes, args.serve_saturation_rate, "precision", seed=2)
+        log(f"serving precision at {rec_s['rate']:.0f}/s: p50 {rec_s['p50_ms']:.2f} ms, "
+            f"p99 {rec_s['p99_ms']:.2f} ms, {rec_s['gterms']:.1f} G terms/s over "
+            f"{rec_s['makespan_s']:.3f} s (device busy {rec_s['device_busy_s']:.3f} s, executed "
+            f"{rec_s['fp32_peak_frac']:.3f} of the FP32 peak over the makespan)")
+        out["precision_saturation"] = rec_s
     n_plain = min(len(scenes), args.serve_plain_jobs)
     if n_plain > 0:
         rec_p, _ = serve_stream(tdc, scenes[:n_plain], args.serve_rate, "plain")
@@ -415,6 +424,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2400.0, help="offered jobs/s")
     ap.add_argument("--serve-capacity-rate", type=float, default=9600.0,
                     help="offered jobs/s of a second precision stream of the same jobs (0: skip)")
+    ap.add_argument("--serve-saturation-rate", type=float, default=100000.0,
+                    help="offered jobs/s of a third precision stream of the same jobs, above the "
+                         "device's capacity (0: skip)")
     ap.add_argument("--serve-plain-jobs", type=int, default=200,
                     help="jobs of the stream also served in the plain form")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -688,7 +700,7 @@ def main():
         out["p50_ms"] = round(pr["p50_ms"], 3)
         out["p99_ms"] = round(pr["p99_ms"], 3)
         target = STREAM_MIN_CHIRPS * PRI_S * 1e3
-        for k in ("precision", "precision_capacity", "plain"):
+        for k in ("precision", "precision_capacity", "precision_saturation", "plain"):
             if k in out["serving"]:
                 out["serving"][k]["p99_target_ms"] = target
                 out["serving"][k]["p99_target_met"] = bool(serving[k]["p99_ms"] <= target)

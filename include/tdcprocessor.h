@@ -16,9 +16,12 @@
  * Coordinates: the focus grid and the antenna positions share one local
  * frame; attitude quaternions (scalar first) rotate the body frame into it.
  *
- * Logging: spdlog is not a dependency. setLoggerSinks() takes sinks of the
- * minimal interface tdclog::sink below (the reference takes spdlog sinks,
- * tdcprocessor.h:128); an adapter to any logging library is a few lines.
+ * Logging: spdlog is not a dependency of the library. setLoggerSinks() takes
+ * sinks of the minimal interface tdclog::sink below; when the including
+ * translation unit can see <spdlog/spdlog.h>, the reference's overload taking
+ * std::vector<spdlog::sink_ptr> (tdcprocessor.h:128) is also declared, inline,
+ * forwarding each line to the spdlog sinks (an empty braced list {} is then
+ * ambiguous between the two overloads: pass a typed vector).
  */
 #ifndef SARLABTDC_AMD_TDCPROCESSOR_H
 #define SARLABTDC_AMD_TDCPROCESSOR_H
@@ -55,6 +58,32 @@ public:
 
 } // namespace tdclog
 
+#if defined(__has_include)
+#if __has_include(<spdlog/spdlog.h>)
+#include <spdlog/spdlog.h>
+#define SARLABTDC_AMD_HAVE_SPDLOG 1
+namespace tdclog {
+/* Forwards the processor's lines to one spdlog sink. */
+class spdlog_sink : public sink
+{
+public:
+    explicit spdlog_sink(spdlog::sink_ptr target) : target_(std::move(target)) {}
+    void log(level lvl, const std::string &logger, const std::string &msg) override
+    {
+        const spdlog::level::level_enum l = lvl == level::debug  ? spdlog::level::debug
+                                            : lvl == level::info ? spdlog::level::info
+                                            : lvl == level::warn ? spdlog::level::warn
+                                                                 : spdlog::level::err;
+        target_->log(spdlog::details::log_msg(logger, l, msg));
+    }
+
+private:
+    spdlog::sink_ptr target_;
+};
+} // namespace tdclog
+#endif
+#endif
+
 class TdcProcessor
 {
 public:
@@ -78,6 +107,16 @@ public:
 
     /* tdcprocessor.h:128 (sink type: see the note at the top of this file) */
     void setLoggerSinks(const std::vector<tdclog::sink_ptr> &sinks);
+#ifdef SARLABTDC_AMD_HAVE_SPDLOG
+    /* tdcprocessor.h:128 with the reference's spdlog sinks */
+    void setLoggerSinks(const std::vector<spdlog::sink_ptr> &sinks)
+    {
+        std::vector<tdclog::sink_ptr> wrapped;
+        for (const spdlog::sink_ptr &s : sinks)
+            wrapped.push_back(std::make_shared<tdclog::spdlog_sink>(s));
+        setLoggerSinks(wrapped);
+    }
+#endif
 
     /* tdcprocessor.h:135: nullptr before the first start() */
     float2 const *imageBuffer() const;

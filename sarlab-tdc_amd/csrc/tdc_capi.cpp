@@ -435,7 +435,7 @@ int stage_inputs(tdc_processor *h)
         // window and storage rounding folded into the per-chirp transform
         TDC_HIP(tdc::launch_stage_profile_folded(
             h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
-            h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp, h->pp.taps,
+            h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp, h->pp.taps, h->pp.reverse,
             h->d_ptab.as<float2>(), s));
         return TDC_OK;
     }
@@ -675,6 +675,8 @@ int precision_inputs(tdc_processor *h, float dop_center, float dop_bw, bool cent
         h->pp.kh = 2.0 * (f0 + krate * (t[0] + (ns / 2) * dt)) / kSpeedOfLight;
         h->pp.M = in.plen;
         h->pp.Mp = in.mp;
+        h->pp.dkm = std::fabs(h->dk) * in.plen;
+        h->pp.reverse = h->dk < 0.0 ? 1 : 0;
         h->ppl = 1;
     } else if (engine == TDC_ENGINE_MATRIX) {
         if (h->force_ppl > 1)
@@ -968,7 +970,7 @@ extern "C" {
 
 const char *tdc_last_error(void) { return t_err.c_str(); }
 
-int tdc_abi_version(void) { return 10400; }
+int tdc_abi_version(void) { return 20000; }
 
 int tdc_device_count(void)
 {
@@ -1517,7 +1519,7 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
         if (in.engine == TDC_ENGINE_PROFILE) {
             TDC_HIP(tdc::launch_stage_profile_folded(
                 h->storage, h->r_raw[set].as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
-                h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp, h->pp.taps,
+                h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp, h->pp.taps, h->pp.reverse,
                 h->r_tab[set].as<float2>(), s));
             TDC_HIP(tdc::launch_profile(h->r_tab[set].as<float2>(),
                                         h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),

@@ -136,8 +136,11 @@ struct ProfileParams {
     double ox, oy, oz; // origin the chirp positions are given from (a pixel of the grid)
     int M;     // FFT length
     int Mp;    // table row length, M + kProfilePad
-    int taps;  // kernel width
-    int over;  // oversampling, M >= over * ns
+    double dkm; // |dk| M: the bin position of a pixel at range R is frac(dkm R / M) M
+    int taps;   // kernel width
+    int over;   // oversampling, M >= over * ns
+    int reverse; // 1 for dk < 0 (down-chirps): table rows hold B[-m] instead of B[m]
+    int pad;
 };
 inline int profile_length(int ns, int over = 2)
 {
@@ -248,17 +251,17 @@ hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, in
 // Range-profile tables [npri][Mp] from the staged raw samples (in `storage`
 // format, range window folded in); corr[i] = 1 / psi_hat((i - h) / M);
 // twiddle[k] = exp(+j 2 pi k / M), k < M / 2; rows rotated by taps / 2 - 1
-// bins for a kernel of `taps` taps.
+// bins for a kernel of `taps` taps, and reversed (B[-m]) when `reverse`.
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
-                                float2 *table, hipStream_t s);
+                                int reverse, float2 *table, hipStream_t s);
 // The same tables from the complex64 input: each sample times the range
 // window wr[i], rounded as `storage` stores it (bit-identical to staging it
 // with launch_stage_precision first).
 hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
                                        const float *corr, const float2 *twiddle, int npri,
-                                       int ns, int M, int Mp, int taps, float2 *table,
-                                       hipStream_t s);
+                                       int ns, int M, int Mp, int taps, int reverse,
+                                       float2 *table, hipStream_t s);
 // `cpos`: chirp positions minus the origin (pp.ox, oy, oz), with |pos - o|^2 in pad.
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
                           const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,

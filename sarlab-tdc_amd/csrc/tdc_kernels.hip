@@ -554,11 +554,12 @@ __device__ __forceinline__ void profile_sample(const void *src, const float *wr,
 // shifted by lead = taps / 2 - 1 bins, row[j] = B[(j - lead) mod M], over M +
 // kProfilePad entries: a pixel whose bin position has integer part f reads
 // its taps B[f - lead .. f - lead + taps - 1] as row[f .. f + taps - 1],
-// which never wrap.
+// which never wrap. For down-chirps (`reverse`) the row holds
+// B[(lead - j) mod M]: the profile at frequency -u is read at u.
 template <class RAW, int BLOCK, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
-    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead,
+    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead, int reverse,
     float2 *__restrict__ table)
 {
     extern __shared__ float2 buf[]; // 2 M entries, sized at launch
@@ -594,7 +595,7 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
         y = tmp;
     }
     float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - lead) & mask];
+    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(reverse ? lead - j : j - lead) & mask];
 }
 
 // The same transform for chirps whose two Stockham buffers would not fit the
@@ -604,7 +605,7 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
 template <class RAW, int BLOCK, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
     const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
-    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead,
+    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead, int reverse,
     float2 *__restrict__ table)
 {
     extern __shared__ float2 x[]; // M entries, sized at launch
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
         __syncthreads();
     }
     float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(j - lead) & mask];
+    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(reverse ? lead - j : j - lead) & mask];
 }
 
 template <int TAPS, int OVER>
@@ -1003,7 +1004,7 @@ namespace {
 template <class RAW, int BLOCK, bool FOLD>
 hipError_t stage_profile_launch(const void *src, const float *wr, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp,
-                                int lead, float2 *table, hipStream_t s)
+                                int lead, int reverse, float2 *table, hipStream_t s)
 {
     const bool inplace = M > kProfileStockhamMaxM;
     const size_t lds = (inplace ? 1 : 2) * sizeof(float2) * static_cast<size_t>(M);
@@ -1018,10 +1019,11 @@ hipError_t stage_profile_launch(const void *src, const float *wr, const float *c
     }
     if (inplace)
         hipLaunchKernelGGL((stage_profile_inplace_kernel<RAW, BLOCK, FOLD>), dim3(npri),
-                           dim3(BLOCK), lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, table);
+                           dim3(BLOCK), lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse,
+                           table);
     else
         hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD>), dim3(npri), dim3(BLOCK), lds,
-                           s, src, wr, corr, twiddle, ns, M, Mp, lead, table);
+                           s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse, table);
     return hipGetLastError();
 }
 
@@ -1031,19 +1033,19 @@ hipError_t stage_profile_launch(const void *src, const float *wr, const float *c
 template <class RAW, bool FOLD>
 hipError_t stage_profile_raw(const void *src, const float *wr, const float *corr,
                              const float2 *twiddle, int npri, int ns, int M, int Mp, int lead,
-                             float2 *table, hipStream_t s)
+                             int reverse, float2 *table, hipStream_t s)
 {
     if (M >= kProfileWideM)
         return stage_profile_launch<RAW, 1024, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp,
-                                                     lead, table, s);
+                                                     lead, reverse, table, s);
     return stage_profile_launch<RAW, kBlock, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
-                                                   table, s);
+                                                   reverse, table, s);
 }
 
 template <bool FOLD>
 hipError_t stage_profile_any(int storage, const void *src, const float *wr, const float *corr,
                              const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
-                             float2 *table, hipStream_t s)
+                             int reverse, float2 *table, hipStream_t s)
 {
     if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || taps < 2 || taps > kProfileTaps ||
         Mp < M + taps - 1)
@@ -1051,14 +1053,14 @@ hipError_t stage_profile_any(int storage, const void *src, const float *wr, cons
     const int lead = taps / 2 - 1;
     switch (storage) {
     case 0:
-        return stage_profile_raw<RawC64, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead, table,
-                                               s);
+        return stage_profile_raw<RawC64, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
+                                               reverse, table, s);
     case 1:
-        return stage_profile_raw<RawF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead, table,
-                                               s);
+        return stage_profile_raw<RawF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
+                                               reverse, table, s);
     case 2:
         return stage_profile_raw<RawBF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
-                                                table, s);
+                                                reverse, table, s);
     default:
         return hipErrorInvalidValue;
     }
@@ -1068,19 +1070,19 @@ hipError_t stage_profile_any(int storage, const void *src, const float *wr, cons
 
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
-                                float2 *table, hipStream_t s)
+                                int reverse, float2 *table, hipStream_t s)
 {
     return stage_profile_any<false>(storage, staged, nullptr, corr, twiddle, npri, ns, M, Mp, taps,
-                                    table, s);
+                                    reverse, table, s);
 }
 
 hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
                                        const float *corr, const float2 *twiddle, int npri,
-                                       int ns, int M, int Mp, int taps, float2 *table,
-                                       hipStream_t s)
+                                       int ns, int M, int Mp, int taps, int reverse,
+                                       float2 *table, hipStream_t s)
 {
-    return stage_profile_any<true>(storage, raw, wr, corr, twiddle, npri, ns, M, Mp, taps, table,
-                                   s);
+    return stage_profile_any<true>(storage, raw, wr, corr, twiddle, npri, ns, M, Mp, taps, reverse,
+                                   table, s);
 }
 
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,

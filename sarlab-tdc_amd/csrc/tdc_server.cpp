@@ -69,6 +69,10 @@ constexpr int kNumBuckets = 4;
 // range of a tile (two of the range-profile tile's 8-chirp FP32 blocks).
 constexpr int kFillTiles = 4 * kBuckets[kNumBuckets - 1];
 constexpr int kMinSplitChirps = 16;
+// The chirp ranges of a job write one partial image each: their bytes are
+// kept within this multiple of the job's input bytes (raw samples and grid),
+// so that cutting a few-pixel job short does not multiply its footprint.
+constexpr double kMaxPartialFactor = 4.0;
 // Upload threads: packing a job into pinned memory runs at host memcpy speed,
 // so several jobs are packed and copied at once.
 constexpr int kUploaders = 8;
@@ -317,6 +321,14 @@ void plan(Job &j)
     const int fill_cps = std::max(kMinSplitChirps, (d.n_pri + want_splits - 1) / want_splits);
     j.cps = std::min(j.cps, fill_cps);
     j.cps = std::min(j.cps, d.n_pri);
+    {
+        const double part_px = prec ? sizeof(double2) : sizeof(float2);
+        const double input = 8.0 * static_cast<double>(nterm) + (prec ? 24.0 : 12.0) * npx;
+        const int max_split =
+            std::max(1, static_cast<int>(kMaxPartialFactor * input / (part_px * npx)));
+        const int min_cps = (d.n_pri + max_split - 1) / max_split;
+        j.cps = std::max(j.cps, std::min(min_cps, d.n_pri));
+    }
     j.nsplit = (d.n_pri + j.cps - 1) / j.cps;
     j.ntiles = static_cast<int64_t>(j.npt) * j.nsplit;
     j.nfin = (j.npx + tdc::kBlock - 1) / tdc::kBlock;
@@ -460,6 +472,8 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
             j.pp.Mp = plen + tdc::kProfilePad;
             j.pp.taps = j.pk.taps;
             j.pp.over = j.pk.over;
+            j.pp.dkm = std::fabs(dk) * plen;
+            j.pp.reverse = dk < 0.0 ? 1 : 0;
         }
     } else {
         // the plain form's phase tables (the processor's, from FP64 inputs) and
@@ -544,7 +558,8 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
             return tdc::launch_stage_profile(
                 d.raw_storage, staged, reinterpret_cast<const float *>(j.arena + j.off_pcorr),
                 reinterpret_cast<const float2 *>(j.arena + j.off_ptw), d.n_pri, d.n_samples,
-                j.pp.M, j.pp.Mp, j.pp.taps, reinterpret_cast<float2 *>(j.arena + j.off_ptab), ss);
+                j.pp.M, j.pp.Mp, j.pp.taps, j.pp.reverse,
+                reinterpret_cast<float2 *>(j.arena + j.off_ptab), ss);
         return tdc::launch_stage_matrix(d.raw_storage, staged, d.n_pri, d.n_samples,
                                         tdc::matrix_parts(d.raw_storage), j.arena + j.off_atab,
                                         reinterpret_cast<float *>(j.arena + j.off_ascale), ss);

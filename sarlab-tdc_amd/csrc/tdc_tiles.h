@@ -802,7 +802,6 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     const double pp2 = fma(pz, pz, fma(py, py, px * px));
     const double qx = -2.0 * px, qy = -2.0 * py, qz = -2.0 * pz;
     const int mask = pp.M - 1;
-    const double fm = static_cast<double>(pp.M);
     // chirp contributions are summed in FP32 over blocks of kProfileBlock
     // chirps and the blocks in FP64
     double accr = 0.0, acci = 0.0;
@@ -813,11 +812,15 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
             const ChirpPos c = as_const(cpos)[n];
             const double rng =
                 range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
-            const double u = fract64(pp.dk * rng) * fm; // in [0, M)
-            const float g = static_cast<float>(fract64(u));
-            // indices in unsigned arithmetic: f < M, so row[f .. f + 11] lies
-            // in the row; window indices are masked to the row
-            const unsigned f = static_cast<unsigned>(static_cast<int>(u));
+            // bin position u = x mod M, x = |dk| M R >= 0: integer part f (the
+            // truncation of x, exact while x < 2^31, i.e. R B < ~8e16 m Hz at
+            // fourfold oversampling) and fraction g; down-chirps read reversed
+            // rows, so x is never negative
+            const double x = pp.dkm * rng;
+            const float g = static_cast<float>(fract64(x));
+            // indices in unsigned arithmetic: f < M, so row[f .. f + TAPS - 1]
+            // lies in the row; window indices are masked to the row
+            const unsigned f = static_cast<unsigned>(static_cast<int>(x)) & static_cast<unsigned>(mask);
             const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
             constexpr unsigned kReach = (64 - TAPS) / 2;
             const unsigned lo = __builtin_amdgcn_readfirstlane(f) - kReach;

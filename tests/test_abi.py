@@ -42,7 +42,8 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_abi_version_and_device_count(lib):
-    assert lib.tdc_abi_version() == 10400
+    # 2.0.0: tdc_stats grew by the range-profile kernel fields
+    assert lib.tdc_abi_version() == 20000
     assert lib.tdc_device_count() >= 0
 
 
@@ -157,3 +158,74 @@ def test_range_window_matches_scipy(lib, ns):
         codes["TDC_ERR_INVALID_ARGUMENT"]
     assert lib.tdc_range_window(codes["TDC_WIN_HANN"], 0.0, 0.0, 0, p) == \
         codes["TDC_ERR_INVALID_ARGUMENT"]
+
+
+def test_cpp_header_takes_spdlog_sinks(tmp_path):
+    """When the including translation unit sees <spdlog/spdlog.h>, the header
+    declares the reference's setLoggerSinks(std::vector<spdlog::sink_ptr>)
+    (tdc-lib/include/tdcprocessor.h:128) and forwards each line to the spdlog
+    sink. spdlog is not installed here, so the test compiles against a minimal
+    header with spdlog's sink interface (sinks::sink::log(const
+    details::log_msg &), level_enum, sink_ptr) and checks the forwarding on the
+    host, without a device."""
+    inc = tmp_path / "inc" / "spdlog"
+    inc.mkdir(parents=True)
+    (inc / "spdlog.h").write_text(r"""
+#pragma once
+#include <memory>
+#include <string>
+namespace spdlog {
+namespace level { enum level_enum { trace, debug, info, warn, err, critical, off }; }
+namespace details {
+struct log_msg {
+    log_msg(const std::string &n, level::level_enum l, const std::string &m)
+        : logger_name(n), level(l), payload(m) {}
+    std::string logger_name;
+    level::level_enum level;
+    std::string payload;
+};
+}
+namespace sinks {
+class sink {
+public:
+    virtual ~sink() = default;
+    virtual void log(const details::log_msg &msg) = 0;
+    virtual void flush() = 0;
+};
+}
+using sink_ptr = std::shared_ptr<sinks::sink>;
+}
+""")
+    src = tmp_path / "user.cpp"
+    src.write_text(r"""
+#include <cstdio>
+#include <vector>
+#include "tdcprocessor.h"
+#ifndef SARLABTDC_AMD_HAVE_SPDLOG
+#error "the spdlog overload is not declared"
+#endif
+struct capture : spdlog::sinks::sink {
+    std::string last; spdlog::level::level_enum lvl = spdlog::level::off;
+    void log(const spdlog::details::log_msg &m) override { last = m.logger_name + ":" + m.payload; lvl = m.level; }
+    void flush() override {}
+};
+void (TdcProcessor::*overload)(const std::vector<spdlog::sink_ptr> &) = &TdcProcessor::setLoggerSinks;
+int main() {
+    auto c = std::make_shared<capture>();
+    tdclog::spdlog_sink fwd(c);
+    fwd.log(tdclog::level::warn, "TDCPROC", "Processed 8 chunks, skipped 0 chunks");
+    std::printf("%s %d\n", c->last.c_str(), int(c->lvl));
+    return (c->last == "TDCPROC:Processed 8 chunks, skipped 0 chunks" && c->lvl == spdlog::level::warn && overload) ? 0 : 1;
+}
+""")
+    import subprocess
+
+    pkg = os.path.join(ROOT, "sarlab-tdc_amd", "sarlabtdc")
+    exe = tmp_path / "user"
+    cmd = ["/opt/rocm/bin/hipcc", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+           "-I", str(tmp_path / "inc"), "-I", os.path.join(ROOT, "include"), "-I/opt/rocm/include",
+           str(src), "-L", pkg, "-lsarlabtdc_amd", "-Wl,-rpath," + pkg, "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert run.returncode == 0, run.stdout + run.stderr

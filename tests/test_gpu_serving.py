@@ -94,14 +94,15 @@ def test_job_is_bit_identical_alone_and_in_a_mix(tdc):
 
 
 @pytest.mark.parametrize("nc,ns,side,tiles", [(2048, 128, 16, 128), (512, 256, 64, 512),
-                                              (600, 128, 256, 8192)])
+                                              (600, 128, 256, 2048)])
 def test_jobs_with_few_pixel_tiles_are_cut_into_short_chirp_ranges(tdc, orc, nc, ns, side, tiles):
     """A job with few 256-pixel tiles is cut into chirp ranges short enough
-    for 8192 tiles, down to 16 chirps: 16^2 pixels x 2048 chirps gives one
-    pixel tile x 128 ranges of 16 chirps, 64^2 x 512 gives 16 x 32, and
-    256^2 x 600 gives 256 x 32 ranges of 19 chirps. The image sums the many
-    partial images within the precision form's tolerance and is the same bit
-    for bit alone and beside other jobs."""
+    for 8192 tiles, down to 16 chirps, while its partial images stay within
+    four times its input bytes: 16^2 pixels x 2048 chirps gives one pixel
+    tile x 128 ranges of 16 chirps, 64^2 x 512 gives 16 x 32, and 256^2 x 600
+    (2.2 MB of input, 1 MB per partial image) 256 x 8 ranges of 75 chirps.
+    The image sums the partial images within the precision form's tolerance
+    and is the same bit for bit alone and beside other jobs."""
     s = job_scene(21, nc, ns, side)
     srv = tdc.Server(0, 16e9)
     alone, info = srv.wait(submit(srv, s))
@@ -162,6 +163,35 @@ def test_admission_respects_the_budget(tdc, orc):
         assert admits[k] >= dones[k - 1] - 1e-3  # admitted only after the previous one left
     px = subset(128 * 128, 200)
     e = orc.rel_l2(res[3][0].reshape(-1)[px], oracle_img(orc, jobs[3], px))
+    assert e < 1e-4
+
+
+def test_few_pixel_jobs_under_a_tight_budget(tdc, orc):
+    """Few-pixel jobs are cut into short chirp ranges, but their partial images
+    stay within four times their input bytes: several such jobs (386 chirps x
+    128 samples, 265^2 pixels) under a budget for two at a time all complete,
+    within it, each the same image it forms alone."""
+    jobs = [job_scene(60 + k, 386, 128, 265) for k in range(4)]
+    srv = tdc.Server(0, 64e9)
+    alone, info = srv.wait(submit(srv, jobs[0]))
+    srv.close()
+    one = info["device_bytes"]
+    npx = 265 * 265
+    inputs = 8 * 386 * 128 + 24 * npx
+    nsplit = info["tiles"] / ((npx + 255) // 256)
+    print("footprint", one, "splits", nsplit, "inputs", inputs)
+    assert 16 * npx * nsplit <= 4 * inputs
+    # arena: upload region, staged tables and partials, rounded up to a size class
+    assert one <= 8 * inputs
+    srv = tdc.Server(0, 2.5 * one)
+    ids = [submit(srv, s) for s in jobs]
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st["bytes_peak"] <= 2.5 * one
+    assert np.array_equal(res[0][0], alone)
+    px = subset(npx, 200, extra=np.argsort(np.abs(res[2][0].reshape(-1)))[-4:])
+    e = orc.rel_l2(res[2][0].reshape(-1)[px], oracle_img(orc, jobs[2], px))
     assert e < 1e-4
 
 
