@@ -548,52 +548,104 @@ __device__ __forceinline__ void profile_sample(const void *src, const float *wr,
     }
 }
 
+// exp(+j 2 pi k / M) for k < 3 M / 4 from the half-turn table (k < M / 2):
+// the second half turn is the first negated.
+__device__ __forceinline__ float2 twiddle_at(const float2 *tw, int k, int half)
+{
+    const float2 w = tw[k < half ? k : k - half];
+    return k < half ? w : make_float2(-w.x, -w.y);
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 w)
+{
+    return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x));
+}
+
 // One workgroup per chirp: b[(i - h) mod M] = c_i * corr[i] (zero elsewhere),
-// then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-2 Stockham passes
-// between two LDS buffers (the output comes out in natural order), written
+// then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-4 Stockham passes (and
+// one radix-2 pass when log2 M is odd) between two LDS buffers, with the
+// twiddles staged in LDS up to kProfileLdsTwiddleM (the output comes out in
+// natural order), written
 // shifted by lead = taps / 2 - 1 bins, row[j] = B[(j - lead) mod M], over M +
 // kProfilePad entries: a pixel whose bin position has integer part f reads
 // its taps B[f - lead .. f - lead + taps - 1] as row[f .. f + taps - 1],
 // which never wrap. For down-chirps (`reverse`) the row holds
 // B[(lead - j) mod M]: the profile at frequency -u is read at u.
-template <class RAW, int BLOCK, bool FOLD>
+template <class RAW, int BLOCK, bool FOLD, bool LDS_TW>
 __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
     const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead, int reverse,
     float2 *__restrict__ table)
 {
-    extern __shared__ float2 buf[]; // 2 M entries, sized at launch
+    // 2 M entries (+ M / 2 twiddles with LDS_TW), sized at launch
+    extern __shared__ float2 buf[];
     const int n = static_cast<int>(blockIdx.x);
     const int tid = static_cast<int>(threadIdx.x);
     const int h = ns / 2;
-    const int mask = M - 1;
+    const int half = M / 2;
     float2 *x = buf;
     float2 *y = buf + M;
-    for (int j = tid; j < M; j += BLOCK) x[j] = make_float2(0.0F, 0.0F);
-    __syncthreads();
-    for (int i = tid; i < ns; i += BLOCK) {
-        float re, im;
-        profile_sample<RAW, FOLD>(src, wr, n, ns, i, re, im);
-        const float c = corr[i];
-        x[(i - h) & mask] = make_float2(re * c, im * c);
+    const float2 *tw = twiddle;
+    if constexpr (LDS_TW) {
+        float2 *t = buf + 2 * M;
+        for (int k = tid; k < half; k += BLOCK) t[k] = twiddle[k];
+        tw = t;
+    }
+    // b[j] = c_i corr[i] for j = (i - h) mod M, zero elsewhere, in one pass
+    for (int j = tid; j < M; j += BLOCK) {
+        const int i = j < ns - h ? j + h : (j >= M - h ? j - M + h : -1);
+        float2 v = make_float2(0.0F, 0.0F);
+        if (i >= 0) {
+            float re, im;
+            profile_sample<RAW, FOLD>(src, wr, n, ns, i, re, im);
+            const float c = corr[i];
+            v = make_float2(re * c, im * c);
+        }
+        x[j] = v;
     }
     __syncthreads();
-    for (int l = M / 2, s = 1, lg = 0; l >= 1; l >>= 1, s <<= 1, ++lg) {
-        for (int idx = tid; idx < M / 2; idx += BLOCK) {
+    // Stockham passes, B[m] = sum_j b[j] exp(+j 2 pi j m / M) in natural order:
+    // radix 4 while a quarter of the remaining length L = M / s is left, then
+    // one radix-2 pass (twiddle 1) when log2 M is odd
+    int s = 1, lg = 0;
+    for (; (M >> lg) >= 4; s <<= 2, lg += 2) {
+        const int q = (M >> lg) >> 2; // L / 4
+        for (int idx = tid; idx < M / 4; idx += BLOCK) {
             const int j = idx >> lg;
             const int k = idx & (s - 1);
-            const float2 w = twiddle[j * s];
-            const float2 a = x[k + s * j];
-            const float2 c = x[k + s * (j + l)];
-            const float dr = a.x - c.x, di = a.y - c.y;
-            y[k + 2 * s * j] = make_float2(a.x + c.x, a.y + c.y);
-            y[k + 2 * s * j + s] = make_float2(fmaf(dr, w.x, -(di * w.y)), fmaf(dr, w.y, di * w.x));
+            const float2 a0 = x[k + s * j];
+            const float2 a1 = x[k + s * (j + q)];
+            const float2 a2 = x[k + s * (j + 2 * q)];
+            const float2 a3 = x[k + s * (j + 3 * q)];
+            const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y);
+            const float2 t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
+            const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
+            // +j (a1 - a3): the quarter turn of the +j 2 pi transform
+            const float2 t3 = make_float2(a3.y - a1.y, a1.x - a3.x);
+            const int e = j * s;
+            float2 *o = y + k + 4 * s * j;
+            o[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+            o[s] = cmulf(make_float2(t1.x + t3.x, t1.y + t3.y), twiddle_at(tw, e, half));
+            o[2 * s] = cmulf(make_float2(t0.x - t2.x, t0.y - t2.y), twiddle_at(tw, 2 * e, half));
+            o[3 * s] = cmulf(make_float2(t1.x - t3.x, t1.y - t3.y), twiddle_at(tw, 3 * e, half));
         }
         __syncthreads();
         float2 *tmp = x;
         x = y;
         y = tmp;
     }
+    if ((M >> lg) == 2) {
+        for (int k = tid; k < half; k += BLOCK) {
+            const float2 a = x[k], c = x[k + half];
+            y[k] = make_float2(a.x + c.x, a.y + c.y);
+            y[k + half] = make_float2(a.x - c.x, a.y - c.y);
+        }
+        __syncthreads();
+        float2 *tmp = x;
+        x = y;
+        y = tmp;
+    }
+    const int mask = M - 1;
     float2 *dst = table + static_cast<size_t>(n) * Mp;
     for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(reverse ? lead - j : j - lead) & mask];
 }
@@ -1001,16 +1053,23 @@ hipError_t launch_doppler_chirp(int ppl, const float4 *rec, const float4 *cpos,
 
 namespace {
 
+// Twiddles are staged in LDS with the two Stockham buffers up to this
+// transform length (2.5 M entries, 80 KiB at 4096); beyond, they are read from
+// the table in global memory
+constexpr int kProfileLdsTwiddleM = 4096;
+
 template <class RAW, int BLOCK, bool FOLD>
 hipError_t stage_profile_launch(const void *src, const float *wr, const float *corr,
                                 const float2 *twiddle, int npri, int ns, int M, int Mp,
                                 int lead, int reverse, float2 *table, hipStream_t s)
 {
     const bool inplace = M > kProfileStockhamMaxM;
-    const size_t lds = (inplace ? 1 : 2) * sizeof(float2) * static_cast<size_t>(M);
+    const bool ldstw = M <= kProfileLdsTwiddleM;
+    const size_t lds = (inplace ? 2 : (ldstw ? 5 : 4)) * sizeof(float2) * static_cast<size_t>(M) / 2;
     const void *kernel =
         inplace ? reinterpret_cast<const void *>(&stage_profile_inplace_kernel<RAW, BLOCK, FOLD>)
-                : reinterpret_cast<const void *>(&stage_profile_kernel<RAW, BLOCK, FOLD>);
+        : ldstw ? reinterpret_cast<const void *>(&stage_profile_kernel<RAW, BLOCK, FOLD, true>)
+                : reinterpret_cast<const void *>(&stage_profile_kernel<RAW, BLOCK, FOLD, false>);
     // more than 64 KiB of dynamic LDS is granted to a kernel on request
     if (lds > (size_t(64) << 10)) {
         const hipError_t e = hipFuncSetAttribute(
@@ -1021,9 +1080,12 @@ hipError_t stage_profile_launch(const void *src, const float *wr, const float *c
         hipLaunchKernelGGL((stage_profile_inplace_kernel<RAW, BLOCK, FOLD>), dim3(npri),
                            dim3(BLOCK), lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse,
                            table);
+    else if (ldstw)
+        hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD, true>), dim3(npri), dim3(BLOCK),
+                           lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse, table);
     else
-        hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD>), dim3(npri), dim3(BLOCK), lds,
-                           s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse, table);
+        hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD, false>), dim3(npri), dim3(BLOCK),
+                           lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse, table);
     return hipGetLastError();
 }
 

@@ -33,16 +33,19 @@ struct alignas(16) ServeJob {
     int npx, npri, ns, nsplit, cps, form, storage, profile;
 };
 
+// A tile names its job by the device address of the job's descriptor, which
+// travels to HBM in the job's own upload (no per-tile read of host memory
+// beyond the 16-byte tile entry).
 struct ServeTile {
-    int job;   // job slot
+    const ServeJob *job;
     int pix0;  // first pixel of the tile
     int split; // chirp range index
-    int pad;
 };
 
 struct FinTile {
-    int job;
+    const ServeJob *job;
     int pix0; // first of up to kBlock pixels
+    int pad;
 };
 
 struct StepTable {
@@ -50,17 +53,21 @@ struct StepTable {
     int nfin;
     int nfast;
     int pad;
+    // the values the two tile counters hold when this step starts: counters
+    // are never reset, each step takes tile t at counter value base + t
+    unsigned int base[2];
+    int pad2[2];
     ServeTile tiles[kMaxStepTiles];
     FinTile fin[kMaxStepFin];
 };
 
-// `counter`: two counters, one per tile family.
-// `matrix`: also launch the matrix-engine family's kernel (a step with no
-// such tiles leaves it out).
-hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
-                             int workgroups, bool matrix, hipStream_t s);
-hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,
-                                 hipStream_t s);
+// `counter`: two counters, one per tile family, read relative to the step
+// table's bases. `matrix`: also launch the matrix-engine family's kernel (a
+// step with no such tiles leaves it out). Every workgroup of a family's
+// kernel advances its counter by one beyond the tiles it takes.
+hipError_t launch_serve_step(const StepTable *table, unsigned int *counter, int workgroups,
+                             bool matrix, hipStream_t s);
+hipError_t launch_serve_finalize(const StepTable *table, int workgroups, hipStream_t s);
 // Plain-form terms (rr, ri, dk_m, 0) of a served job from its complex64 raw
 // samples (windows applied) and the per-sample dk_m.
 hipError_t launch_stage_plain_chirp(const float *raw, const float *dk, int npri, int ns,

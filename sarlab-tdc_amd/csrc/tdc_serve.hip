@@ -1,17 +1,18 @@
 // tdc_serve.hip -- kernels of the serving form.
 //
-// A step of the server runs two kernels, captured once per grid-size bucket in
-// a hipGraph and replayed, and a finalize kernel on its own stream:
-//   1. serve_reset_kernel   zeroes the step's two tile counters (a kernel, not a
-//                           memset node, so the graph replays cleanly);
-//   2. serve_kernel         persistent: every workgroup takes the next tile
-//                           from the step table with one atomic, runs the
+// A step of the server runs one persistent kernel per tile family it holds,
+// captured once per grid-size bucket in a hipGraph and replayed, and a
+// finalize kernel behind it:
+//   1. serve_kernel         persistent: every workgroup takes the next tile
+//                           from the step table with one atomic on a counter
+//                           that is never reset (the step table holds the
+//                           counter's value at the step's start), runs the
 //                           tile body of its job (plain; precision with the
 //                           range-profile or matrix engine; any raw storage)
-//                           and writes the job's partial image for
-//                           that (pixel tile, chirp range); all workgroups
-//                           leave once the counter passes the tile count;
-//   3. serve_finalize_kernel sums each finished job's partial images in
+//                           and writes the job's partial image for that
+//                           (pixel tile, chirp range); all workgroups leave
+//                           once the counter passes the tile count;
+//   2. serve_finalize_kernel sums each finished job's partial images in
 //                           chirp-range order into its image.
 // A job's tiles, their chirp ranges and the finalize order depend only on
 // the job's own shape, so its image is bit-identical whatever else shares the
@@ -24,33 +25,28 @@
 namespace tdc {
 namespace {
 
-__global__ void serve_reset_kernel(unsigned int *counter)
-{
-    counter[0] = 0U;
-    counter[1] = 0U;
-}
-
 // FAMILY 0 runs the table's first nfast tiles (plain form and range-profile
 // engine, at most ~62 VGPRs, 8 waves per SIMD), FAMILY 1 the rest (matrix
 // engine, which needs ~130 VGPRs); each family has its own kernel so that
 // the matrix tile's registers do not limit the occupancy of the others.
 template <int FAMILY>
-__global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restrict__ jobs,
-                                                       const StepTable *__restrict__ table,
+__global__ __launch_bounds__(kBlock) void serve_kernel(const StepTable *__restrict__ table,
                                                        unsigned int *__restrict__ counter)
 {
-    __shared__ int s_tile;
+    __shared__ unsigned int s_tile;
     const cptr<StepTable> tab = as_const(table);
     const int first = FAMILY == 0 ? 0 : tab->nfast;
-    const int count = FAMILY == 0 ? tab->nfast : tab->ntiles - tab->nfast;
+    const unsigned int count =
+        static_cast<unsigned int>(FAMILY == 0 ? tab->nfast : tab->ntiles - tab->nfast);
+    const unsigned int base = tab->base[FAMILY];
     for (;;) {
-        if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(counter + FAMILY, 1U));
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter + FAMILY, 1U) - base;
         __syncthreads();
-        const int t = __builtin_amdgcn_readfirstlane(s_tile);
+        const unsigned int t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if (t >= count) break; // every wave of the workgroup takes this exit
-        const ServeTile tl = tab->tiles[first + t];
-        const ServeJob j = as_const(jobs)[tl.job];
+        const ServeTile tl = tab->tiles[first + static_cast<int>(t)];
+        const ServeJob j = as_const(tl.job)[0];
         const int n0 = tl.split * j.cps;
         const int n1 = min(j.npri, n0 + j.cps);
         if (FAMILY == 0) {
@@ -86,14 +82,13 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ServeJob *__restric
     }
 }
 
-__global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const ServeJob *__restrict__ jobs,
-                                                                const StepTable *__restrict__ table)
+__global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const StepTable *__restrict__ table)
 {
     const cptr<StepTable> tab = as_const(table);
     const int nfin = tab->nfin;
     for (int f = static_cast<int>(blockIdx.x); f < nfin; f += static_cast<int>(gridDim.x)) {
         const FinTile ft = tab->fin[f];
-        const ServeJob j = as_const(jobs)[ft.job];
+        const ServeJob j = as_const(ft.job)[0];
         const int p = ft.pix0 + static_cast<int>(threadIdx.x);
         if (p >= j.npx) continue;
         if (j.form == 0) {
@@ -141,21 +136,18 @@ __global__ __launch_bounds__(kBlock) void stage_plain_chirp_kernel(const float2 
 
 } // namespace
 
-hipError_t launch_serve_step(const ServeJob *jobs, const StepTable *table, unsigned int *counter,
-                             int workgroups, bool matrix, hipStream_t s)
+hipError_t launch_serve_step(const StepTable *table, unsigned int *counter, int workgroups,
+                             bool matrix, hipStream_t s)
 {
-    hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL(serve_kernel<0>, dim3(workgroups), dim3(kBlock), 0, s, jobs, table, counter);
+    hipLaunchKernelGGL(serve_kernel<0>, dim3(workgroups), dim3(kBlock), 0, s, table, counter);
     if (matrix)
-        hipLaunchKernelGGL(serve_kernel<1>, dim3(workgroups), dim3(kBlock), 0, s, jobs, table,
-                           counter);
+        hipLaunchKernelGGL(serve_kernel<1>, dim3(workgroups), dim3(kBlock), 0, s, table, counter);
     return hipGetLastError();
 }
 
-hipError_t launch_serve_finalize(const ServeJob *jobs, const StepTable *table, int workgroups,
-                                 hipStream_t s)
+hipError_t launch_serve_finalize(const StepTable *table, int workgroups, hipStream_t s)
 {
-    hipLaunchKernelGGL(serve_finalize_kernel, dim3(workgroups), dim3(kBlock), 0, s, jobs, table);
+    hipLaunchKernelGGL(serve_finalize_kernel, dim3(workgroups), dim3(kBlock), 0, s, table);
     return hipGetLastError();
 }
 

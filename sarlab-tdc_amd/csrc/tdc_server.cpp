@@ -6,7 +6,8 @@
 //   uploaders  (kUploaders threads) take admitted jobs in arrival order, pack
 //              each job's inputs into pinned staging memory and move them with
 //              one large host-to-device copy (an SDMA transfer, no kernel) on
-//              their own stream, then mark the job resident;
+//              their own stream, run the job's staging kernels behind it, then
+//              mark the job resident;
 //   scheduler  admits queued jobs in arrival order while their HBM footprint
 //              fits the budget, builds steps and completes them.
 // A step is a table of tiles (job, pixel tile, chirp range) taken from the
@@ -20,14 +21,14 @@
 // step before it (on the other stream) has ended too; that covers all of its
 // tiles.
 //
-// Nothing besides the steps needs a CU while steps run back to back: small
-// host-to-device copies would run as blit kernels and wait behind the
+// Small host-to-device copies would run as blit kernels and wait behind the
 // persistent workgroups, so job descriptors and step tables live in pinned
 // host memory that the kernels read directly (a few hundred bytes per tile),
-// finished images are written by the finalize straight into pinned host
-// memory, and the staging kernels of a job (plain records, FP16/BF16
-// conversion, range-profile FFT tables or matrix A tiles) run on the step
-// stream ahead of the job's first step.
+// and finished images are written by the finalize straight into pinned host
+// memory. The staging kernels of a job (plain records, FP16/BF16 conversion,
+// range-profile FFT tables or matrix A tiles) run on its uploader's stream
+// right behind its upload, beside the steps of the jobs already resident;
+// the job joins the steps once they have run.
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -224,12 +225,15 @@ struct Job {
     tdc::ProfileChoice pk; // its kernel and FFT length
     tdc::ProfileParams pp{};
     // device arena layout: [upload region | staged records or raw | partials]
-    size_t off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
+    size_t off_desc = 0, off_cpos = 0, off_ka = 0, off_grid = 0, off_kt = 0, off_ones = 0, off_stage = 0,
            off_pcorr = 0, off_ptw = 0, off_ptab = 0,
            off_atab = 0, off_ascale = 0, off_part = 0, up_bytes = 0, bytes = 0;
-    bool needs_stage = false;
     char *arena = nullptr;
-    int slot = -1;
+    int slot = -1; // admission slot (at most kMaxJobSlots jobs admitted at once)
+    const tdc::ServeJob *desc() const
+    {
+        return reinterpret_cast<const tdc::ServeJob *>(arena + off_desc);
+    }
     float *h_image = nullptr; // pinned, written by the finalize kernel, read by wait
     // host
     double t_submit = 0, t_admit = 0, t_first = -1, t_done = 0;
@@ -241,7 +245,6 @@ struct Step {
     int buf = 0;
     std::vector<Job *> finishing;
     std::vector<Job *> touched;
-    std::vector<Job *> stage; // jobs whose staging kernels precede this step
 };
 
 } // namespace
@@ -254,13 +257,13 @@ struct tdc_server {
     hipStream_t up_stream[kUploaders] = {};          // uploads, one per uploader thread
     DeviceArenas arenas; // job arenas, recycled
     hipEvent_t ev_start[kDepth] = {}, ev_end[kDepth] = {};
-    hipEvent_t ev_fin[kDepth] = {}, ev_staged[kDepth] = {};
+    hipEvent_t ev_fin[kDepth] = {};
     hipEvent_t ev_origin = nullptr;
-    tdc::ServeJob *h_jobs = nullptr;       // pinned job table
-    tdc::ServeJob *dev_jobs = nullptr;     // its device address
     tdc::StepTable *h_table[kDepth] = {};   // pinned step tables
     tdc::StepTable *dev_table[kDepth] = {}; // their device addresses
-    unsigned int *d_counter = nullptr; // kDepth counters, 256 bytes apart
+    unsigned int *d_counter = nullptr; // kDepth pairs of tile counters, 256 bytes apart
+    // the value each counter holds when the next step on its table starts
+    unsigned int counter_next[kDepth][2] = {};
     // [with the matrix-engine family][grid-size bucket][step table]
     hipGraphExec_t graph[2][kNumBuckets][kDepth] = {};
     int ngraphs = 0;
@@ -361,8 +364,9 @@ void plan(Job &j)
         j.off_grid = o;
         o += align256(24 * npx);
     }
+    j.off_desc = o; // the job's descriptor (device copy read by the kernels)
+    o += align256(sizeof(tdc::ServeJob));
     j.up_bytes = o;
-    j.needs_stage = true;
     j.off_stage = o;
     if (!prec) {
         o += align256(sizeof(float4) * nterm);
@@ -414,7 +418,9 @@ int validate(const tdc_job_desc *d)
         }                                                                                      \
     } while (0)
 
-// Pack, upload and describe one admitted job (uploader thread).
+hipError_t stage_job(const Job &j, hipStream_t ss);
+
+// Pack, upload, stage and describe one admitted job (uploader thread).
 bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
 {
     tdc::Range range("tdc_server_upload");
@@ -489,22 +495,22 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
     }
     char *arena = nullptr;
     hipError_t e = s->arenas.get(j.bytes, s->budget, &arena);
+    void *img = nullptr, *img_dev = nullptr;
     if (e == hipSuccess) {
         j.arena = arena;
-        e = hipMemcpyAsync(arena, stage_mem, j.up_bytes, hipMemcpyHostToDevice, us);
+        e = s->coherent.get(sizeof(float2) * j.npx, &img);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(us);
-    s->pinned.put(stage_mem, j.up_bytes);
+    if (e == hipSuccess) {
+        j.h_image = static_cast<float *>(img);
+        e = hipHostGetDevicePointer(&img_dev, img, 0);
+    }
     if (e != hipSuccess) {
-        err = std::string("job upload failed: ") + hipGetErrorString(e);
+        s->pinned.put(stage_mem, j.up_bytes);
+        err = std::string("job set-up failed: ") + hipGetErrorString(e);
         return false;
     }
-    void *img = nullptr;
-    SRV_HIP(s->coherent.get(sizeof(float2) * j.npx, &img));
-    j.h_image = static_cast<float *>(img);
-    void *img_dev = nullptr;
-    SRV_HIP(hipHostGetDevicePointer(&img_dev, img, 0));
-
+    // the descriptor the kernels read, with the job's device addresses; it
+    // travels in the job's upload and tiles point at its device copy
     tdc::ServeJob sj{};
     sj.cpos = reinterpret_cast<const tdc::ChirpPos *>(j.arena + j.off_cpos);
     sj.image = static_cast<float2 *>(img_dev);
@@ -532,13 +538,22 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
         sj.grid64 = reinterpret_cast<const double *>(j.arena + j.off_grid);
         sj.kanchor = reinterpret_cast<const double *>(j.arena + j.off_ka);
     }
-    // the kernels read the descriptor from pinned memory; it is written before
-    // any step that names the job is launched
-    s->h_jobs[j.slot] = sj;
+    std::memcpy(h + j.off_desc, &sj, sizeof(sj));
+    e = hipMemcpyAsync(arena, stage_mem, j.up_bytes, hipMemcpyHostToDevice, us);
+    // the job's staging kernels (FFT tables, A tiles, records) right behind its
+    // upload on this thread's stream: they overlap the steps of other jobs,
+    // and the job is resident once they have run
+    if (e == hipSuccess) e = stage_job(j, us);
+    if (e == hipSuccess) e = hipStreamSynchronize(us);
+    s->pinned.put(stage_mem, j.up_bytes);
+    if (e != hipSuccess) {
+        err = std::string("job upload failed: ") + hipGetErrorString(e);
+        return false;
+    }
     return true;
 }
 
-// Staging kernels of a job, on the stream of its first step (scheduler thread).
+// Staging kernels of a job, on its uploader's stream right behind its upload.
 hipError_t stage_job(const Job &j, hipStream_t ss)
 {
     const tdc_job_desc &d = j.d;
@@ -578,8 +593,8 @@ bool ensure_graph(tdc_server *s, int matrix, int bucket, int buf, std::string &e
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        e = tdc::launch_serve_step(s->dev_jobs, s->dev_table[buf], s->d_counter + 64 * buf,
-                                   kBuckets[bucket], matrix != 0, cs);
+        e = tdc::launch_serve_step(s->dev_table[buf], s->d_counter + 64 * buf, kBuckets[bucket],
+                                   matrix != 0, cs);
         hipError_t e2 = hipStreamEndCapture(cs, &g);
         if (e == hipSuccess) e = e2;
     }
@@ -754,7 +769,7 @@ void scheduler(tdc_server *s)
                     const int64_t t = j->next_tile++;
                     const int split = static_cast<int>(t / j->npt);
                     const int ptile = static_cast<int>(t % j->npt);
-                    const tdc::ServeTile tl{j->slot, ptile * j->tile_px, split, 0};
+                    const tdc::ServeTile tl{j->desc(), ptile * j->tile_px, split};
                     if (j->d.form == TDC_FORM_PRECISION && !j->profile) mtiles.push_back(tl);
                     else tab->tiles[nt - static_cast<int>(mtiles.size())] = tl;
                     ++nt;
@@ -769,7 +784,6 @@ void scheduler(tdc_server *s)
                 if (took) {
                     if (j->t_first < 0) {
                         j->t_first = s->now();
-                        if (j->needs_stage) st.stage.push_back(j);
                     }
                     if (std::find(st.touched.begin(), st.touched.end(), j) == st.touched.end()) {
                         ++j->steps;
@@ -788,7 +802,8 @@ void scheduler(tdc_server *s)
             for (Job *j : s->active) {
                 // jobs whose last tile is in this step (or an earlier one) finish here
                 if (j->next_tile == j->ntiles && nf + j->nfin <= tdc::kMaxStepFin) {
-                    for (int f = 0; f < j->nfin; ++f) tab->fin[nf++] = tdc::FinTile{j->slot, f * tdc::kBlock};
+                    for (int f = 0; f < j->nfin; ++f)
+                        tab->fin[nf++] = tdc::FinTile{j->desc(), f * tdc::kBlock, 0};
                     st.finishing.push_back(j);
                     j->next_tile = j->ntiles + 1; // finalize scheduled
                 }
@@ -808,6 +823,8 @@ void scheduler(tdc_server *s)
         std::copy(mtiles.begin(), mtiles.end(), tab->tiles + tab->nfast);
         tab->ntiles = nt;
         tab->nfin = nf;
+        tab->base[0] = s->counter_next[st.buf][0];
+        tab->base[1] = s->counter_next[st.buf][1];
         int bucket = kNumBuckets - 1;
         for (int b = 0; b < kNumBuckets; ++b) {
             if (kBuckets[b] >= nt) {
@@ -823,15 +840,14 @@ void scheduler(tdc_server *s)
         const int p = st.buf;
         const int prev = (p + kDepth - 1) % kDepth; // the step before, on the other stream
         hipStream_t ss = s->step_stream[p & 1];
-        hipError_t e = hipSuccess;
-        // jobs staged by the previous step (the other stream) must be staged
-        // before this step reads them
-        if (step_index > 0) e = hipStreamWaitEvent(ss, s->ev_staged[prev], 0);
-        for (Job *j : st.stage)
-            if (e == hipSuccess) e = stage_job(*j, ss);
-        if (e == hipSuccess) e = hipEventRecord(s->ev_staged[p], ss);
-        if (e == hipSuccess) e = hipEventRecord(s->ev_start[p], ss);
+        hipError_t e = hipEventRecord(s->ev_start[p], ss);
         if (e == hipSuccess) e = hipGraphLaunch(s->graph[matrix][bucket][p], ss);
+        // every workgroup of a family's kernel takes one counter value past the
+        // family's tiles
+        s->counter_next[p][0] += static_cast<unsigned int>(tab->nfast + kBuckets[bucket]);
+        if (matrix)
+            s->counter_next[p][1] +=
+                static_cast<unsigned int>(nt - tab->nfast + kBuckets[bucket]);
         if (e == hipSuccess) e = hipEventRecord(s->ev_end[p], ss);
         // finalize on the same stream, right behind this step, once the previous
         // step (the other stream) has ended too: the two cover every earlier
@@ -839,7 +855,7 @@ void scheduler(tdc_server *s)
         if (e == hipSuccess && step_index > 0 && nf > 0)
             e = hipStreamWaitEvent(ss, s->ev_end[prev], 0);
         if (e == hipSuccess && nf > 0)
-            e = tdc::launch_serve_finalize(s->dev_jobs, s->dev_table[p], kFinWorkgroups, ss);
+            e = tdc::launch_serve_finalize(s->dev_table[p], kFinWorkgroups, ss);
         if (e == hipSuccess) e = hipEventRecord(s->ev_fin[p], ss);
         if (e != hipSuccess) return die(std::string("step launch failed: ") + hipGetErrorString(e));
         {
@@ -881,7 +897,6 @@ int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, t
             SRV_HIP(hipEventCreate(&s->ev_start[i]));
             SRV_HIP(hipEventCreate(&s->ev_end[i]));
             SRV_HIP(hipEventCreateWithFlags(&s->ev_fin[i], hipEventDisableTiming));
-            SRV_HIP(hipEventCreateWithFlags(&s->ev_staged[i], hipEventDisableTiming));
             // step tables and the job table are rewritten by the host between
             // launches: coherent memory, so no GPU cache keeps a stale copy
             SRV_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_table[i]), sizeof(tdc::StepTable),
@@ -889,11 +904,11 @@ int tdc_server_create(int gpu_num, double hbm_budget_bytes, double step_terms, t
             SRV_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->dev_table[i]),
                                             s->h_table[i], 0));
         }
-        SRV_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_jobs),
-                              sizeof(tdc::ServeJob) * tdc::kMaxJobSlots,
-                              hipHostMallocCoherent | hipHostMallocMapped));
-        SRV_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->dev_jobs), s->h_jobs, 0));
         SRV_HIP(hipMalloc(reinterpret_cast<void **>(&s->d_counter), 256 * kDepth));
+        // the tile counters start at zero and are never reset (steps read them
+        // relative to the bases in their tables)
+        SRV_HIP(hipMemset(s->d_counter, 0, 256 * kDepth));
+        SRV_HIP(hipDeviceSynchronize());
         SRV_HIP(hipEventRecord(s->ev_origin, s->step_stream[0]));
         SRV_HIP(hipEventSynchronize(s->ev_origin));
         return true;
@@ -1035,13 +1050,11 @@ int tdc_server_destroy(tdc_server *s)
         if (s->ev_start[i]) (void) hipEventDestroy(s->ev_start[i]);
         if (s->ev_end[i]) (void) hipEventDestroy(s->ev_end[i]);
         if (s->ev_fin[i]) (void) hipEventDestroy(s->ev_fin[i]);
-        if (s->ev_staged[i]) (void) hipEventDestroy(s->ev_staged[i]);
         if (s->h_table[i]) (void) hipHostFree(s->h_table[i]);
     }
     for (auto &ss : s->step_stream)
         if (ss) (void) hipStreamDestroy(ss);
     if (s->ev_origin) (void) hipEventDestroy(s->ev_origin);
-    if (s->h_jobs) (void) hipHostFree(s->h_jobs);
     if (s->d_counter) (void) hipFree(s->d_counter);
     for (auto &us : s->up_stream)
         if (us) (void) hipStreamDestroy(us);
