@@ -78,6 +78,11 @@ typedef struct tdc_processor tdc_processor;
  * per (pixel, chirp) and scaled the chirp's sum. */
 #define TDC_PER_CHIRP_RANGE 1
 #define TDC_PER_CHIRP_WINDOW 2
+/* Not per chirp: the antenna moved within chirps, the range was evaluated per
+ * term, and its offset from the chirp's first position came from the
+ * two-term expansion of the square root (the motion was small enough against
+ * the shortest range for a remainder below 5e-8 cycles of phase). */
+#define TDC_PER_TERM_SERIES 4
 
 typedef struct tdc_stats {
     double kernel_ms;       /* device time of the last run divided by its passes, each pass the

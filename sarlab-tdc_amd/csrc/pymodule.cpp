@@ -240,6 +240,7 @@ public:
         d["processed_chunks"] = st.processed_chunks;
         d["range_per_chirp"] = (st.per_chirp & TDC_PER_CHIRP_RANGE) != 0;
         d["window_per_chirp"] = (st.per_chirp & TDC_PER_CHIRP_WINDOW) != 0;
+        d["range_series"] = (st.per_chirp & TDC_PER_TERM_SERIES) != 0;
         d["engine"] = st.engine == TDC_ENGINE_MATRIX    ? "matrix"
                       : st.engine == TDC_ENGINE_VECTOR  ? "vector"
                       : st.engine == TDC_ENGINE_PROFILE ? "profile"

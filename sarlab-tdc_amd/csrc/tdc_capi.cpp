@@ -236,6 +236,7 @@ struct tdc_processor {
     int run_engine = 0;
     bool doppler = false;
     bool chirp_path = false; // plain form: range (and window) once per (pixel, chirp)
+    bool series = false;     // plain form, range per term: expansion of the range offset
     int ant_stride = 0;      // samples between the antenna states of consecutive chirps on the device
     // tdc_start uploads the focus grid behind the chirp inputs, in slabs that
     // overlap the correlate-sum of the slabs before them
@@ -367,6 +368,55 @@ bool doppler_window_is_one(const tdc_processor *h, const tdc::AntennaScan &scan,
     return true;
 }
 
+// True when the range offset of a term from its chirp's first position, dR,
+// may come from the two-term expansion x - x^2 / (2 R0) (plain_tile): its
+// remainder is at most k |delta|^3 / (2 R0^2) cycles of phase, with |delta|
+// the antenna's largest offset within a chirp (the scan) and R0 at least the
+// distance between the bounding boxes of the positions and the grid; the
+// expansion is taken when that bound is below 5e-8 cycles and |delta| below
+// 1% of the distance.
+bool series_offset_ok(const tdc_processor *h, const tdc::AntennaScan &scan,
+                      const std::vector<double> &k)
+{
+    if (!scan.finite || !h->grid) return false;
+    const size_t npx = static_cast<size_t>(h->rows) * h->cols;
+    if (npx == 0) return false;
+    float glo[3], ghi[3];
+    for (int a = 0; a < 3; ++a) glo[a] = ghi[a] = h->grid[a];
+    std::mutex mu;
+    bool finite = true;
+    tdc::parallel_for(npx, 8192, [&](size_t i0, size_t i1) {
+        float lo[3] = {glo[0], glo[1], glo[2]}, hi[3] = {ghi[0], ghi[1], ghi[2]};
+        bool fin = true;
+        for (size_t i = i0; i < i1; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                const float v = h->grid[3 * i + a];
+                fin = fin && std::isfinite(v);
+                lo[a] = std::min(lo[a], v);
+                hi[a] = std::max(hi[a], v);
+            }
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        finite = finite && fin;
+        for (int a = 0; a < 3; ++a) {
+            glo[a] = std::min(glo[a], lo[a]);
+            ghi[a] = std::max(ghi[a], hi[a]);
+        }
+    });
+    if (!finite) return false;
+    double gap2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double g = std::max({0.0, double(glo[a]) - double(scan.hi[a]),
+                                   double(scan.lo[a]) - double(ghi[a])});
+        gap2 += g * g;
+    }
+    double kmax = 0.0;
+    for (double v : k) kmax = std::max(kmax, std::fabs(v));
+    const double delta = std::sqrt(scan.dmax2);
+    return gap2 > 0.0 && delta * delta <= 1e-4 * gap2 &&
+           kmax * delta * delta * delta <= 1e-7 * gap2;
+}
+
 // Queues a copy of `count` host values into `b` on the processor's ingest;
 // `src` may be released on return. The compute stream waits for it after
 // the next ingest fence.
@@ -477,6 +527,7 @@ int prepare_plain(tdc_processor *h, float dop_center, float dop_bw, bool centroi
     // range of a term depends only on its pixel and chirp; with velocity and
     // attitude constant too, so does its Doppler window
     h->chirp_path = scan.pos_const && (!h->doppler || (scan.vel_const && scan.att_const));
+    h->series = !h->chirp_path && series_offset_ok(h, scan, k);
     hipStream_t s = h->stream;
     int rc;
     tdc::Range uploads("tdc_upload_inputs");
@@ -1249,14 +1300,15 @@ int launch_main(tdc_processor *h, const tdc::Geom &g, size_t p0, int *wgs)
                                               grid, ka, g, h->dp, dst,
                                               h->d_counters.as<unsigned long long>(), s, wgs));
         } else if (h->doppler) {
-            TDC_HIP(tdc::launch_doppler(h->ppl, h->d_rec.as<tdc::DopRec>(), grid, ka, g, h->dp,
-                                        dst, h->d_counters.as<unsigned long long>(), s, wgs));
+            TDC_HIP(tdc::launch_doppler(h->ppl, h->series, h->d_rec.as<tdc::DopRec>(), grid, ka, g,
+                                        h->dp, dst, h->d_counters.as<unsigned long long>(), s,
+                                        wgs));
         } else if (h->chirp_path) {
             TDC_HIP(tdc::launch_range(h->ppl, h->d_rec.as<float4>(), cpos, grid, ka, g, dst, s,
                                       wgs));
         } else {
-            TDC_HIP(tdc::launch_plain(h->ppl, h->d_rec.as<tdc::PlainRec>(), cpos, grid, ka, g, dst,
-                                      s, wgs));
+            TDC_HIP(tdc::launch_plain(h->ppl, h->series, h->d_rec.as<tdc::PlainRec>(), cpos, grid,
+                                      ka, g, dst, s, wgs));
         }
         return TDC_OK;
     }
@@ -1329,6 +1381,8 @@ int finish_stats(tdc_processor *h, double main_ms, double total_ms, int wgs)
     st.per_chirp = 0;
     if (h->run_form == TDC_FORM_PLAIN && h->chirp_path)
         st.per_chirp = TDC_PER_CHIRP_RANGE | (h->doppler ? TDC_PER_CHIRP_WINDOW : 0);
+    if (h->run_form == TDC_FORM_PLAIN && !h->chirp_path && h->series)
+        st.per_chirp = TDC_PER_TERM_SERIES;
     if (h->doppler && g.npx > 0) {
         unsigned long long c[2] = {0, 0};
         TDC_HIP(hipMemcpy(c, h->d_counters.p, sizeof c, hipMemcpyDeviceToHost));

@@ -174,7 +174,7 @@ void scan_chirps(const float *pos, const float *vel, const float *att, int ns, s
 {
     uint32_t bad = 0, dpos = 0, dvel = 0, datt = 0;
     float lo0 = s.lo[0], lo1 = s.lo[1], lo2 = s.lo[2], hi0 = s.hi[0], hi1 = s.hi[1], hi2 = s.hi[2];
-    double vmax2 = s.vmax2, qmax2 = s.qmax2;
+    double vmax2 = s.vmax2, qmax2 = s.qmax2, dmax2 = s.dmax2;
     for (size_t n = n0; n < n1; ++n) {
         const size_t e0 = n * static_cast<size_t>(ns);
         const float *p = pos + 3 * e0;
@@ -184,6 +184,9 @@ void scan_chirps(const float *pos, const float *vel, const float *att, int ns, s
             const uint32_t bx = bits_of(x), by = bits_of(y), bz = bits_of(z);
             bad |= nonfinite(bx) | nonfinite(by) | nonfinite(bz);
             dpos |= (bx ^ px) | (by ^ py) | (bz ^ pz);
+            const double ox = double(x) - double(p[0]), oy = double(y) - double(p[1]),
+                         oz = double(z) - double(p[2]);
+            dmax2 = std::max(dmax2, ox * ox + oy * oy + oz * oz);
             lo0 = std::min(lo0, x);
             lo1 = std::min(lo1, y);
             lo2 = std::min(lo2, z);
@@ -216,6 +219,7 @@ void scan_chirps(const float *pos, const float *vel, const float *att, int ns, s
             }
         }
     }
+    s.dmax2 = dmax2;
     s.finite = s.finite && bad == 0;
     s.pos_const = s.pos_const && dpos == 0;
     s.vel_const = s.vel_const && dvel == 0;
@@ -253,6 +257,7 @@ AntennaScan scan_antenna(const float *pos, const float *vel, const float *att, i
         out.att_const = out.att_const && part.att_const;
         out.vmax2 = std::max(out.vmax2, part.vmax2);
         out.qmax2 = std::max(out.qmax2, part.qmax2);
+        out.dmax2 = std::max(out.dmax2, part.dmax2);
         for (int a = 0; a < 3; ++a) {
             out.lo[a] = std::min(out.lo[a], part.lo[a]);
             out.hi[a] = std::max(out.hi[a], part.hi[a]);

@@ -192,11 +192,13 @@ hipError_t launch_stage_doppler(const float *raw, const float *pos, const float 
                                 const float *att, const float *dk, const float *wr, int npri,
                                 int ns, float two_over_lambda, const float *bore, int centroid,
                                 float fc_fixed, DopRec *out, hipStream_t s);
-hipError_t launch_plain(int ppl, const PlainRec *rec, const float4 *cpos, const float *grid,
-                        const double *kanchor, const Geom &g, float2 *out, hipStream_t s,
-                        int *workgroups);
-hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const double *kanchor,
-                          const Geom &g, const DopParams &dp, float2 *out,
+// `series`: the range offset within a chirp from the two-term expansion
+// (plain_tile; the host has bounded the antenna's motion within a chirp).
+hipError_t launch_plain(int ppl, bool series, const PlainRec *rec, const float4 *cpos,
+                        const float *grid, const double *kanchor, const Geom &g, float2 *out,
+                        hipStream_t s, int *workgroups);
+hipError_t launch_doppler(int ppl, bool series, const DopRec *rec, const float *grid,
+                          const double *kanchor, const Geom &g, const DopParams &dp, float2 *out,
                           unsigned long long *counters, hipStream_t s, int *workgroups);
 // Stop-and-hop records: terms (rr, ri, dk_m, 0) and per chirp the position of
 // its first sample; with `vel` non-null also (2/lambda_c) v and the window
@@ -339,13 +341,15 @@ void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_
 
 // One pass over per-sample antenna states [npri][ns][3|3|4] (vel and att may
 // be null): finiteness, the largest |v|^2 and |q|^2, the bounding box of the
-// positions, and whether each of pos, vel, att is bitwise the same for all
-// samples of every chirp.
+// positions, the largest squared distance of a sample's position from its
+// chirp's first, and whether each of pos, vel, att is bitwise the same for
+// all samples of every chirp.
 struct AntennaScan {
     bool has_vel = false, has_att = false;
     bool finite = true;
     bool pos_const = true, vel_const = true, att_const = true;
     double vmax2 = 0.0, qmax2 = 0.0;
+    double dmax2 = 0.0;
     float lo[3] = {0.0F, 0.0F, 0.0F}, hi[3] = {0.0F, 0.0F, 0.0F};
 };
 AntennaScan scan_antenna(const float *pos, const float *vel, const float *att, int npri, int ns);

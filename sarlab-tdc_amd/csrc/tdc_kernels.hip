@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void stage_chirps_kernel(
 // Plain form, Doppler window identically one (proved on the host).
 // ---------------------------------------------------------------------------
 
-template <int P>
+template <int P, bool SERIES>
 __global__ __launch_bounds__(kBlock) void plain_kernel(const PlainRec *__restrict__ rec,
                                                        const float4 *__restrict__ cpos,
                                                        const float *__restrict__ grid,
@@ -199,8 +199,8 @@ __global__ __launch_bounds__(kBlock) void plain_kernel(const PlainRec *__restric
     if (tile >= g.ntiles) return;
     const int n0 = split * g.cps;
     const int n1 = min(g.npri, n0 + g.cps);
-    plain_tile<P>(rec, cpos, grid, kanchor, g.npx, g.ns, n0, n1, tile * P * kBlock,
-                  out + static_cast<size_t>(split) * g.pstride);
+    plain_tile<P, SERIES>(rec, cpos, grid, kanchor, g.npx, g.ns, n0, n1, tile * P * kBlock,
+                          out + static_cast<size_t>(split) * g.pstride);
 }
 
 // ---------------------------------------------------------------------------
@@ -252,7 +252,7 @@ __device__ __forceinline__ bool chunk_processed(const DopRec *__restrict__ rec, 
     return mn <= dp.half_bw;
 }
 
-template <int P>
+template <int P, bool SERIES>
 __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restrict__ rec,
                                                          const float *__restrict__ grid,
                                                          const double *__restrict__ kanchor,
@@ -305,11 +305,12 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
                 // the phase's FP64 range from the chirp's first position (plain_tile)
                 const float cx = r[0].x, cy = r[0].y, cz = r[0].z;
                 double rd[P];
-                float r32[P], d0x[P], d0y[P], d0z[P];
+                float r32[P], hr[P], d0x[P], d0y[P], d0z[P];
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
                     rd[j] = range64(px[j], py[j], pz[j], cx, cy, cz);
                     r32[j] = static_cast<float>(rd[j]);
+                    hr[j] = static_cast<float>(0.5 / rd[j]);
                     d0x[j] = px[j] - cx;
                     d0y[j] = py[j] - cy;
                     d0z[j] = pz[j] - cz;
@@ -345,7 +346,14 @@ __global__ __launch_bounds__(kBlock) void doppler_kernel(const DopRec *__restric
                             // |d|^2 - R0^2 = -e . (d + d0), with d = d0 - e
                             const float num = -fmaf(ez, dz + d0z[j],
                                                     fmaf(ey, dy + d0y[j], ex * (dx + d0x[j])));
-                            const float dr = num * __builtin_amdgcn_rcpf(r2 * rs + r32[j]);
+                            // the offset of the range (plain_tile; the expansion with SERIES)
+                            float dr;
+                            if constexpr (SERIES) {
+                                const float x = num * hr[j];
+                                dr = fmaf(-(x * hr[j]), x, x);
+                            } else {
+                                dr = num * __builtin_amdgcn_rcpf(r2 * rs + r32[j]);
+                            }
                             const float th = __builtin_amdgcn_fractf(
                                 fmaf(t.k, r32[j] + dr, fmaf(kb32, dr, thb[j])));
                             const float cs = inband ? cos_rev(th) * wa : 0.0F;
@@ -930,24 +938,25 @@ hipError_t launch_stage_doppler(const float *raw, const float *pos, const float 
     return hipGetLastError();
 }
 
-hipError_t launch_plain(int ppl, const PlainRec *rec, const float4 *cpos, const float *grid,
-                        const double *kanchor, const Geom &g, float2 *out, hipStream_t s,
-                        int *workgroups)
+namespace {
+
+template <bool SERIES>
+hipError_t plain_launch(int ppl, int nb, const PlainRec *rec, const float4 *cpos,
+                        const float *grid, const double *kanchor, const Geom &g, float2 *out,
+                        hipStream_t s)
 {
-    const int nb = correlate_grid(g);
-    *workgroups = nb;
     switch (ppl) {
     case 1:
-        hipLaunchKernelGGL(plain_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
-                           g, out);
+        hipLaunchKernelGGL((plain_kernel<1, SERIES>), dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid,
+                           kanchor, g, out);
         break;
     case 2:
-        hipLaunchKernelGGL(plain_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
-                           g, out);
+        hipLaunchKernelGGL((plain_kernel<2, SERIES>), dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid,
+                           kanchor, g, out);
         break;
     case 4:
-        hipLaunchKernelGGL(plain_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid, kanchor,
-                           g, out);
+        hipLaunchKernelGGL((plain_kernel<4, SERIES>), dim3(nb), dim3(kBlock), 0, s, rec, cpos, grid,
+                           kanchor, g, out);
         break;
     default:
         return hipErrorInvalidValue;
@@ -955,29 +964,50 @@ hipError_t launch_plain(int ppl, const PlainRec *rec, const float4 *cpos, const 
     return hipGetLastError();
 }
 
-hipError_t launch_doppler(int ppl, const DopRec *rec, const float *grid, const double *kanchor,
-                          const Geom &g, const DopParams &dp, float2 *out,
-                          unsigned long long *counters, hipStream_t s, int *workgroups)
+template <bool SERIES>
+hipError_t doppler_launch(int ppl, int nb, const DopRec *rec, const float *grid,
+                          const double *kanchor, const Geom &g, const DopParams &dp, float2 *out,
+                          unsigned long long *counters, hipStream_t s)
 {
-    const int nb = correlate_grid(g);
-    *workgroups = nb;
     switch (ppl) {
     case 1:
-        hipLaunchKernelGGL(doppler_kernel<1>, dim3(nb), dim3(kBlock), 0, s, rec, grid, kanchor, g,
-                           dp, out, counters);
+        hipLaunchKernelGGL((doppler_kernel<1, SERIES>), dim3(nb), dim3(kBlock), 0, s, rec, grid,
+                           kanchor, g, dp, out, counters);
         break;
     case 2:
-        hipLaunchKernelGGL(doppler_kernel<2>, dim3(nb), dim3(kBlock), 0, s, rec, grid, kanchor, g,
-                           dp, out, counters);
+        hipLaunchKernelGGL((doppler_kernel<2, SERIES>), dim3(nb), dim3(kBlock), 0, s, rec, grid,
+                           kanchor, g, dp, out, counters);
         break;
     case 4:
-        hipLaunchKernelGGL(doppler_kernel<4>, dim3(nb), dim3(kBlock), 0, s, rec, grid, kanchor, g,
-                           dp, out, counters);
+        hipLaunchKernelGGL((doppler_kernel<4, SERIES>), dim3(nb), dim3(kBlock), 0, s, rec, grid,
+                           kanchor, g, dp, out, counters);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_plain(int ppl, bool series, const PlainRec *rec, const float4 *cpos,
+                        const float *grid, const double *kanchor, const Geom &g, float2 *out,
+                        hipStream_t s, int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    return series ? plain_launch<true>(ppl, nb, rec, cpos, grid, kanchor, g, out, s)
+                  : plain_launch<false>(ppl, nb, rec, cpos, grid, kanchor, g, out, s);
+}
+
+hipError_t launch_doppler(int ppl, bool series, const DopRec *rec, const float *grid,
+                          const double *kanchor, const Geom &g, const DopParams &dp, float2 *out,
+                          unsigned long long *counters, hipStream_t s, int *workgroups)
+{
+    const int nb = correlate_grid(g);
+    *workgroups = nb;
+    return series ? doppler_launch<true>(ppl, nb, rec, grid, kanchor, g, dp, out, counters, s)
+                  : doppler_launch<false>(ppl, nb, rec, grid, kanchor, g, dp, out, counters, s);
 }
 
 hipError_t launch_stage_terms(const float *raw, const float *dk, const float *wr, int npri, int ns,

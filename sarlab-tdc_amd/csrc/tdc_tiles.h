@@ -168,7 +168,12 @@ struct RawBF16 {
 // dR = num / (|p - pos_m| + R0), formed from the small delta without
 // cancellation; then theta_m = fract(theta_b + k_b dR + dk_m (R0 + dR)). For
 // delta = 0 (stop-and-hop input) every extra term is an exact zero and the
-// phase is bit-identical to the per-chirp form above.
+// phase is bit-identical to the per-chirp form above. When the host has shown
+// that the antenna moves little within a chirp against the shortest range
+// (SERIES: k |delta|^3 / R^2 below 1e-7 cycles), dR comes from the first two
+// terms of the expansion of the square root, dR = x - x^2 / (2 R0) with
+// x = num / (2 R0), whose remainder x^3 / (2 R0^2) is below that bound: no
+// square root and no reciprocal per term.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double range64(double px, double py, double pz, double cx, double cy,
@@ -193,7 +198,7 @@ __device__ __forceinline__ float anchor_phase(double kb, double r)
 // [ceil(ns / kAnchorBlock)] FP64 anchor wavenumbers.
 constexpr int kStageRecords = 256;
 
-template <int P>
+template <int P, bool SERIES>
 __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
                                            const float4 *__restrict__ cpos,
                                            const float *__restrict__ grid,
@@ -218,12 +223,13 @@ __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
     for (int n = n0; n < n1; ++n) {
         const float4 c = as_const(cpos)[n];
         double rd[P];
-        float r32[P], r0sq[P], mx[P], my[P], mz[P];
+        float r32[P], r0sq[P], hr[P], mx[P], my[P], mz[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             rd[j] = range64(px[j], py[j], pz[j], c.x, c.y, c.z);
             r32[j] = static_cast<float>(rd[j]);
             r0sq[j] = static_cast<float>(rd[j] * rd[j]);
+            hr[j] = static_cast<float>(0.5 / rd[j]);
             // -2 d0, d0 = p - c
             mx[j] = static_cast<float>(-2.0 * (px[j] - c.x));
             my[j] = static_cast<float>(-2.0 * (py[j] - c.y));
@@ -251,8 +257,14 @@ __device__ __forceinline__ void plain_tile(const PlainRec *__restrict__ rec,
                     for (int j = 0; j < P; ++j) {
                         const float num =
                             fmaf(b.x, b.x + mz[j], fmaf(a.w, a.w + my[j], a.z * (a.z + mx[j])));
-                        const float rm = __builtin_amdgcn_sqrtf(r0sq[j] + num);
-                        const float dr = num * __builtin_amdgcn_rcpf(rm + r32[j]);
+                        float dr;
+                        if constexpr (SERIES) {
+                            const float x = num * hr[j];
+                            dr = fmaf(-(x * hr[j]), x, x);
+                        } else {
+                            const float rm = __builtin_amdgcn_sqrtf(r0sq[j] + num);
+                            dr = num * __builtin_amdgcn_rcpf(rm + r32[j]);
+                        }
                         const float th = __builtin_amdgcn_fractf(
                             fmaf(b.y, r32[j] + dr, fmaf(kb32, dr, thb[j])));
                         const float sn = sin_rev(th);

@@ -60,6 +60,11 @@ tdc::AntennaScan serial_scan(const float *pos, const float *vel, const float *at
                 if (v > s.hi[a]) s.hi[a] = v;
             }
             if (std::memcmp(pos + 3 * e, pos + 3 * e0, 12) != 0) s.pos_const = false;
+            const double ox = double(pos[3 * e]) - double(pos[3 * e0]),
+                         oy = double(pos[3 * e + 1]) - double(pos[3 * e0 + 1]),
+                         oz = double(pos[3 * e + 2]) - double(pos[3 * e0 + 2]);
+            const double d2 = ox * ox + oy * oy + oz * oz;
+            if (d2 > s.dmax2) s.dmax2 = d2;
             if (vel) {
                 double v2 = 0.0;
                 for (int a = 0; a < 3; ++a) {
@@ -89,6 +94,7 @@ bool same(const tdc::AntennaScan &a, const tdc::AntennaScan &b)
     return a.has_vel == b.has_vel && a.has_att == b.has_att && a.finite == b.finite &&
            a.pos_const == b.pos_const && a.vel_const == b.vel_const &&
            a.att_const == b.att_const && a.vmax2 == b.vmax2 && a.qmax2 == b.qmax2 &&
+           a.dmax2 == b.dmax2 &&
            std::memcmp(a.lo, b.lo, sizeof a.lo) == 0 && std::memcmp(a.hi, b.hi, sizeof a.hi) == 0;
 }
 

@@ -88,6 +88,35 @@ def moving_within_chirp(d, s):
 # ---------------------------------------------------------------------------- plain form
 
 
+@pytest.mark.parametrize("motion,series", [(3e-4, True), (0.05, False)])
+def test_plain_motion_within_chirps(tdc, orc, motion, series):
+    """The antenna moves along x during every chirp (per-sample positions): by
+    0.3 mm the range offset from the chirp's first position takes the two-term
+    expansion, by 5 cm at 0.6 m the exact form num / (R_m + R0); both against
+    the oracle at the metric's bound with the Doppler window off, and with it
+    on at the windowed tolerance."""
+    s = ragged_scene(npri=90, ns=128, rows=11, cols=13, seed=int(motion * 1e5))
+    d = s.dropin()
+    t = (s.sample_times - s.sample_times[0]) / (s.sample_times[-1] - s.sample_times[0])
+    pos = d["position"].astype(np.float64)
+    pos[:, :, 0] += motion * t[None, :]
+    d["position"] = np.ascontiguousarray(pos.astype(np.float32))
+    vel = np.zeros_like(d["velocity"])
+    vel[..., 0] = 5.0
+    d["velocity"] = vel
+    img, st = form(tdc, d)
+    assert not st["range_per_chirp"] and st["range_series"] == series
+    check(orc, img, oracle_dropin(orc, d), PLAIN_TOL)
+    w = dict(velocity=d["velocity"], attitude=d["attitude"], boresight=d["bodyBoresight"],
+             center=0.0, bandwidth=2000.0, centroid=True, taper=True)
+    img, st = form(tdc, d, center=0.0, bw=2000.0, centroid=True, taper=True)
+    assert st["doppler_active"] and st["range_series"] == series
+    # the window's band edge may fall between the FP32 and FP64 Doppler
+    # frequencies of a term (as in the random drop-in tests)
+    check(orc, img, oracle_dropin(orc, d, doppler=w), 5e-3)
+
+
+
 def test_plain_line_small(tdc, orc):
     from sarlabtdc import scene
 
