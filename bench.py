@@ -181,10 +181,10 @@ def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=No
                 workgroups=st["workgroups"], image=img, keep=keep)
 
 
-def served_kernel(ns):
+def served_kernel(ns, npx):
     """(taps, oversampling) the server's range-profile tiles run for chirps of
-    ns samples: the library's automatic choice."""
-    return (8, 4) if profile_length(ns) * 2 <= 16384 else (10, 2)
+    ns samples and npx pixels: the library's automatic choice."""
+    return (8, 4) if profile_length(ns) * 2 <= 16384 and npx >= 32768 else (10, 2)
 
 
 def serve_stream(tdc, scenes, rate, form, storage="c64", seed=0, budget=200e9):
@@ -239,7 +239,7 @@ def serve_stream(tdc, scenes, rate, form, storage="c64", seed=0, budget=200e9):
         nc, ns, npx = sc.n_chirps, sc.n_samples, sc.n_pixels
         aperture_ms.append(nc * PRI_S * 1e3)
         if form == "precision":
-            taps, over = served_kernel(ns)
+            taps, over = served_kernel(ns, npx)
             m = profile_length(ns) * over // 2
             pairs = float(npx) * nc * profile_fp32eq_per_pair(taps)
             executed += pairs + nc * 5.0 * m * np.log2(m)

@@ -696,7 +696,7 @@ int precision_inputs(tdc_processor *h, float dop_center, float dop_bw, bool cent
         if (h->force_ppl > 1)
             return fail(TDC_ERR_INVALID_ARGUMENT, "the range-profile engine runs one pixel per lane");
         tdc::ProfileChoice pc;
-        if (!tdc::profile_choice(ns, h->profile_taps, h->profile_over, pc, why))
+        if (!tdc::profile_choice(ns, npx, h->profile_taps, h->profile_over, pc, why))
             return fail(TDC_ERR_UNSUPPORTED, why);
         in.plen = pc.M;
         in.tabs = tdc::profile_tables(ns, in.plen, pc.taps, pc.over);

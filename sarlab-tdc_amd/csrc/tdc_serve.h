@@ -11,9 +11,11 @@ namespace tdc {
 // engine's tile for the precision form, one pixel per lane for the plain form's
 // per-chirp range tile).
 constexpr int kServeTilePixels = kBlock;
-// Capacity of one step.
+// Capacity of one step: tiles, finalize blocks, and runs of one job's tiles
+// (a workgroup reads a step's runs of its tile family into LDS once).
 constexpr int kMaxStepTiles = 32768;
 constexpr int kMaxStepFin = 32768;
+constexpr int kMaxStepRuns = 256;
 constexpr int kMaxJobSlots = 8192;
 
 // One admitted job as the kernels see it.
@@ -31,15 +33,19 @@ struct alignas(16) ServeJob {
     double dk;                // precision, matrix engine: 2 K dt / c
     ProfileParams pp;         // precision, range-profile engine
     int npx, npri, ns, nsplit, cps, form, storage, profile;
+    int npt;                  // pixel tiles of kServeTilePixels pixels
+    int pad;
 };
 
-// A tile names its job by the device address of the job's descriptor, which
-// travels to HBM in the job's own upload (no per-tile read of host memory
-// beyond the 16-byte tile entry).
-struct ServeTile {
+// A run of consecutive tiles of one job in a step: the step's tiles
+// [first, first + count) of the run's family are the job's own tiles
+// [tile0, tile0 + count), job tile t being pixel tile t % npt of chirp range
+// t / npt. A run names its job by the device address of the job's
+// descriptor, which travels to HBM in the job's own upload.
+struct StepRun {
     const ServeJob *job;
-    int pix0;  // first pixel of the tile
-    int split; // chirp range index
+    int first;
+    int count;
 };
 
 struct FinTile {
@@ -49,15 +55,15 @@ struct FinTile {
 };
 
 struct StepTable {
-    int ntiles; // tiles[0, nfast): plain and range-profile; [nfast, ntiles): matrix engine
+    int ntiles[2]; // tiles of family 0 (plain and range-profile) and 1 (matrix engine)
+    int nruns[2];  // runs[0, nruns[0]): family 0; [nruns[0], nruns[0] + nruns[1]): family 1
     int nfin;
-    int nfast;
     int pad;
     // the values the two tile counters hold when this step starts: counters
     // are never reset, each step takes tile t at counter value base + t
     unsigned int base[2];
-    int pad2[2];
-    ServeTile tiles[kMaxStepTiles];
+    StepRun runs[kMaxStepRuns];
+    int tile0[kMaxStepRuns]; // the job tile of each run's first tile
     FinTile fin[kMaxStepFin];
 };
 

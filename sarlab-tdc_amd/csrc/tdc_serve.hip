@@ -25,6 +25,20 @@
 namespace tdc {
 namespace {
 
+// A pointer that is the same in every lane, as the compiler's uniform value.
+template <class T> __device__ __forceinline__ const T *uniform_ptr(const T *p)
+{
+    const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+    const unsigned int lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(a));
+    const unsigned int hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(a >> 32));
+    return reinterpret_cast<const T *>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+struct ServeTileIndex {
+    int pix0;  // first pixel of the tile
+    int split; // chirp range index
+};
+
 // FAMILY 0 runs the table's first nfast tiles (plain form and range-profile
 // engine, at most ~62 VGPRs, 8 waves per SIMD), FAMILY 1 the rest (matrix
 // engine, which needs ~130 VGPRs); each family has its own kernel so that
@@ -33,20 +47,46 @@ template <int FAMILY>
 __global__ __launch_bounds__(kBlock) void serve_kernel(const StepTable *__restrict__ table,
                                                        unsigned int *__restrict__ counter)
 {
-    __shared__ unsigned int s_tile;
+    __shared__ int s_first[kMaxStepRuns];
+    __shared__ int s_tile0[kMaxStepRuns];
+    __shared__ const ServeJob *s_job[kMaxStepRuns];
+    __shared__ unsigned int s_next;
     const cptr<StepTable> tab = as_const(table);
-    const int first = FAMILY == 0 ? 0 : tab->nfast;
-    const unsigned int count =
-        static_cast<unsigned int>(FAMILY == 0 ? tab->nfast : tab->ntiles - tab->nfast);
+    const unsigned int count = static_cast<unsigned int>(tab->ntiles[FAMILY]);
+    const int r0 = FAMILY == 0 ? 0 : tab->nruns[0];
+    const int nruns = tab->nruns[FAMILY];
     const unsigned int base = tab->base[FAMILY];
+    // the family's runs, read once from the step table
+    for (int r = static_cast<int>(threadIdx.x); r < nruns; r += kBlock) {
+        const StepRun run = table->runs[r0 + r];
+        s_first[r] = run.first;
+        s_tile0[r] = table->tile0[r0 + r];
+        s_job[r] = run.job;
+    }
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter + FAMILY, 1U) - base;
+        // the next tile (taking it when the previous one has ended keeps the
+        // tiles balanced over the workgroups to the last one)
+        if (threadIdx.x == 0) s_next = atomicAdd(counter + FAMILY, 1U) - base;
         __syncthreads();
-        const unsigned int t = __builtin_amdgcn_readfirstlane(s_tile);
+        const unsigned int t = __builtin_amdgcn_readfirstlane(s_next);
         __syncthreads();
         if (t >= count) break; // every wave of the workgroup takes this exit
-        const ServeTile tl = tab->tiles[first + static_cast<int>(t)];
-        const ServeJob j = as_const(tl.job)[0];
+        // the run holding tile t: the last run whose first tile is <= t
+        int lo = 0, hi = nruns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (static_cast<unsigned int>(s_first[mid]) <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        // values read from LDS are uniform here: made so for the compiler, which
+        // then keeps the job's descriptor and every loop bound in SGPRs
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        const int jt = __builtin_amdgcn_readfirstlane(s_tile0[lo]) + static_cast<int>(t) -
+                       __builtin_amdgcn_readfirstlane(s_first[lo]);
+        const ServeJob j = as_const(uniform_ptr(s_job[lo]))[0];
+        const int split = jt / j.npt;
+        const int pix0 = (jt - split * j.npt) * kServeTilePixels;
+        const ServeTileIndex tl{pix0, split};
         const int n0 = tl.split * j.cps;
         const int n1 = min(j.npri, n0 + j.cps);
         if (FAMILY == 0) {
@@ -60,8 +100,8 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const StepTable *__restri
                 const auto *tab = static_cast<const float2 *>(j.raw);
                 double2 *out = static_cast<double2 *>(j.partial) +
                                static_cast<size_t>(tl.split) * j.npx;
-                // served jobs run the automatic kernel: (8, 4), or (10, 2) for chirps
-                // whose fourfold transform would not fit
+                // served jobs run the automatic kernel: (8, 4), or (10, 2) for small
+                // images and chirps whose fourfold transform would not fit
                 if (j.pp.over == 4)
                     profile_tile<8, 4>(tab, j.cpos, j.grid64, j.pp, j.npx, n0, n1, tl.pix0, out);
                 else

@@ -309,7 +309,7 @@ void plan(Job &j)
     j.npx = static_cast<int>(npx);
     const bool prec = d.form == TDC_FORM_PRECISION;
     std::string why;
-    j.profile = prec && tdc::profile_choice(d.n_samples, 0, 0, j.pk, why);
+    j.profile = prec && tdc::profile_choice(d.n_samples, static_cast<int64_t>(npx), 0, 0, j.pk, why);
     const int plen = j.pk.M;
     j.pair_cost = j.profile ? kProfilePairCost : (prec ? 1.0 : kPlainTermCost) * d.n_samples;
     j.tile_px = tdc::kServeTilePixels;
@@ -519,6 +519,7 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
     sj.ns = d.n_samples;
     sj.nsplit = j.nsplit;
     sj.cps = j.cps;
+    sj.npt = j.npt;
     sj.form = prec ? 1 : 0;
     sj.storage = d.raw_storage;
     sj.partial = j.arena + j.off_part;
@@ -749,8 +750,15 @@ void scheduler(tdc_server *s)
         st.buf = static_cast<int>(step_index % kDepth);
         tdc::StepTable *tab = s->h_table[st.buf];
         int nt = 0, nf = 0;
-        // matrix-engine tiles go after the others (their own persistent kernel)
-        std::vector<tdc::ServeTile> mtiles;
+        // runs of consecutive tiles of one job, per tile family: 0 plain and
+        // range-profile tiles, 1 matrix-engine tiles (their own persistent kernel)
+        struct Run {
+            const tdc::ServeJob *job;
+            int first, count;
+            int64_t tile0;
+        };
+        std::vector<Run> runs[2];
+        int nfam[2] = {0, 0};
         double work = 0.0;
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -764,14 +772,22 @@ void scheduler(tdc_server *s)
             auto take_tiles = [&](Job *j, double limit) {
                 double got = 0.0;
                 bool took = false;
+                const int fam = (j->d.form == TDC_FORM_PRECISION && !j->profile) ? 1 : 0;
+                std::vector<Run> &rr = runs[fam];
+                // the job's tiles continue its last run of this step, or open a new
+                // run while the step has room for one
+                const bool extend = !rr.empty() && rr.back().job == j->desc() &&
+                                    rr.back().tile0 + rr.back().count == j->next_tile;
+                if (!extend && runs[0].size() + runs[1].size() >= size_t(tdc::kMaxStepRuns))
+                    return;
+                if (!extend) rr.push_back(Run{j->desc(), nfam[fam], 0, j->next_tile});
                 while (j->next_tile < j->ntiles && nt < tdc::kMaxStepTiles &&
                        (got < limit || !took) && (work < s->step_terms || nt == 0)) {
                     const int64_t t = j->next_tile++;
                     const int split = static_cast<int>(t / j->npt);
                     const int ptile = static_cast<int>(t % j->npt);
-                    const tdc::ServeTile tl{j->desc(), ptile * j->tile_px, split};
-                    if (j->d.form == TDC_FORM_PRECISION && !j->profile) mtiles.push_back(tl);
-                    else tab->tiles[nt - static_cast<int>(mtiles.size())] = tl;
+                    ++rr.back().count;
+                    ++nfam[fam];
                     ++nt;
                     const int n0 = split * j->cps;
                     const int nch = std::min(j->d.n_pri, n0 + j->cps) - n0;
@@ -781,6 +797,7 @@ void scheduler(tdc_server *s)
                     got += w;
                     took = true;
                 }
+                if (!took && !extend) rr.pop_back();
                 if (took) {
                     if (j->t_first < 0) {
                         j->t_first = s->now();
@@ -819,9 +836,18 @@ void scheduler(tdc_server *s)
             }
             continue;
         }
-        tab->nfast = nt - static_cast<int>(mtiles.size());
-        std::copy(mtiles.begin(), mtiles.end(), tab->tiles + tab->nfast);
-        tab->ntiles = nt;
+        {
+            int r = 0;
+            for (int fam = 0; fam < 2; ++fam) {
+                tab->ntiles[fam] = nfam[fam];
+                tab->nruns[fam] = static_cast<int>(runs[fam].size());
+                for (const Run &run : runs[fam]) {
+                    tab->runs[r] = tdc::StepRun{run.job, run.first, run.count};
+                    tab->tile0[r] = static_cast<int>(run.tile0);
+                    ++r;
+                }
+            }
+        }
         tab->nfin = nf;
         tab->base[0] = s->counter_next[st.buf][0];
         tab->base[1] = s->counter_next[st.buf][1];
@@ -834,7 +860,7 @@ void scheduler(tdc_server *s)
         }
         // a step without matrix-engine tiles replays the graph without their
         // (otherwise empty) persistent kernel
-        const int matrix = mtiles.empty() ? 0 : 1;
+        const int matrix = nfam[1] > 0 ? 1 : 0;
         if (!ensure_graph(s, matrix, bucket, st.buf, err)) return die(err);
         tdc::Range range("tdc_server_launch_step");
         const int p = st.buf;
@@ -844,10 +870,8 @@ void scheduler(tdc_server *s)
         if (e == hipSuccess) e = hipGraphLaunch(s->graph[matrix][bucket][p], ss);
         // every workgroup of a family's kernel takes one counter value past the
         // family's tiles
-        s->counter_next[p][0] += static_cast<unsigned int>(tab->nfast + kBuckets[bucket]);
-        if (matrix)
-            s->counter_next[p][1] +=
-                static_cast<unsigned int>(nt - tab->nfast + kBuckets[bucket]);
+        s->counter_next[p][0] += static_cast<unsigned int>(nfam[0] + kBuckets[bucket]);
+        if (matrix) s->counter_next[p][1] += static_cast<unsigned int>(nfam[1] + kBuckets[bucket]);
         if (e == hipSuccess) e = hipEventRecord(s->ev_end[p], ss);
         // finalize on the same stream, right behind this step, once the previous
         // step (the other stream) has ended too: the two cover every earlier

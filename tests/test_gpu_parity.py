@@ -383,17 +383,18 @@ def test_profile_engine_kernels(tdc, orc, taps, over, ns):
 
 
 def test_profile_kernel_choice(tdc):
-    """The automatic kernel: (8, 4) up to 4096 samples per chirp, (10, 2) above;
-    a kernel that is not built, or fourfold oversampling beyond 4096 samples,
-    is refused."""
+    """The automatic kernel: (8, 4) up to 4096 samples per chirp for images of
+    at least 32768 pixels, (10, 2) for smaller images or longer chirps; a
+    kernel that is not built, or fourfold oversampling beyond 4096 samples, is
+    refused."""
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
     with pytest.raises(ValueError):
         p.setProfileKernel(6, 4)
     with pytest.raises(ValueError):
         p.setProfileKernel(10, 4)
-    for ns, want in ((4096, (8, 4)), (4097, (10, 2))):
-        s = ragged_scene(npri=4, ns=ns, rows=2, cols=3, seed=ns)
+    for ns, side, want in ((4096, 182, (8, 4)), (4096, 181, (10, 2)), (4097, 182, (10, 2))):
+        s = ragged_scene(npri=4, ns=ns, rows=side, cols=side, seed=ns + side)
         f = s.f64()
         p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"],
                         f["modRate"], f["startFreq"])
