@@ -187,19 +187,20 @@ def served_kernel(ns, npx):
     return (8, 4) if profile_length(ns) * 2 <= 16384 and npx >= 32768 else (10, 2)
 
 
-def serve_stream(tdc, scenes, rate, form, storage="c64", seed=0, budget=200e9):
-    """Poisson arrivals at `rate` jobs/s into one server; returns per-job records."""
+def serve_stream(tdc, srv, scenes, rate, form, storage="c64", seed=0):
+    """Poisson arrivals at `rate` jobs/s into the server `srv` (idle when
+    called); returns the stream's record and per-job results."""
     import threading
 
     rng = np.random.default_rng([5, seed, 1])
     gaps = rng.exponential(1.0 / rate, size=len(scenes))
     arrivals = np.cumsum(gaps) - gaps[0]
-    srv = tdc.Server(0, budget)
-    # warm-up: one job per grid bucket shape class, not counted
+    # warm-up: three jobs, not counted
     for s in scenes[:3]:
         f = s.f64()
         srv.wait(srv.submit(f["rawData"], f["sampleTimes"], f["position"], f["focusGrid"],
                             f["modRate"], f["startFreq"], form=form, storage=storage))
+    st0 = srv.stats()
     inputs = [s.f64() for s in scenes]
     ids = [None] * len(scenes)
     results = [None] * len(scenes)
@@ -222,7 +223,6 @@ def serve_stream(tdc, scenes, rate, form, storage="c64", seed=0, budget=200e9):
         ready.release()
     th.join()
     st = srv.stats()
-    srv.close()
     lat = np.array([r[1]["latency_ms"] for r in results])
     t_sub = min(r[1]["t_submit"] for r in results)
     t_end = max(r[1]["t_done"] for r in results)
@@ -248,12 +248,13 @@ def serve_stream(tdc, scenes, rate, form, storage="c64", seed=0, budget=200e9):
             executed += float(npx) * nc * ns * EXEC_FLOP_PER_TERM["range"]
             in_steps = executed
     direct = terms * DIRECT_FLOP_PER_TERM[form]
-    busy_s = st["device_busy_ms"] / 1e3
+    busy_s = (st["device_busy_ms"] - st0["device_busy_ms"]) / 1e3
     realtime = float(np.mean(lat <= np.array(aperture_ms)))
     return dict(form=form, storage=storage, jobs=len(scenes), rate=rate,
                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 mean_ms=float(lat.mean()), max_ms=float(lat.max()), makespan_s=makespan,
-                gterms=terms / makespan / 1e9, terms=terms, steps=st["steps"],
+                gterms=terms / makespan / 1e9, terms=terms, steps=st["steps"] - st0["steps"],
+                # the server's peak device footprint so far (all streams up to this one)
                 device_busy_s=busy_s, bytes_peak=st["bytes_peak"], graphs=st["graphs"],
                 # executed FP32(-equivalent) flops over the makespan, and the
                 # steps' own flops over the steps' device time, against the FP32
@@ -355,7 +356,13 @@ def serve_leg(tdc, args):
         if (j + 1) % 200 == 0:
             log(f"serving: generated {j + 1}/{len(shapes)} jobs ({time.perf_counter() - t0:.1f} s)")
     out = {}
-    rec, res = serve_stream(tdc, scenes, args.serve_rate, "precision")
+    # one server for every stream, as deployed, warmed by one unmeasured pass
+    # of the stream offered all at once: its pinned-memory and device-arena
+    # pools then hold what the streams need (a cold server's first burst pays
+    # those allocations, and uploads stalled for tens of ms meanwhile, measured)
+    srv = tdc.Server(0, 200e9)
+    serve_stream(tdc, srv, scenes, 1e9, "precision", seed=9)
+    rec, res = serve_stream(tdc, srv, scenes, args.serve_rate, "precision")
     log(f"serving precision: {rec['jobs']} jobs at {rec['rate']:.0f}/s: p50 {rec['p50_ms']:.2f} ms, "
         f"p99 {rec['p99_ms']:.2f} ms, {rec['gterms']:.1f} G terms/s over {rec['makespan_s']:.2f} s "
         f"(device busy {rec['device_busy_s']:.2f} s, {rec['steps']} steps)")
@@ -363,7 +370,7 @@ def serve_leg(tdc, args):
     if args.serve_capacity_rate > 0:
         # the same jobs offered faster, to show throughput and latency near the
         # device's capacity
-        rec_c, _ = serve_stream(tdc, scenes, args.serve_capacity_rate, "precision", seed=1)
+        rec_c, _ = serve_stream(tdc, srv, scenes, args.serve_capacity_rate, "precision", seed=1)
         log(f"serving precision at {rec_c['rate']:.0f}/s: p50 {rec_c['p50_ms']:.2f} ms, "
             f"p99 {rec_c['p99_ms']:.2f} ms, {rec_c['gterms']:.1f} G terms/s over "
             f"{rec_c['makespan_s']:.2f} s (device busy {rec_c['device_busy_s']:.2f} s)")
@@ -371,7 +378,7 @@ def serve_leg(tdc, args):
     if args.serve_saturation_rate > 0:
         # the same jobs offered faster than the device forms them: the makespan
         # is the server's, so the executed fraction is the device-bound one
-        rec_s, _ = serve_stream(tdc, scenes, args.serve_saturation_rate, "precision", seed=2)
+        rec_s, _ = serve_stream(tdc, srv, scenes, args.serve_saturation_rate, "precision", seed=2)
         log(f"serving precision at {rec_s['rate']:.0f}/s: p50 {rec_s['p50_ms']:.2f} ms, "
             f"p99 {rec_s['p99_ms']:.2f} ms, {rec_s['gterms']:.1f} G terms/s over "
             f"{rec_s['makespan_s']:.3f} s (device busy {rec_s['device_busy_s']:.3f} s, executed "
@@ -379,10 +386,11 @@ def serve_leg(tdc, args):
         out["precision_saturation"] = rec_s
     n_plain = min(len(scenes), args.serve_plain_jobs)
     if n_plain > 0:
-        rec_p, _ = serve_stream(tdc, scenes[:n_plain], args.serve_rate, "plain")
+        rec_p, _ = serve_stream(tdc, srv, scenes[:n_plain], args.serve_rate, "plain")
         log(f"serving plain: {rec_p['jobs']} jobs at {rec_p['rate']:.0f}/s: p50 "
             f"{rec_p['p50_ms']:.2f} ms, p99 {rec_p['p99_ms']:.2f} ms, {rec_p['gterms']:.1f} G terms/s")
         out["plain"] = rec_p
+    srv.close()
     # accuracy of a few served jobs against the FP64 oracle (checker only)
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

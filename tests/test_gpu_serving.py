@@ -219,8 +219,10 @@ def test_arenas_are_recycled_within_the_budget(tdc):
 
 
 def test_small_jobs_overtake_a_large_one(tdc):
-    """Jobs submitted while a large one runs join it on the device and finish first."""
-    big = job_scene(30, 2048, 512, 768)
+    """Jobs submitted while a large one runs join it on the device and finish first
+    (the large job, 2048 chirps over 1536^2 pixels, takes ~10 ms of device time
+    in several steps, so the small ones arrive well before its last step)."""
+    big = job_scene(30, 2048, 512, 1536, spacing=1.5e-3)
     small = [job_scene(31 + k, 64, 128, 32) for k in range(5)]
     srv = tdc.Server(0, 64e9)
     # warm every upload thread, its stream and the pinned pools
