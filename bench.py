@@ -177,7 +177,7 @@ def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=No
                 splits=st["chirp_splits"], ppl=st["pixels_per_lane"], engine=st["engine"],
                 doppler_active=st["doppler_active"], skipped_chunks=st["skipped_chunks"],
                 processed_chunks=st["processed_chunks"], window_per_chirp=st["window_per_chirp"],
-                range_per_chirp=st["range_per_chirp"],
+                range_per_chirp=st["range_per_chirp"], range_series=st["range_series"],
                 workgroups=st["workgroups"], image=img, keep=keep)
 
 
@@ -701,7 +701,8 @@ def main():
                     e2e_ms=round(r["e2e_ms"], 3), e2e_first_ms=round(r["e2e_first_ms"], 3),
                     gterms_e2e_pcie=round(r["gterms_e2e"], 3),
                     chirp_splits=r["splits"], engine=r["engine"],
-                    **({"range_per_chirp": bool(r["range_per_chirp"])} if r["kind"] == "plain" else {}),
+                    **({"range_per_chirp": bool(r["range_per_chirp"]),
+                        "range_series": bool(r["range_series"])} if r["kind"] == "plain" else {}),
                     **({"skipped_chunks": r["skipped_chunks"],
                         "processed_chunks": r["processed_chunks"],
                         "window_per_chirp": bool(r["window_per_chirp"])}

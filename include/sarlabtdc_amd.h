@@ -48,10 +48,11 @@ extern "C" {
 /* Engines of the precision form (additive). All compute the same sums to
  * FP32-level accuracy. The range-profile engine takes each chirp's
  * oversampled spectrum once (an FFT in LDS) and interpolates it at every
- * pixel's range with a 12-tap Gaussian kernel (a type-2 non-uniform FFT; up
- * to 8192 samples per chirp). The matrix engine forms the 16-sample group
- * sums of every 128-sample block with FP16 hi/lo MFMA (32x32x16). The vector
- * engine runs a one-sample rotation recurrence on the VALU. */
+ * pixel's range with an 8- to 12-tap Gaussian kernel (a type-2 non-uniform
+ * FFT; up to 8192 samples per chirp; see tdc_set_profile_kernel). The matrix
+ * engine forms the 16-sample group sums of every 128-sample block with FP16
+ * hi/lo MFMA (32x32x16). The vector engine runs a one-sample rotation
+ * recurrence on the VALU. */
 #define TDC_ENGINE_AUTO 0    /* range profile when the chirp has at most 8192 samples, else
                                 matrix; vector when pixels_per_lane > 1 is forced */
 #define TDC_ENGINE_VECTOR 1
@@ -241,8 +242,7 @@ int tdc_set_device_budget(tdc_processor *h, unsigned long long bytes);
  * reports it); with that count forced on tdc_start (tdc_set_launch) the two
  * images are bit-identical. Inputs too large for the array, even with one
  * split, give TDC_ERR_INVALID_ARGUMENT and a message with the bytes needed
- * (config 3, 1024 chirps x 512 samples x 512^2 pixels, needs ~55 MB with 8
- * splits). Calls on one GPU share the array and are serialised inside the
+ * (1024 chirps x 512 samples x 512^2 pixels needs ~55 MB with 8 splits). Calls on one GPU share the array and are serialised inside the
  * process; the array keeps the last call's contents. Blocks until the image
  * is in image_iq. The processor's prepared state (tdc_prepare) must be
  * prepared again afterwards. */

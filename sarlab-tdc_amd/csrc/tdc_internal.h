@@ -122,9 +122,8 @@ inline double profile_bfactor(int taps, int over)
     return taps == 12 ? 0.79 : 0.80;
 }
 // The FFT of one chirp runs in the LDS, of which a gfx950 workgroup may
-// address 160 KiB: radix-2 Stockham between two buffers (2 M entries) up to
-// M = 8192, in place in one buffer (M entries, 128 KiB) for M = 16384 (chirps
-// of up to 8192 samples)
+// address 160 KiB: radix-4 Stockham between two buffers (2 M entries) up to
+// M = 8192, radix-2 in place in one buffer (M entries, 128 KiB) for M = 16384
 constexpr int kProfileStockhamMaxM = 8192;
 constexpr int kProfileMaxM = 16384;
 // FFT length from which the per-chirp transform runs 1024 threads
