@@ -421,11 +421,15 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    # the GPU's clocks settle after ~100 steps of sustained work (measured:
-    # 0.63-0.66 ms per step after 2 warm-up steps, 0.56 after 30, 0.55 after 100)
+    # the GPU's clocks settle under sustained work: from an idle GPU the
+    # range-profile kernel on 1024 x 512 x 512^2 took 0.50-0.55 ms for its first
+    # steps, 0.43 ms after 50 and 0.418-0.422 ms from 150 to 2000 (measured)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="arc", choices=["line_jitter", "arc", "helix"])
-    ap.add_argument("--variants", default="c64,f16,bf16,matrix,vector,plain,doppler,plain_moving",
+    # variants run in this order: the plain form's legs (~30-60 ms per step)
+    # come first and leave the GPU at sustained clocks for the precision
+    # forms, whose steps are ~0.5 ms
+    ap.add_argument("--variants", default="plain,doppler,plain_moving,vector,matrix,f16,bf16,c64",
                     help="comma list of c64, f16, bf16 (precision form, automatic engine), "
                          "matrix and vector (precision form, c64, that engine), plain, doppler "
                          "(the plain form with the Doppler window on) and plain_moving (the "
