@@ -144,41 +144,59 @@ def make_proc(tdc, s, kind, storage="c64", engine="auto", moving=False):
 DOPPLER = dict(center=0.0, bandwidth=4000.0, centroid=True, taper=True)
 
 
-def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=None,
-                 moving=False):
+def setup_variant(tdc, s, kind, storage, engine="auto", doppler=None, moving=False):
+    """A processor for one variant with its inputs in HBM: one end-to-end
+    start() (host arrays -> H2D -> compute -> D2H; the first call also allocates
+    the processor's device buffers), then prepare()."""
     p, keep = make_proc(tdc, s, kind, storage, engine, moving)
     win = ((doppler["center"], doppler["bandwidth"], doppler["centroid"], False,
             doppler["taper"]) if doppler else (0.0, 1e30, False, False, False))
-    # end-to-end start(): host arrays -> H2D -> compute -> D2H (PCIe inclusive); the
-    # first call also allocates the processor's device buffers
     p.start(*win)
     e2e_first_ms = p.stats()["end_to_end_ms"]
     p.prepare(*win)
+    return dict(p=p, keep=keep, win=win, kind=kind, storage=storage, e2e_first_ms=e2e_first_ms)
+
+
+def time_prepared(v, steps, warmup):
+    """`warmup` untimed steps, then exactly `steps` timed ones."""
+    p = v["p"]
     if warmup > 0:
         p.run(warmup)
     t0 = time.perf_counter()
     p.run(steps)
     t1 = time.perf_counter()
-    st = p.stats()
+    v["ms_per_step"] = (t1 - t0) * 1e3 / steps
+    v["stats"] = p.stats()
+
+
+def finish_variant(v):
+    """The variant's image and its steady end-to-end time: the median of three
+    start() calls on the processor, reusing its device buffers."""
+    p, st = v["p"], v["stats"]
     p.fetchImage()
     img = p.getFocusedImage().copy()
-    ms_step = (t1 - t0) * 1e3 / steps
-    # the steady end-to-end value: median of three start() calls right after the
-    # timed steps, at the clocks of sustained work, reusing the device buffers
     again = []
     for _ in range(3):
-        p.start(*win)
+        p.start(*v["win"])
         again.append(p.stats()["end_to_end_ms"])
     e2e_ms = sorted(again)[1]
-    return dict(kind=kind, storage=storage, ms_per_step=ms_step, main_kernel_ms=st["main_kernel_ms"],
+    return dict(kind=v["kind"], storage=v["storage"], ms_per_step=v["ms_per_step"],
+                main_kernel_ms=st["main_kernel_ms"],
                 taps=st["taps"], oversampling=st["oversampling"],
-                kernel_ms=st["kernel_ms"], e2e_ms=e2e_ms, e2e_first_ms=e2e_first_ms,
+                kernel_ms=st["kernel_ms"], e2e_ms=e2e_ms, e2e_first_ms=v["e2e_first_ms"],
                 terms=st["terms"],
                 splits=st["chirp_splits"], ppl=st["pixels_per_lane"], engine=st["engine"],
                 doppler_active=st["doppler_active"], skipped_chunks=st["skipped_chunks"],
                 processed_chunks=st["processed_chunks"], window_per_chirp=st["window_per_chirp"],
                 range_per_chirp=st["range_per_chirp"], range_series=st["range_series"],
-                workgroups=st["workgroups"], image=img, keep=keep)
+                workgroups=st["workgroups"], image=img, keep=v["keep"])
+
+
+def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=None,
+                 moving=False):
+    v = setup_variant(tdc, s, kind, storage, engine, doppler, moving)
+    time_prepared(v, steps, warmup)
+    return finish_variant(v)
 
 
 def served_kernel(ns, npx):
@@ -492,12 +510,22 @@ def main():
         p.start(0.0, 1e30, False, False, False)
 
     results = {}
-    for v in [x.strip() for x in args.variants.split(",") if x.strip()]:
+    # every variant is set up first (inputs in HBM, one end-to-end start()),
+    # then the variants' warm-up and timed steps run back to back, so that no
+    # host-side set-up idles the GPU between them and every variant is timed at
+    # the clocks of sustained work; the images and end-to-end times follow
+    names = [x.strip() for x in args.variants.split(",") if x.strip()]
+    prepared = {}
+    for v in names:
         kind = "plain" if v in ("plain", "doppler", "plain_moving") else "precision"
         storage = "c64" if kind == "plain" or v in ("vector", "matrix") else v
         engine = v if v in ("vector", "matrix") else "auto"
-        r = time_variant(tdc, s, kind, storage, args.steps, args.warmup, engine,
-                         DOPPLER if v == "doppler" else None, v == "plain_moving")
+        prepared[v] = setup_variant(tdc, s, kind, storage, engine,
+                                    DOPPLER if v == "doppler" else None, v == "plain_moving")
+    for v in names:
+        time_prepared(prepared[v], args.steps, args.warmup)
+    for v in names:
+        r = finish_variant(prepared.pop(v))
         r["gterms"] = r["terms"] / (r["ms_per_step"] * 1e-3) / 1e9
         r["gterms_kernel"] = r["terms"] / (r["main_kernel_ms"] * 1e-3) / 1e9
         r["gterms_e2e"] = r["terms"] / (r["e2e_ms"] * 1e-3) / 1e9
