@@ -58,22 +58,73 @@ EXEC_FLOP_PER_TERM = {"plain": 27.0, "range": 12.0, "vector": 8.0}
 # FP16 MFMA flops the matrix engine executes per term: 8 per product (one
 # complex MAC as a real 2x2 block) times hi*hi + hi*lo (+ lo*hi for c64 raw)
 MFMA_FLOP_PER_TERM = {"c64": 24.0, "f16": 16.0, "bf16": 16.0}
-# Arithmetic the range-profile kernel executes per (pixel, chirp): FP64 --
-# |pos|^2 - 2 pos.p + |p|^2 (7), the clamped Newton range step (5), the bin
-# position (2), the anchor phase (1), the block sums (2 per 8 chirps); FP32 --
-# the kernel exponent arguments and exp2 (6), 2 taps - 1 weight products,
-# `taps` complex-by-real MACs (4 taps), the anchor sin/cos and the rotation
-# into the block sum (10). Counted as FP32-equivalent flops (an FP64 flop
-# issues at half the FP32 rate) against the FP32 peak.
-PROFILE_FP64_PER_PAIR = 15.25
+# Arithmetic the range-profile kernel executes per (pixel, chirp). Every
+# pair, FP32: the kernel exponent arguments and exp2 (6), 2 taps - 1 weight
+# products, `taps` complex-by-real MACs (4 taps), the anchor sin/cos and the
+# rotation into the block sum (10); FP64: the block sums (2 per 8 chirps).
+# A chirp whose range is evaluated in FP64 (every chirp, or the first of each
+# block of 8 with the incremental range) adds FP64 |pos|^2 - 2 pos.p + |p|^2
+# (7), the clamped Newton range step (5), the bin position (2) and the anchor
+# phase (1), and with the incremental range FP32 0.5 / R (2); the block's
+# other chirps add FP32 delta (6), e0 and t (2), the series (4), dR (1), R and
+# 0.5 / R (3), the bin position (3) and the anchor phase (4). Counted as
+# FP32-equivalent flops (an FP64 flop issues at half the FP32 rate) against
+# the FP32 peak.
+PROFILE_BLOCK = 8
+PROFILE_FP64_BLOCK_SUMS = 0.25
+PROFILE_FP64_EXACT_RANGE = 15.0
+PROFILE_FP32_INCR_FIRST = 2.0
+PROFILE_FP32_INCR_STEP = 23.0
 
 
-def profile_fp32_per_pair(taps):
-    return 6.0 + (2.0 * taps - 1.0) + 4.0 * taps + 10.0
+def profile_fp32_per_pair(taps, incr=False):
+    base = 6.0 + (2.0 * taps - 1.0) + 4.0 * taps + 10.0
+    if not incr:
+        return base
+    return base + (PROFILE_FP32_INCR_FIRST + (PROFILE_BLOCK - 1) * PROFILE_FP32_INCR_STEP) / PROFILE_BLOCK
 
 
-def profile_fp32eq_per_pair(taps):
-    return profile_fp32_per_pair(taps) + 2.0 * PROFILE_FP64_PER_PAIR
+def profile_fp64_per_pair(incr=False):
+    exact = PROFILE_FP64_EXACT_RANGE / (PROFILE_BLOCK if incr else 1)
+    return exact + PROFILE_FP64_BLOCK_SUMS
+
+
+def profile_fp32eq_per_pair(taps, incr=False):
+    return profile_fp32_per_pair(taps, incr) + 2.0 * profile_fp64_per_pair(incr)
+
+
+def profile_incremental(sc):
+    """Whether the library advances a served job's ranges by FP32 increments
+    (the host bound of the range-profile engine, restated: positions and
+    pixels from the midpoint of the first and last pixels, the smallest
+    distance between their bounding boxes, the largest chirp step, and the
+    larger of the anchor wavenumber and |dk| M)."""
+    f = sc.f64()
+    grid = f["focusGrid"].reshape(-1, 3)
+    o = 0.5 * (grid[0] + grid[-1])
+    c = f["position"] - o
+    g = grid - o
+    if len(c) < 2:
+        return False
+    gap = np.maximum(0.0, np.maximum(g.min(0) - c.max(0), c.min(0) - g.max(0)))
+    rmin = float(np.linalg.norm(gap))
+    if not rmin > 0.0:
+        return False
+    dc = float(np.linalg.norm(np.diff(c, axis=0), axis=1).max())
+    cn = float(np.linalg.norm(c, axis=1).max())
+    pn = float(np.linalg.norm(np.maximum(np.abs(g.min(0)), np.abs(g.max(0)))))
+    ns = sc.n_samples
+    t = sc.sample_times
+    dt = (t[-1] - t[0]) / (ns - 1) if ns > 1 else 0.0
+    kh = 2.0 * (sc.f0 + sc.mod_rate * (t[0] + (ns // 2) * dt)) / 299792458.0
+    taps, over = served_kernel(ns, sc.n_pixels)
+    m = profile_length(ns) * over // 2
+    dkm = abs(2.0 * sc.mod_rate * dt / 299792458.0) * m
+    eps = 2.0 ** -24
+    e0 = dc * (1.0 + dc / rmin)
+    err = (eps * (5.0 * cn + 6.0 * pn) * dc / (2.0 * rmin) + 4.0 * eps * e0
+           + 0.625 * e0 ** 4 / rmin ** 3)
+    return max(abs(kh), dkm) * err <= 2e-6
 
 
 def profile_length(ns):
@@ -182,7 +233,7 @@ def finish_variant(v):
     e2e_ms = sorted(again)[1]
     return dict(kind=v["kind"], storage=v["storage"], ms_per_step=v["ms_per_step"],
                 main_kernel_ms=st["main_kernel_ms"],
-                taps=st["taps"], oversampling=st["oversampling"],
+                taps=st["taps"], oversampling=st["oversampling"], incremental=st["incremental"],
                 kernel_ms=st["kernel_ms"], e2e_ms=e2e_ms, e2e_first_ms=v["e2e_first_ms"],
                 terms=st["terms"],
                 splits=st["chirp_splits"], ppl=st["pixels_per_lane"], engine=st["engine"],
@@ -259,7 +310,7 @@ def serve_stream(tdc, srv, scenes, rate, form, storage="c64", seed=0):
         if form == "precision":
             taps, over = served_kernel(ns, npx)
             m = profile_length(ns) * over // 2
-            pairs = float(npx) * nc * profile_fp32eq_per_pair(taps)
+            pairs = float(npx) * nc * profile_fp32eq_per_pair(taps, profile_incremental(sc))
             executed += pairs + nc * 5.0 * m * np.log2(m)
             in_steps += pairs
         else:
@@ -659,8 +710,9 @@ def main():
         # the range-profile kernel's own arithmetic per (pixel, chirp), on the
         # vector ALUs (no MFMA), over its measured launch time
         taps = head["taps"]
+        incr = bool(head["incremental"])
         pairs_per_s = terms_per_s / s.n_samples
-        fp32eq = profile_fp32eq_per_pair(taps)
+        fp32eq = profile_fp32eq_per_pair(taps, incr)
         achieved = pairs_per_s * fp32eq / 1e12
         if simd is not None and sq.get("SQ_INSTS_VALU"):
             wave_chirps = s.n_pixels / 64.0 * s.n_chirps
@@ -677,8 +729,9 @@ def main():
                     unit="TFLOP/s", frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=traffic,
                     kernel=(f"profile_kernel ({head['storage']}, {taps} taps, "
                             f"{head['oversampling']}x oversampling)"),
-                    executed=dict(fp32_flop_per_pixel_chirp=profile_fp32_per_pair(taps),
-                                  fp64_flop_per_pixel_chirp=PROFILE_FP64_PER_PAIR,
+                    executed=dict(fp32_flop_per_pixel_chirp=profile_fp32_per_pair(taps, incr),
+                                  fp64_flop_per_pixel_chirp=profile_fp64_per_pair(incr),
+                                  incremental_range=incr,
                                   fp32_equivalent_per_pixel_chirp=fp32eq,
                                   pixel_chirp_pairs_per_launch=float(s.n_pixels) * s.n_chirps),
                     direct_sum_equiv=direct_equiv, simd_cycles=simd)
@@ -733,6 +786,9 @@ def main():
                     e2e_ms=round(r["e2e_ms"], 3), e2e_first_ms=round(r["e2e_first_ms"], 3),
                     gterms_e2e_pcie=round(r["gterms_e2e"], 3),
                     chirp_splits=r["splits"], engine=r["engine"],
+                    **({"taps": r["taps"], "oversampling": r["oversampling"],
+                        "incremental_range": bool(r["incremental"])}
+                       if r["engine"] == "profile" else {}),
                     **({"range_per_chirp": bool(r["range_per_chirp"]),
                         "range_series": bool(r["range_series"])} if r["kind"] == "plain" else {}),
                     **({"skipped_chunks": r["skipped_chunks"],

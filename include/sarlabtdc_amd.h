@@ -107,6 +107,10 @@ typedef struct tdc_stats {
                                  within every chirp (stop-and-hop) */
     int32_t profile_taps;         /* range-profile engine: taps of its kernel, else 0 */
     int32_t profile_oversampling; /* range-profile engine: M / samples (2 or 4), else 0 */
+    int32_t profile_incremental;  /* range-profile engine: 1 when the range advanced by FP32
+                                     increments within blocks of 8 chirps (see
+                                     tdc_set_profile_exact_range), else 0 */
+    int32_t reserved;
 } tdc_stats;
 
 /* Message for the last failing call on this thread ("" when none). */
@@ -210,6 +214,16 @@ int tdc_get_stats(const tdc_processor *h, tdc_stats *out);
  * a chirp's profile (rel-L2); each two taps fewer save ~8 of ~90 instructions
  * per (pixel, chirp), fourfold oversampling doubles the per-chirp table. */
 int tdc_set_profile_kernel(tdc_processor *h, int taps, int oversampling);
+
+/* Range of the range-profile engine. exact = 0 (the default): within each
+ * block of 8 chirps the first chirp's range, bin position and anchor phase
+ * are evaluated in FP64 and the next chirps' advanced by FP32 increments from
+ * the chirps' position steps, when a bound on that error (from the largest
+ * step, the positions' and pixels' extents and their smallest distance) stays
+ * below 2e-6 cycles per step; otherwise, and with exact = 1, every (pixel,
+ * chirp) is evaluated in FP64. tdc_stats.profile_incremental reports which
+ * ran. */
+int tdc_set_profile_exact_range(tdc_processor *h, int exact);
 
 /* Force a launch configuration for tuning and tests: pixels per lane
  * (0 = automatic; 1, 2 or 4 for the plain form, up to 8 for the precision

@@ -132,6 +132,7 @@ public:
     void setRawStorage(int storage);    /* TDC_RAW_* */
     void setEngine(int engine);         /* TDC_ENGINE_* (precision form) */
     void setProfileKernel(int taps, int oversampling); /* range-profile kernel (0, 0: auto) */
+    void setProfileExactRange(bool exact);              /* FP64 range for every chirp */
     void setRangeWindow(int kind, double p0 = 0.0, double p1 = 0.0); /* TDC_WIN_* */
     void setLaunch(int pixelsPerLane, int chirpSplits);
     void setDeviceBudget(unsigned long long bytes); /* 0: no limit */

@@ -193,6 +193,7 @@ public:
     void setRawStorage(const py::object &s) { proc_->setRawStorage(parse_storage(s)); }
     void setEngine(const py::object &e) { proc_->setEngine(parse_engine(e)); }
     void setProfileKernel(int taps, int over) { proc_->setProfileKernel(taps, over); }
+    void setProfileExactRange(bool exact) { proc_->setProfileExactRange(exact); }
     void setRangeWindow(const py::object &kind, double p0, double p1)
     {
         int k = parse_window(kind);
@@ -247,6 +248,7 @@ public:
                                                         : "none";
         d["taps"] = st.profile_taps;
         d["oversampling"] = st.profile_oversampling;
+        d["incremental"] = st.profile_incremental != 0;
         return d;
     }
 
@@ -409,6 +411,7 @@ PYBIND11_MODULE(_sarlabtdc, m)
         .def("setEngine", &PyTdcProcessor::setEngine, py::arg("engine"))
         .def("setProfileKernel", &PyTdcProcessor::setProfileKernel, py::arg("taps") = 0,
              py::arg("oversampling") = 0)
+        .def("setProfileExactRange", &PyTdcProcessor::setProfileExactRange, py::arg("exact") = true)
         .def("setRangeWindow", &PyTdcProcessor::setRangeWindow, py::arg("kind"),
              py::arg("p0") = 0.0, py::arg("p1") = 0.0)
         .def("setLaunch", &PyTdcProcessor::setLaunch, py::arg("pixelsPerLane"),

@@ -216,6 +216,7 @@ struct tdc_processor {
     double win_p0 = 25.0 / 46.0, win_p1 = 0.0; // tuning.h:31
     int force_ppl = 0, force_split = 0;
     int profile_taps = 0, profile_over = 0; // range-profile kernel (0, 0: automatic)
+    bool profile_exact = false;             // range-profile engine: FP64 range for every chirp
     uint64_t budget = 0; // device bytes for one image's buffers (0: no limit)
 
     // device buffers
@@ -485,8 +486,8 @@ int stage_inputs(tdc_processor *h)
         // window and storage rounding folded into the per-chirp transform
         TDC_HIP(tdc::launch_stage_profile_folded(
             h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
-            h->d_ptw.as<float2>(), npri, ns, h->pp.M, h->pp.Mp, h->pp.taps, h->pp.reverse,
-            h->d_ptab.as<float2>(), s));
+            h->d_ptw.as<float2>(), h->d_pcpos.as<tdc::ChirpPos>(), npri, ns, h->pp.M, h->pp.Mp,
+            h->pp.taps, h->pp.reverse, h->d_ptab.as<float2>(), s));
         return TDC_OK;
     }
     TDC_HIP(tdc::launch_stage_precision(h->run_storage, h->d_raw.as<float>(), h->d_wr.as<float>(),
@@ -728,6 +729,17 @@ int precision_inputs(tdc_processor *h, float dop_center, float dop_bw, bool cent
         h->pp.Mp = in.mp;
         h->pp.dkm = std::fabs(h->dk) * in.plen;
         h->pp.reverse = h->dk < 0.0 ? 1 : 0;
+        h->pp.incr = 0;
+        if (!h->profile_exact && npx > 0) {
+            double glo[3], ghi[3];
+            tdc::grid_bounds(gridp, static_cast<size_t>(npx), glo, ghi);
+            for (int a = 0; a < 3; ++a) {
+                glo[a] -= o[a];
+                ghi[a] -= o[a];
+            }
+            const double kmax = std::max(std::fabs(h->pp.kh), h->pp.dkm);
+            h->pp.incr = tdc::profile_incremental_ok(in.pc.data(), npri, glo, ghi, kmax) ? 1 : 0;
+        }
         h->ppl = 1;
     } else if (engine == TDC_ENGINE_MATRIX) {
         if (h->force_ppl > 1)
@@ -1011,6 +1023,7 @@ void noalloc_finish(tdc_processor *h, const NoallocInputs &in, int nsplit, int w
     st.engine = TDC_ENGINE_PROFILE;
     st.profile_taps = in.pp.taps;
     st.profile_oversampling = in.pp.over;
+    st.profile_incremental = in.pp.incr;
     h->stats = st;
     clear_last_error();
 }
@@ -1243,6 +1256,14 @@ int tdc_set_profile_kernel(tdc_processor *h, int taps, int oversampling)
     return ok();
 }
 
+int tdc_set_profile_exact_range(tdc_processor *h, int exact)
+{
+    if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
+    h->profile_exact = exact != 0;
+    h->prepared = false;
+    return ok();
+}
+
 int tdc_set_device_budget(tdc_processor *h, unsigned long long bytes)
 {
     if (!h) return fail(TDC_ERR_INVALID_ARGUMENT, "handle is NULL");
@@ -1378,6 +1399,7 @@ int finish_stats(tdc_processor *h, double main_ms, double total_ms, int wgs)
     st.engine = h->run_form == TDC_FORM_PRECISION ? h->run_engine : 0;
     st.profile_taps = st.engine == TDC_ENGINE_PROFILE ? h->pp.taps : 0;
     st.profile_oversampling = st.engine == TDC_ENGINE_PROFILE ? h->pp.over : 0;
+    st.profile_incremental = st.engine == TDC_ENGINE_PROFILE ? h->pp.incr : 0;
     st.per_chirp = 0;
     if (h->run_form == TDC_FORM_PLAIN && h->chirp_path)
         st.per_chirp = TDC_PER_CHIRP_RANGE | (h->doppler ? TDC_PER_CHIRP_WINDOW : 0);
@@ -1573,8 +1595,8 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
         if (in.engine == TDC_ENGINE_PROFILE) {
             TDC_HIP(tdc::launch_stage_profile_folded(
                 h->storage, h->r_raw[set].as<float>(), h->d_wr.as<float>(), h->d_pcorr.as<float>(),
-                h->d_ptw.as<float2>(), nk, ns, in.plen, in.mp, h->pp.taps, h->pp.reverse,
-                h->r_tab[set].as<float2>(), s));
+                h->d_ptw.as<float2>(), h->r_cp[set].as<tdc::ChirpPos>(), nk, ns, in.plen, in.mp,
+                h->pp.taps, h->pp.reverse, h->r_tab[set].as<float2>(), s));
             TDC_HIP(tdc::launch_profile(h->r_tab[set].as<float2>(),
                                         h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),
                                         h->pp, gs, h->d_part.as<double2>(), s, &w));

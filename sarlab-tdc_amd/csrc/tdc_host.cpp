@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -152,6 +153,37 @@ void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_
         const size_t b = static_cast<size_t>(i) * step;
         if (b < n) fn(b, std::min(n, b + step));
     });
+}
+
+void grid_bounds(const double *grid, size_t npx, double lo[3], double hi[3])
+{
+    const double inf = std::numeric_limits<double>::infinity();
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = inf;
+        hi[a] = -inf;
+    }
+    std::mutex mu;
+    bool nan = false;
+    parallel_for(npx, size_t(1) << 15, [&](size_t i0, size_t i1) {
+        double l[3] = {inf, inf, inf}, h[3] = {-inf, -inf, -inf};
+        bool bad = false;
+        for (size_t i = i0; i < i1; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                const double v = grid[3 * i + a];
+                bad = bad || v != v;
+                l[a] = std::min(l[a], v);
+                h[a] = std::max(h[a], v);
+            }
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        nan = nan || bad;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], l[a]);
+            hi[a] = std::max(hi[a], h[a]);
+        }
+    });
+    if (nan)
+        for (int a = 0; a < 3; ++a) lo[a] = hi[a] = std::numeric_limits<double>::quiet_NaN();
 }
 
 namespace {

@@ -1,6 +1,8 @@
 // tdc_internal.h -- shared between the HIP kernels and the host C ABI.
 #pragma once
 
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <functional>
@@ -139,8 +141,68 @@ struct ProfileParams {
     int taps;   // kernel width
     int over;   // oversampling, M >= over * ns
     int reverse; // 1 for dk < 0 (down-chirps): table rows hold B[-m] instead of B[m]
-    int pad;
+    int incr;    // 1: within a block of chirps, the range, bin position and anchor advance
+                 // by FP32 increments from the block's first chirp (profile_incremental_ok)
 };
+// The last two entries of a table row (never read as taps, which reach at
+// most M + kProfileTaps - 2) hold the chirp's position increment: float4
+// (c_n - c_{n-1}, |c_n|^2 - |c_{n-1}|^2) of the origin-relative positions
+constexpr int kProfileDeltaEntry = kProfilePad - 2;
+
+// Whether the range-profile tile may advance a pixel's range by FP32
+// increments within a block of chirps (the block's first chirp is exact in
+// FP64) instead of evaluating it in FP64 for every chirp. A step changes the
+// range by dR = delta / (R_n + R_{n-1}), delta = R_n^2 - R_{n-1}^2 =
+// (|c_n|^2 - |c_{n-1}|^2) - 2 p . (c_n - c_{n-1}), formed in FP32 from
+// rounded inputs by three FMAs with absolute error below eps32 (4 |c|max +
+// 5 |p|max) |dc|max (taken as eps32 (5 |c|max + 6 |p|max) |dc|max), and taken from the
+// series dR = e0 (1 - t (1 - 2 t)), e0 = delta / (2 R_{n-1}), t = e0 / (2
+// R_{n-1}), whose remainder is below 0.625 e0^4 / R^3 with |e0| <= |dc|max
+// (1 + |dc|max / R); the FP32 products that form dR (the reciprocal of R,
+// e0 and the series) add at most 4 eps32 |e0|. Their sum, times the largest
+// wavenumber kmax, must stay below kIncrementalCycles per step; R is bounded below by the distance between the
+// bounding boxes of the chirp positions and the pixels. `c`: origin-relative
+// chirp positions; `gmin`/`gmax`: the grid's bounding box relative to the
+// same origin.
+constexpr double kIncrementalCycles = 2e-6;
+inline bool profile_incremental_ok(const ChirpPos *c, int npri, const double gmin[3],
+                                   const double gmax[3], double kmax)
+{
+    if (npri < 2) return false;
+    double cmin[3], cmax[3], step2 = 0.0, cnorm2 = 0.0;
+    for (int a = 0; a < 3; ++a) cmin[a] = cmax[a] = (&c[0].x)[a];
+    for (int n = 0; n < npri; ++n) {
+        const double v[3] = {c[n].x, c[n].y, c[n].z};
+        double s2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            if (!std::isfinite(v[a])) return false;
+            cmin[a] = std::min(cmin[a], v[a]);
+            cmax[a] = std::max(cmax[a], v[a]);
+            if (n > 0) {
+                const double d = v[a] - (&c[n - 1].x)[a];
+                s2 += d * d;
+            }
+        }
+        step2 = std::max(step2, s2);
+        cnorm2 = std::max(cnorm2, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    double gap2 = 0.0, pnorm2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(gmin[a]) || !std::isfinite(gmax[a])) return false;
+        const double gap = std::max({0.0, gmin[a] - cmax[a], cmin[a] - gmax[a]});
+        gap2 += gap * gap;
+        const double pa = std::max(std::fabs(gmin[a]), std::fabs(gmax[a]));
+        pnorm2 += pa * pa;
+    }
+    const double rmin = std::sqrt(gap2), dc = std::sqrt(step2);
+    if (!(rmin > 0.0) || !(kmax > 0.0)) return false;
+    const double eps32 = 1.0 / 16777216.0;
+    const double e0 = dc * (1.0 + dc / rmin);
+    const double err_delta = eps32 * (5.0 * std::sqrt(cnorm2) + 6.0 * std::sqrt(pnorm2)) * dc;
+    const double err_dr = err_delta / (2.0 * rmin) + 4.0 * eps32 * e0 +
+                          0.625 * e0 * e0 * e0 * e0 / (rmin * rmin * rmin);
+    return kmax * err_dr <= kIncrementalCycles;
+}
 inline int profile_length(int ns, int over = 2)
 {
     int m = 16;
@@ -260,17 +322,19 @@ hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, in
 // Range-profile tables [npri][Mp] from the staged raw samples (in `storage`
 // format, range window folded in); corr[i] = 1 / psi_hat((i - h) / M);
 // twiddle[k] = exp(+j 2 pi k / M), k < M / 2; rows rotated by taps / 2 - 1
-// bins for a kernel of `taps` taps, and reversed (B[-m]) when `reverse`.
+// bins for a kernel of `taps` taps, and reversed (B[-m]) when `reverse`;
+// Mp = M + kProfilePad, and each row ends with the chirp's position increment
+// from `cpos` (the records launch_profile reads; kProfileDeltaEntry).
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
-                                const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
-                                int reverse, float2 *table, hipStream_t s);
+                                const float2 *twiddle, const ChirpPos *cpos, int npri, int ns,
+                                int M, int Mp, int taps, int reverse, float2 *table, hipStream_t s);
 // The same tables from the complex64 input: each sample times the range
 // window wr[i], rounded as `storage` stores it (bit-identical to staging it
 // with launch_stage_precision first).
 hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
-                                       const float *corr, const float2 *twiddle, int npri,
-                                       int ns, int M, int Mp, int taps, int reverse,
-                                       float2 *table, hipStream_t s);
+                                       const float *corr, const float2 *twiddle,
+                                       const ChirpPos *cpos, int npri, int ns, int M, int Mp,
+                                       int taps, int reverse, float2 *table, hipStream_t s);
 // `cpos`: chirp positions minus the origin (pp.ox, oy, oz), with |pos - o|^2 in pad.
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
                           const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,
@@ -345,6 +409,9 @@ constexpr unsigned kHostThreads = 8;
 // `grain` items, on the host pool and the calling thread; returns when all
 // parts have run.
 void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_t)> &fn);
+// Bounding box of npx FP64 points [npx][3] (on the host thread pool); NaN
+// bounds when a coordinate is NaN.
+void grid_bounds(const double *grid, size_t npx, double lo[3], double hi[3]);
 
 // One pass over per-sample antenna states [npri][ns][3|3|4] (vel and att may
 // be null): finiteness, the largest |v|^2 and |q|^2, the bounding box of the

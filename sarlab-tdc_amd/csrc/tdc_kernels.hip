@@ -569,6 +569,31 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 w)
     return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x));
 }
 
+// Row n of the table from the transform x in LDS: row[j] = x[(j - lead) mod M]
+// (x[(lead - j) mod M] for down-chirps) for j < kProfileDeltaEntry past M,
+// then the chirp's position increment against chirp n - 1 (zero for the
+// first chirp) as one float4 in the last two entries, rounded from FP64.
+template <int BLOCK>
+__device__ __forceinline__ void write_profile_row(const float2 *x, const ChirpPos *__restrict__ cpos,
+                                                  int n, int M, int Mp, int lead, int reverse,
+                                                  float2 *__restrict__ table)
+{
+    const int mask = M - 1;
+    const int tid = static_cast<int>(threadIdx.x);
+    float2 *dst = table + static_cast<size_t>(n) * Mp;
+    const int rows = M + kProfileDeltaEntry;
+    for (int j = tid; j < rows; j += BLOCK) dst[j] = x[(reverse ? lead - j : j - lead) & mask];
+    if (tid == 0) {
+        float4 d = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+        if (n > 0) {
+            const ChirpPos c = cpos[n], p = cpos[n - 1];
+            d = make_float4(static_cast<float>(c.x - p.x), static_cast<float>(c.y - p.y),
+                            static_cast<float>(c.z - p.z), static_cast<float>(c.pad - p.pad));
+        }
+        *reinterpret_cast<float4 *>(dst + rows) = d;
+    }
+}
+
 // One workgroup per chirp: b[(i - h) mod M] = c_i * corr[i] (zero elsewhere),
 // then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-4 Stockham passes (and
 // one radix-2 pass when log2 M is odd) between two LDS buffers, with the
@@ -578,12 +603,13 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 w)
 // kProfilePad entries: a pixel whose bin position has integer part f reads
 // its taps B[f - lead .. f - lead + taps - 1] as row[f .. f + taps - 1],
 // which never wrap. For down-chirps (`reverse`) the row holds
-// B[(lead - j) mod M]: the profile at frequency -u is read at u.
+// B[(lead - j) mod M]: the profile at frequency -u is read at u. The row's
+// last two entries hold the chirp's position increment (write_profile_row).
 template <class RAW, int BLOCK, bool FOLD, bool LDS_TW>
 __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
-    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead, int reverse,
-    float2 *__restrict__ table)
+    const float2 *__restrict__ twiddle, const ChirpPos *__restrict__ cpos, int ns, int M, int Mp,
+    int lead, int reverse, float2 *__restrict__ table)
 {
     // 2 M entries (+ M / 2 twiddles with LDS_TW), sized at launch
     extern __shared__ float2 buf[];
@@ -653,9 +679,7 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
         x = y;
         y = tmp;
     }
-    const int mask = M - 1;
-    float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(reverse ? lead - j : j - lead) & mask];
+    write_profile_row<BLOCK>(x, cpos, n, M, Mp, lead, reverse, table);
 }
 
 // The same transform for chirps whose two Stockham buffers would not fit the
@@ -665,8 +689,8 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
 template <class RAW, int BLOCK, bool FOLD>
 __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
     const void *__restrict__ src, const float *__restrict__ wr, const float *__restrict__ corr,
-    const float2 *__restrict__ twiddle, int ns, int M, int Mp, int lead, int reverse,
-    float2 *__restrict__ table)
+    const float2 *__restrict__ twiddle, const ChirpPos *__restrict__ cpos, int ns, int M, int Mp,
+    int lead, int reverse, float2 *__restrict__ table)
 {
     extern __shared__ float2 x[]; // M entries, sized at launch
     const int n = static_cast<int>(blockIdx.x);
@@ -701,8 +725,7 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_inplace_kernel(
         }
         __syncthreads();
     }
-    float2 *dst = table + static_cast<size_t>(n) * Mp;
-    for (int j = tid; j < Mp; j += BLOCK) dst[j] = x[(reverse ? lead - j : j - lead) & mask];
+    write_profile_row<BLOCK>(x, cpos, n, M, Mp, lead, reverse, table);
 }
 
 template <int TAPS, int OVER>
@@ -1090,8 +1113,8 @@ constexpr int kProfileLdsTwiddleM = 4096;
 
 template <class RAW, int BLOCK, bool FOLD>
 hipError_t stage_profile_launch(const void *src, const float *wr, const float *corr,
-                                const float2 *twiddle, int npri, int ns, int M, int Mp,
-                                int lead, int reverse, float2 *table, hipStream_t s)
+                                const float2 *twiddle, const ChirpPos *cpos, int npri, int ns,
+                                int M, int Mp, int lead, int reverse, float2 *table, hipStream_t s)
 {
     const bool inplace = M > kProfileStockhamMaxM;
     const bool ldstw = M <= kProfileLdsTwiddleM;
@@ -1108,14 +1131,14 @@ hipError_t stage_profile_launch(const void *src, const float *wr, const float *c
     }
     if (inplace)
         hipLaunchKernelGGL((stage_profile_inplace_kernel<RAW, BLOCK, FOLD>), dim3(npri),
-                           dim3(BLOCK), lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse,
-                           table);
+                           dim3(BLOCK), lds, s, src, wr, corr, twiddle, cpos, ns, M, Mp, lead,
+                           reverse, table);
     else if (ldstw)
         hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD, true>), dim3(npri), dim3(BLOCK),
-                           lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse, table);
+                           lds, s, src, wr, corr, twiddle, cpos, ns, M, Mp, lead, reverse, table);
     else
         hipLaunchKernelGGL((stage_profile_kernel<RAW, BLOCK, FOLD, false>), dim3(npri), dim3(BLOCK),
-                           lds, s, src, wr, corr, twiddle, ns, M, Mp, lead, reverse, table);
+                           lds, s, src, wr, corr, twiddle, cpos, ns, M, Mp, lead, reverse, table);
     return hipGetLastError();
 }
 
@@ -1124,35 +1147,37 @@ hipError_t stage_profile_launch(const void *src, const float *wr, const float *c
 // LDS busy across the passes' barriers; short ones run 256.
 template <class RAW, bool FOLD>
 hipError_t stage_profile_raw(const void *src, const float *wr, const float *corr,
-                             const float2 *twiddle, int npri, int ns, int M, int Mp, int lead,
-                             int reverse, float2 *table, hipStream_t s)
+                             const float2 *twiddle, const ChirpPos *cpos, int npri, int ns, int M,
+                             int Mp, int lead, int reverse, float2 *table, hipStream_t s)
 {
     if (M >= kProfileWideM)
-        return stage_profile_launch<RAW, 1024, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp,
-                                                     lead, reverse, table, s);
-    return stage_profile_launch<RAW, kBlock, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
-                                                   reverse, table, s);
+        return stage_profile_launch<RAW, 1024, FOLD>(src, wr, corr, twiddle, cpos, npri, ns, M,
+                                                     Mp, lead, reverse, table, s);
+    return stage_profile_launch<RAW, kBlock, FOLD>(src, wr, corr, twiddle, cpos, npri, ns, M, Mp,
+                                                   lead, reverse, table, s);
 }
 
 template <bool FOLD>
 hipError_t stage_profile_any(int storage, const void *src, const float *wr, const float *corr,
-                             const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
-                             int reverse, float2 *table, hipStream_t s)
+                             const float2 *twiddle, const ChirpPos *cpos, int npri, int ns, int M,
+                             int Mp, int taps, int reverse, float2 *table, hipStream_t s)
 {
+    // taps reach row[M + taps - 2]; the position increment sits after
+    // kProfileDeltaEntry entries past M
     if (M > kProfileMaxM || M < 2 || (M & (M - 1)) != 0 || taps < 2 || taps > kProfileTaps ||
-        Mp < M + taps - 1)
+        Mp != M + kProfilePad || M + taps - 1 > M + kProfileDeltaEntry || !cpos)
         return hipErrorInvalidValue;
     const int lead = taps / 2 - 1;
     switch (storage) {
     case 0:
-        return stage_profile_raw<RawC64, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
+        return stage_profile_raw<RawC64, FOLD>(src, wr, corr, twiddle, cpos, npri, ns, M, Mp, lead,
                                                reverse, table, s);
     case 1:
-        return stage_profile_raw<RawF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
+        return stage_profile_raw<RawF16, FOLD>(src, wr, corr, twiddle, cpos, npri, ns, M, Mp, lead,
                                                reverse, table, s);
     case 2:
-        return stage_profile_raw<RawBF16, FOLD>(src, wr, corr, twiddle, npri, ns, M, Mp, lead,
-                                                reverse, table, s);
+        return stage_profile_raw<RawBF16, FOLD>(src, wr, corr, twiddle, cpos, npri, ns, M, Mp,
+                                                lead, reverse, table, s);
     default:
         return hipErrorInvalidValue;
     }
@@ -1161,20 +1186,20 @@ hipError_t stage_profile_any(int storage, const void *src, const float *wr, cons
 } // namespace
 
 hipError_t launch_stage_profile(int storage, const void *staged, const float *corr,
-                                const float2 *twiddle, int npri, int ns, int M, int Mp, int taps,
-                                int reverse, float2 *table, hipStream_t s)
+                                const float2 *twiddle, const ChirpPos *cpos, int npri, int ns,
+                                int M, int Mp, int taps, int reverse, float2 *table, hipStream_t s)
 {
-    return stage_profile_any<false>(storage, staged, nullptr, corr, twiddle, npri, ns, M, Mp, taps,
-                                    reverse, table, s);
+    return stage_profile_any<false>(storage, staged, nullptr, corr, twiddle, cpos, npri, ns, M, Mp,
+                                    taps, reverse, table, s);
 }
 
 hipError_t launch_stage_profile_folded(int storage, const float *raw, const float *wr,
-                                       const float *corr, const float2 *twiddle, int npri,
-                                       int ns, int M, int Mp, int taps, int reverse,
-                                       float2 *table, hipStream_t s)
+                                       const float *corr, const float2 *twiddle,
+                                       const ChirpPos *cpos, int npri, int ns, int M, int Mp,
+                                       int taps, int reverse, float2 *table, hipStream_t s)
 {
-    return stage_profile_any<true>(storage, raw, wr, corr, twiddle, npri, ns, M, Mp, taps, reverse,
-                                   table, s);
+    return stage_profile_any<true>(storage, raw, wr, corr, twiddle, cpos, npri, ns, M, Mp, taps,
+                                   reverse, table, s);
 }
 
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,

@@ -171,8 +171,9 @@ extern "C" int tdc_start_noalloc(tdc_processor *h, float doppler_win_center,
     tdc::Geom g{};
     tdc::make_launch_geom(g, in.npx, in.npri, in.ns, 1, nsplit);
     int wgs = 0;
-    NA_HIP(tdc::launch_stage_profile_folded(in.storage, raw, wr, corr, tw, in.npri, in.ns, in.pp.M,
-                                            in.pp.Mp, in.pp.taps, in.pp.reverse, table, s));
+    NA_HIP(tdc::launch_stage_profile_folded(in.storage, raw, wr, corr, tw, pcpos, in.npri, in.ns,
+                                            in.pp.M, in.pp.Mp, in.pp.taps, in.pp.reverse, table,
+                                            s));
     if (npx) {
         NA_HIP(tdc::launch_profile(table, pcpos, grid, in.pp, g, part, s, &wgs));
         NA_HIP(tdc::launch_finalize_f64(part, nsplit, in.npx, img, s));

@@ -36,6 +36,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -466,6 +467,35 @@ int validate(const tdc_job_desc *d)
 
 hipError_t stage_job(const Job &j, hipStream_t ss);
 
+// Copies npx FP64 points [npx][3] and returns their bounding box (NaN bounds
+// when a coordinate is NaN), in one pass over the source.
+void copy_grid_bounds(double *dst, const double *src, size_t npx, double lo[3], double hi[3])
+{
+    const double inf = std::numeric_limits<double>::infinity();
+    double l0 = inf, l1 = inf, l2 = inf, h0 = -inf, h1 = -inf, h2 = -inf;
+    bool nan = false;
+    for (size_t i = 0; i < npx; ++i) {
+        const double x = src[3 * i], y = src[3 * i + 1], z = src[3 * i + 2];
+        dst[3 * i] = x;
+        dst[3 * i + 1] = y;
+        dst[3 * i + 2] = z;
+        nan = nan || x != x || y != y || z != z;
+        l0 = std::min(l0, x);
+        l1 = std::min(l1, y);
+        l2 = std::min(l2, z);
+        h0 = std::max(h0, x);
+        h1 = std::max(h1, y);
+        h2 = std::max(h2, z);
+    }
+    const double q = std::numeric_limits<double>::quiet_NaN();
+    lo[0] = nan ? q : l0;
+    lo[1] = nan ? q : l1;
+    lo[2] = nan ? q : l2;
+    hi[0] = nan ? q : h0;
+    hi[1] = nan ? q : h1;
+    hi[2] = nan ? q : h2;
+}
+
 // Pack, upload, stage and describe one admitted job (uploader thread).
 bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
 {
@@ -487,9 +517,15 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
         const double *c = d.chirp_position + 3 * static_cast<size_t>(n);
         hc[n] = tdc::ChirpPos{c[0], c[1], c[2], 0.0};
     }
+    // the grid's bounding box comes with its copy (range-profile jobs)
+    double glo[3], ghi[3];
     if (prec) {
         std::memcpy(h + j.off_ka, kanchor.data(), 8 * kanchor.size());
-        std::memcpy(h + j.off_grid, d.grid, 24 * static_cast<size_t>(j.npx));
+        if (j.profile)
+            copy_grid_bounds(reinterpret_cast<double *>(h + j.off_grid), d.grid,
+                             static_cast<size_t>(j.npx), glo, ghi);
+        else
+            std::memcpy(h + j.off_grid, d.grid, 24 * static_cast<size_t>(j.npx));
         if (d.raw_storage != TDC_RAW_C64) {
             // served raw data arrive windowed: the conversion uses a unit window
             auto *w = reinterpret_cast<float *>(h + j.off_ones);
@@ -526,6 +562,14 @@ bool upload_job(tdc_server *s, Job &j, hipStream_t us, std::string &err)
             j.pp.over = j.pk.over;
             j.pp.dkm = std::fabs(dk) * plen;
             j.pp.reverse = dk < 0.0 ? 1 : 0;
+            for (int a = 0; a < 3; ++a) {
+                glo[a] -= o[a];
+                ghi[a] -= o[a];
+            }
+            j.pp.incr = tdc::profile_incremental_ok(hc, d.n_pri, glo, ghi,
+                                                    std::max(std::fabs(j.pp.kh), j.pp.dkm))
+                            ? 1
+                            : 0;
         }
     } else {
         // the plain form's phase tables (the processor's, from FP64 inputs) and
@@ -619,8 +663,9 @@ hipError_t stage_job(const Job &j, hipStream_t ss)
         if (j.profile)
             return tdc::launch_stage_profile(
                 d.raw_storage, staged, reinterpret_cast<const float *>(j.arena + j.off_pcorr),
-                reinterpret_cast<const float2 *>(j.arena + j.off_ptw), d.n_pri, d.n_samples,
-                j.pp.M, j.pp.Mp, j.pp.taps, j.pp.reverse,
+                reinterpret_cast<const float2 *>(j.arena + j.off_ptw),
+                reinterpret_cast<const tdc::ChirpPos *>(j.arena + j.off_cpos), d.n_pri,
+                d.n_samples, j.pp.M, j.pp.Mp, j.pp.taps, j.pp.reverse,
                 reinterpret_cast<float2 *>(j.arena + j.off_ptab), ss);
         return tdc::launch_stage_matrix(d.raw_storage, staged, d.n_pri, d.n_samples,
                                         tdc::matrix_parts(d.raw_storage), j.arena + j.off_atab,

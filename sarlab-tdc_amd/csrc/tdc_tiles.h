@@ -726,11 +726,14 @@ __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
 }
 
 // Range-profile engine over one tile (one pixel per thread, chirps n0..n1):
-// per (pixel, chirp) the FP64 range R, the bin position u = frac(dk R) M
-// split into an integer bin f and a fraction g, the TAPS table entries around
-// u weighted by the Gaussian kernel (two exp2 per pair, two multiplies per
-// tap), and the FP64 anchor exp(j 2 pi k_h R); chirps accumulate in FP32
-// blocks of kProfileBlock chirps and the blocks in FP64.
+// per (pixel, chirp) the range R, the bin position u = frac(dk R) M split
+// into an integer bin f and a fraction g, the TAPS table entries around u
+// weighted by the Gaussian kernel (two exp2 per pair, two multiplies per
+// tap), and the anchor exp(j 2 pi k_h R); chirps accumulate in FP32 blocks
+// of kProfileBlock chirps and the blocks in FP64. R, u and the anchor phase
+// are evaluated in FP64 for the first chirp of each block and, with pp.incr,
+// advanced by FP32 increments for the block's other chirps (else in FP64 for
+// every chirp).
 namespace profile {
 // A pixel's bin position u = frac(dk R) M has integer part f and fraction g;
 // tap t (0 .. TAPS - 1) is bin f - lead + t, lead = TAPS / 2 - 1, at distance
@@ -814,26 +817,62 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     const double pp2 = fma(pz, pz, fma(py, py, px * px));
     const double qx = -2.0 * px, qy = -2.0 * py, qz = -2.0 * pz;
     const int mask = pp.M - 1;
+    // FP32 operands of the incremental range (pp.incr): -2 p', |dk| M, and k_h
+    // as the sum of two floats
+    const float qx32 = static_cast<float>(qx), qy32 = static_cast<float>(qy),
+                qz32 = static_cast<float>(qz);
+    const float dkm32 = static_cast<float>(pp.dkm);
+    const float kh_hi = static_cast<float>(pp.kh);
+    const float kh_lo = static_cast<float>(pp.kh - static_cast<double>(kh_hi));
     // chirp contributions are summed in FP32 over blocks of kProfileBlock
     // chirps and the blocks in FP64
     double accr = 0.0, acci = 0.0;
     for (int nb = n0; nb < n1; nb += profile::kProfileBlock) {
         const int ne = min(n1, nb + profile::kProfileBlock);
         float br = 0.0F, bi = 0.0F;
+        // the pixel's state within the block: FP32 range s and 0.5 / s, bin f
+        // and fraction g of its position, anchor phase a = frac(k_h R)
+        float s = 0.0F, hs = 0.0F, g = 0.0F, a = 0.0F;
+        unsigned f = 0;
         for (int n = nb; n < ne; ++n) {
-            const ChirpPos c = as_const(cpos)[n];
-            const double rng =
-                range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
-            // bin position u = x mod M, x = |dk| M R >= 0: integer part f (the
-            // truncation of x, exact while x < 2^31, i.e. R B < ~8e16 m Hz at
-            // fourfold oversampling) and fraction g; down-chirps read reversed
-            // rows, so x is never negative
-            const double x = pp.dkm * rng;
-            const float g = static_cast<float>(fract64(x));
-            // indices in unsigned arithmetic: f < M, so row[f .. f + TAPS - 1]
-            // lies in the row; window indices are masked to the row
-            const unsigned f = static_cast<unsigned>(static_cast<int>(x)) & static_cast<unsigned>(mask);
             const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
+            if (n == nb || !pp.incr) {
+                const ChirpPos c = as_const(cpos)[n];
+                const double rng =
+                    range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
+                // bin position u = x mod M, x = |dk| M R >= 0: integer part f (the
+                // truncation of x, exact while x < 2^31, i.e. R B < ~8e16 m Hz at
+                // fourfold oversampling) and fraction g; down-chirps read reversed
+                // rows, so x is never negative
+                const double x = pp.dkm * rng;
+                g = static_cast<float>(fract64(x));
+                // indices in unsigned arithmetic: f < M, so row[f .. f + TAPS - 1]
+                // lies in the row; window indices are masked to the row
+                f = static_cast<unsigned>(static_cast<int>(x)) & static_cast<unsigned>(mask);
+                a = static_cast<float>(fract64(pp.kh * rng));
+                if (pp.incr) {
+                    s = static_cast<float>(rng);
+                    hs = 0.5F * __builtin_amdgcn_rcpf(s);
+                }
+            } else {
+                // the next chirp of the block: delta = R_n^2 - R_{n-1}^2 from the
+                // chirp's position increment (the row's last entries), dR from
+                // its series, then s, the bin position and the anchor phase
+                // advanced by dR (bounds: profile_incremental_ok)
+                const float4 d = as_const(
+                    reinterpret_cast<const float4 *>(row + pp.M + kProfileDeltaEntry))[0];
+                const float delta = fmaf(qz32, d.z, fmaf(qy32, d.y, fmaf(qx32, d.x, d.w)));
+                const float e0 = delta * hs;
+                const float t = e0 * hs;
+                const float dr = e0 * fmaf(-t, fmaf(-2.0F, t, 1.0F), 1.0F);
+                s += dr;
+                hs = 0.5F * __builtin_amdgcn_rcpf(s);
+                const float gx = fmaf(dkm32, dr, g);
+                const float fl = __builtin_floorf(gx);
+                g = gx - fl;
+                f = (f + static_cast<unsigned>(static_cast<int>(fl))) & static_cast<unsigned>(mask);
+                a = __builtin_amdgcn_fractf(a + fmaf(kh_hi, dr, kh_lo * dr));
+            }
             constexpr unsigned kReach = (64 - TAPS) / 2;
             const unsigned lo = __builtin_amdgcn_readfirstlane(f) - kReach;
             const unsigned off = f - lo;
@@ -864,8 +903,7 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 ai = fmaf(w, v[t].y, ai);
                 up *= e1;
             }
-            const float xa = static_cast<float>(fract64(pp.kh * rng));
-            const float zr = cos_rev(xa), zi = sin_rev(xa);
+            const float zr = cos_rev(a), zi = sin_rev(a);
             br = fmaf(zr, ar, fmaf(-zi, ai, br));
             bi = fmaf(zr, ai, fmaf(zi, ar, bi));
         }

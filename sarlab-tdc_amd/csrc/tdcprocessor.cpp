@@ -119,6 +119,10 @@ void TdcProcessor::setProfileKernel(int taps, int oversampling)
 {
     check(tdc_set_profile_kernel(h_, taps, oversampling));
 }
+void TdcProcessor::setProfileExactRange(bool exact)
+{
+    check(tdc_set_profile_exact_range(h_, exact ? 1 : 0));
+}
 void TdcProcessor::setRangeWindow(int kind, double p0, double p1)
 {
     check(tdc_set_range_window(h_, kind, p0, p1));

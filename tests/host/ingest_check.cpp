@@ -2,6 +2,7 @@
 // tdc::parallel_for visits every index exactly once, also when called from
 // several threads at once, and tdc::scan_antenna agrees with a plain serial
 // pass over the same per-sample antenna states. Prints "OK" and exits 0.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -134,6 +135,138 @@ void check_scan(const Antenna &a, bool with_vel, bool with_att, const char *what
 
 } // namespace
 
+// Origin-relative chirp positions of an arc of radius `radius` about the
+// origin, npri chirps a `step` apart, with Gaussian jitter.
+std::vector<tdc::ChirpPos> arc_track(int npri, double radius, double step, double jitter,
+                                     std::mt19937 &rng)
+{
+    std::normal_distribution<double> n01(0.0, 1.0);
+    std::vector<tdc::ChirpPos> c(npri);
+    for (int n = 0; n < npri; ++n) {
+        const double ang = (n - npri / 2) * step / radius;
+        const double x = radius * std::sin(ang) + jitter * n01(rng);
+        const double y = -radius * std::cos(ang) + jitter * n01(rng);
+        const double z = 0.3 + jitter * n01(rng);
+        c[n] = tdc::ChirpPos{x, y, z, x * x + y * y + z * z};
+    }
+    return c;
+}
+
+// The range increments the range-profile tile forms in FP32 (the same
+// operations) against the FP64 ranges, over blocks of 8 chirps: the largest
+// step error in metres.
+double incremental_error(const std::vector<tdc::ChirpPos> &c, const std::vector<double> &px)
+{
+    double worst = 0.0;
+    const int npri = static_cast<int>(c.size());
+    for (size_t p = 0; p + 2 < px.size(); p += 3) {
+        const float qx = static_cast<float>(-2.0 * px[p]), qy = static_cast<float>(-2.0 * px[p + 1]),
+                    qz = static_cast<float>(-2.0 * px[p + 2]);
+        auto range = [&](int n) {
+            const double dx = c[n].x - px[p], dy = c[n].y - px[p + 1], dz = c[n].z - px[p + 2];
+            return std::sqrt(dx * dx + dy * dy + dz * dz);
+        };
+        for (int nb = 0; nb < npri; nb += 8) {
+            float s = static_cast<float>(range(nb));
+            float hs = 0.5F / s;
+            for (int n = nb + 1; n < std::min(npri, nb + 8); ++n) {
+                const float dx = static_cast<float>(c[n].x - c[n - 1].x);
+                const float dy = static_cast<float>(c[n].y - c[n - 1].y);
+                const float dz = static_cast<float>(c[n].z - c[n - 1].z);
+                const float dw = static_cast<float>(c[n].pad - c[n - 1].pad);
+                const float delta = std::fmaf(qz, dz, std::fmaf(qy, dy, std::fmaf(qx, dx, dw)));
+                const float e0 = delta * hs;
+                const float t = e0 * hs;
+                const float dr = e0 * std::fmaf(-t, std::fmaf(-2.0F, t, 1.0F), 1.0F);
+                worst = std::max(worst, std::fabs(static_cast<double>(dr) - (range(n) - range(n - 1))));
+                s += dr;
+                hs = 0.5F / s;
+            }
+        }
+    }
+    return worst;
+}
+
+// The largest wavenumber for which profile_incremental_ok holds (bisection):
+// the bound it applies per step is then kIncrementalCycles / kmax metres.
+double kmax_limit(const std::vector<tdc::ChirpPos> &c, const double gmin[3], const double gmax[3])
+{
+    double lo = 0.0, hi = 1e12;
+    for (int i = 0; i < 200; ++i) {
+        const double mid = 0.5 * (lo + hi);
+        if (tdc::profile_incremental_ok(c.data(), static_cast<int>(c.size()), gmin, gmax, mid)) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+void check_incremental(std::mt19937 &rng)
+{
+    // a 0.5 m grid at the origin, a 1.5 m arc, 1 mm steps with 0.5 mm jitter
+    std::vector<tdc::ChirpPos> c = arc_track(1024, 1.5, 1e-3, 5e-4, rng);
+    std::uniform_real_distribution<double> u(-0.25, 0.25);
+    std::vector<double> px(3 * 400);
+    for (double &v : px) v = u(rng);
+    const double gmin[3] = {-0.25, -0.25, -0.25}, gmax[3] = {0.25, 0.25, 0.25};
+    expect(tdc::profile_incremental_ok(c.data(), 1024, gmin, gmax, 527.0),
+           "incremental range allowed for a 1.5 m arc with 1 mm steps at 77 GHz");
+    const double k = kmax_limit(c, gmin, gmax);
+    const double err = incremental_error(c, px);
+    std::printf("incremental range: largest step error %.3e m, bound %.3e m\n", err,
+                tdc::kIncrementalCycles / k);
+    expect(err <= tdc::kIncrementalCycles / k, "FP32 range steps within the bound");
+    // the same with 2 cm steps: the series' remainder fails the bound
+    std::vector<tdc::ChirpPos> big = arc_track(64, 1.5, 2e-2, 0.0, rng);
+    expect(!tdc::profile_incremental_ok(big.data(), 64, gmin, gmax, 527.0),
+           "2 cm steps take the FP64 range");
+    // pixels reaching the antenna's path: no positive distance bound
+    const double gmin2[3] = {-2.0, -2.0, -2.0}, gmax2[3] = {2.0, 2.0, 2.0};
+    expect(!tdc::profile_incremental_ok(c.data(), 1024, gmin2, gmax2, 527.0),
+           "overlapping boxes take the FP64 range");
+    // an antenna 10 km away: delta grows with the range, its rounding error
+    // over 2 R does not; a wavenumber 1000 times larger fails the bound
+    std::vector<tdc::ChirpPos> far = c;
+    for (auto &q : far) {
+        q.y -= 1e4;
+        q.pad = q.x * q.x + q.y * q.y + q.z * q.z;
+    }
+    expect(tdc::profile_incremental_ok(far.data(), 1024, gmin, gmax, 527.0),
+           "a distant antenna keeps the incremental range");
+    const double kf = kmax_limit(far, gmin, gmax);
+    const double errf = incremental_error(far, px);
+    std::printf("distant antenna: largest step error %.3e m, bound %.3e m\n", errf,
+                tdc::kIncrementalCycles / kf);
+    expect(errf <= tdc::kIncrementalCycles / kf,
+           "FP32 range steps of a distant antenna within the bound");
+    expect(!tdc::profile_incremental_ok(c.data(), 1024, gmin, gmax, 527e3),
+           "a 1000 times larger wavenumber takes the FP64 range");
+    expect(!tdc::profile_incremental_ok(c.data(), 1, gmin, gmax, 527.0), "one chirp: FP64");
+    std::vector<tdc::ChirpPos> bad = c;
+    bad[7].x = std::numeric_limits<double>::quiet_NaN();
+    expect(!tdc::profile_incremental_ok(bad.data(), 1024, gmin, gmax, 527.0), "NaN position: FP64");
+    const double gnan[3] = {std::numeric_limits<double>::quiet_NaN(), 0.0, 0.0};
+    expect(!tdc::profile_incremental_ok(c.data(), 1024, gnan, gmax, 527.0), "NaN grid bound: FP64");
+
+    // grid_bounds against a serial pass, and NaN
+    std::vector<double> g(3 * 100003);
+    for (double &v : g) v = u(rng) * 7.0;
+    double lo[3], hi[3];
+    tdc::grid_bounds(g.data(), g.size() / 3, lo, hi);
+    bool same = true;
+    for (int a = 0; a < 3; ++a) {
+        double l = g[a], h = g[a];
+        for (size_t i = 0; i < g.size() / 3; ++i) {
+            l = std::min(l, g[3 * i + a]);
+            h = std::max(h, g[3 * i + a]);
+        }
+        same = same && l == lo[a] && h == hi[a];
+    }
+    expect(same, "grid_bounds equals a serial pass");
+    g[3 * 5000 + 1] = std::numeric_limits<double>::quiet_NaN();
+    tdc::grid_bounds(g.data(), g.size() / 3, lo, hi);
+    expect(std::isnan(lo[0]) && std::isnan(hi[2]), "grid_bounds reports NaN");
+}
+
 int main()
 {
     for (size_t n : {size_t(0), size_t(1), size_t(5), size_t(1000), size_t(1000003)})
@@ -152,6 +285,7 @@ int main()
     expect(bad.load() == 0, "parallel_for from four threads at once");
 
     std::mt19937 rng(11);
+    check_incremental(rng);
     Antenna a = make_antenna(1021, 64, rng);
     check_scan(a, true, true, "stop-and-hop states");
     expect(tdc::scan_antenna(a.pos.data(), a.vel.data(), a.att.data(), a.npri, a.ns).pos_const,

@@ -26,7 +26,8 @@ class Stats(ctypes.Structure):
                 ("pixels_per_lane", ctypes.c_int32), ("workgroups", ctypes.c_int32),
                 ("skipped_chunks", ctypes.c_int64), ("processed_chunks", ctypes.c_int64),
                 ("engine", ctypes.c_int32), ("per_chirp", ctypes.c_int32),
-                ("profile_taps", ctypes.c_int32), ("profile_oversampling", ctypes.c_int32)]
+                ("profile_taps", ctypes.c_int32), ("profile_oversampling", ctypes.c_int32),
+                ("profile_incremental", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 @pytest.fixture(scope="module")

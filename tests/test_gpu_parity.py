@@ -408,6 +408,43 @@ def test_profile_kernel_choice(tdc):
         p.start(0.0, 1e30, False, False, False)
 
 
+def test_profile_incremental_range(tdc, orc):
+    """Within blocks of 8 chirps the range-profile tile advances each pixel's
+    range, bin position and anchor phase by FP32 increments when the host's
+    error bound allows it (config 2's 256 chirps, 2.7 mm steps at 0.8 m: it
+    does; config 1's 64 chirps, 4.8 mm steps at 0.4 m: it does not), and every
+    chirp's range in FP64 when asked. Both meet the oracle at 1e-4 and agree
+    with each other to 2e-6."""
+    from sarlabtdc import scene
+
+    def run(s, exact):
+        f = s.f64()
+        p = tdc.TdcProcessor(0)
+        p.setLogging(False)
+        p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"],
+                        f["modRate"], f["startFreq"])
+        p.setFocusGridF64(f["focusGrid"])
+        p.setProfileExactRange(exact)
+        p.start(0.0, 1e30, False, False, False)
+        return p.getFocusedImage().reshape(-1).copy(), p.stats()
+
+    s = scene.line_jitter()
+    inc, st_inc = run(s, False)
+    ex, st_ex = run(s, True)
+    assert st_inc["engine"] == "profile" and st_inc["incremental"]
+    assert not st_ex["incremental"]
+    agree = orc.rel_l2(inc, ex)
+    px = subset(inc.size, 768, extra=np.argsort(np.abs(ex))[-16:])
+    ref = oracle_f64(orc, s, pixels=px)
+    print("incremental vs FP64 range", agree, "oracle", orc.rel_l2(inc[px], ref),
+          orc.rel_l2(ex[px], ref))
+    assert agree < 2e-6
+    check(orc, inc[px], ref, PREC_TOL)
+    check(orc, ex[px], ref, PREC_TOL)
+    _, st_small = run(scene.line_small(), False)
+    assert st_small["engine"] == "profile" and not st_small["incremental"]
+
+
 def test_profile_engine_sample_limit(tdc, orc):
     """Beyond 8192 samples per chirp the range-profile engine declines with a
     message and the automatic choice takes the matrix engine."""

@@ -1,7 +1,10 @@
 """Host ingest helpers of the library (no device calls): the thread pool that
-splits passes over the inputs and the fused antenna-state scan that decides
-the per-chirp paths. A small C++ harness links the built library and checks
-them against serial passes (tests/host/ingest_check.cpp)."""
+splits passes over the inputs, the fused antenna-state scan that decides the
+per-chirp paths, the grid's bounding box, and the bound that lets the
+range-profile tile advance ranges by FP32 increments (checked against the
+same FP32 operations emulated on the host over a 1024-chirp arc). A small C++
+harness links the built library and checks them against serial passes
+(tests/host/ingest_check.cpp)."""
 import os
 import shutil
 import subprocess
