@@ -83,6 +83,32 @@ def test_random_geometry(tdc, orc, case):
             p.fetchImage()
             again = p.getFocusedImage()
             assert np.array_equal(img.view(np.uint32), again.view(np.uint32))
+            if engine == "profile" and storage == "c64":
+                # the incremental range (when the host's bound allows it) against
+                # the FP64 range for every chirp
+                INCREMENTAL.append(bool(p.stats()["incremental"]))
+                if INCREMENTAL[-1]:
+                    p.setProfileExactRange(True)
+                    p.start(0.0, 1e30, False, False, False)
+                    assert not p.stats()["incremental"]
+                    ex = p.getFocusedImage().reshape(-1)
+                    agree = float(orc.rel_l2(img.reshape(-1), ex))
+                    print(f"  incremental vs FP64 range: {agree:.2e}", flush=True)
+                    assert agree <= 5e-6
+                    assert float(orc.rel_l2(ex, ref)) <= tol
+
+
+# cases of test_random_geometry whose range-profile pass took the incremental range
+INCREMENTAL = []
+
+
+def test_random_geometry_covers_incremental_range():
+    """Runs after the random geometries: a share of them (small antenna steps
+    against the stand-off) took the incremental range, the rest the FP64 one."""
+    print("incremental range in", sum(INCREMENTAL), "of", len(INCREMENTAL), "cases")
+    if len(INCREMENTAL) < N_CASES - 2:
+        pytest.skip("runs after the whole random-geometry set")
+    assert 2 <= sum(INCREMENTAL) <= len(INCREMENTAL) - 3
 
 
 # ---------------------------------------------------------------------------- plain form
