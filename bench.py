@@ -253,7 +253,7 @@ def time_variant(tdc, s, kind, storage, steps, warmup, engine="auto", doppler=No
 def served_kernel(ns, npx):
     """(taps, oversampling) the server's range-profile tiles run for chirps of
     ns samples and npx pixels: the library's automatic choice."""
-    return (8, 4) if profile_length(ns) * 2 <= 16384 and npx >= 32768 else (10, 2)
+    return (6, 4) if profile_length(ns) * 2 <= 16384 and npx >= 32768 else (10, 2)
 
 
 def serve_stream(tdc, srv, scenes, rate, form, storage="c64", seed=0):

@@ -207,12 +207,13 @@ int tdc_fetch_image(tdc_processor *h);
 int tdc_get_stats(const tdc_processor *h, tdc_stats *out);
 
 /* Interpolation kernel of the range-profile engine: (taps, oversampling) =
- * (8, 2), (10, 2), (12, 2) or (8, 4); (0, 0) = automatic, (8, 4) while the
- * fourfold transform fits (chirps of up to 4096 samples) and the image has at
- * least 32768 pixels (the larger table then pays for itself), else (10, 2). The
- * kernel's error against the direct sum is ~1e-5, ~1e-6, ~1e-7 and ~2e-6 of
- * a chirp's profile (rel-L2); each two taps fewer save ~8 of ~90 instructions
- * per (pixel, chirp), fourfold oversampling doubles the per-chirp table. */
+ * (8, 2), (10, 2), (12, 2), (6, 4) or (8, 4); (0, 0) = automatic, (6, 4) while
+ * the fourfold transform fits (chirps of up to 4096 samples) and the image has
+ * at least 32768 pixels (the larger table then pays for itself), else (10, 2).
+ * The kernel's error against the direct sum is ~1e-5, ~1e-6, ~1e-7, ~4e-5 and
+ * ~2e-6 of a chirp's profile (rel-L2); each two taps fewer save 8 of ~74
+ * instructions per (pixel, chirp), fourfold oversampling doubles the per-chirp
+ * table. */
 int tdc_set_profile_kernel(tdc_processor *h, int taps, int oversampling);
 
 /* Range of the range-profile engine. exact = 0 (the default): within each

@@ -1249,7 +1249,7 @@ int tdc_set_profile_kernel(tdc_processor *h, int taps, int oversampling)
     if (!((taps == 0 && oversampling == 0) || tdc::profile_kernel_valid(taps, oversampling)))
         return fail(TDC_ERR_INVALID_ARGUMENT,
                     "the range-profile kernels are (taps, oversampling) = (8, 2), (10, 2), "
-                    "(12, 2) and (8, 4), or (0, 0) for the automatic choice");
+                    "(12, 2), (6, 4) and (8, 4), or (0, 0) for the automatic choice");
     h->profile_taps = taps;
     h->profile_over = oversampling;
     h->prepared = false;

@@ -109,18 +109,21 @@ struct DopParams {
 // chosen per kernel for the smallest error. Built kernels (taps, over) and
 // their error against the direct sum (rel-L2 of one chirp's profile at
 // random frequencies, 512 Hann-windowed samples): (12, 2) 6.3e-8,
-// (10, 2) 8.0e-7, (8, 2) 1.1e-5, (8, 4) 2.3e-6 -- fourfold oversampling buys
-// the accuracy of about two more taps for twice the table.
+// (10, 2) 8.0e-7, (8, 2) 1.1e-5, (8, 4) 2.3e-6, (6, 4) 4.0e-5 -- fourfold
+// oversampling buys the accuracy of about two more taps for twice the table.
+// A Gaussian cut at |u| = taps / 2 cannot do much better than
+// exp(-pi taps / 2) whatever the oversampling: (6, 8) measures 2.1e-5.
 constexpr int kProfileTaps = 12; // the widest kernel
 inline bool profile_kernel_valid(int taps, int over)
 {
-    return (over == 2 && (taps == 8 || taps == 10 || taps == 12)) || (over == 4 && taps == 8);
+    return (over == 2 && (taps == 8 || taps == 10 || taps == 12)) ||
+           (over == 4 && (taps == 6 || taps == 8));
 }
 // k of the kernel (taps, over); tdc_tiles.h holds the device constants of
 // exp(-b u^2) generated from the same values
 inline double profile_bfactor(int taps, int over)
 {
-    if (over == 4) return 0.88;
+    if (over == 4) return taps == 6 ? 0.87 : 0.88;
     return taps == 12 ? 0.79 : 0.80;
 }
 // The FFT of one chirp runs in the LDS, of which a gfx950 workgroup may
@@ -211,12 +214,16 @@ inline int profile_length(int ns, int over = 2)
 }
 // The range-profile kernel for chirps of ns samples and an image of npx
 // pixels: `taps` and `over` as requested, or (0, 0) for the automatic choice:
-// 8 taps with fourfold oversampling when the transform fits (M <=
+// 6 taps with fourfold oversampling when the transform fits (M <=
 // kProfileMaxM) and the image has at least kProfileOver4Pixels pixels, else
 // 10 taps with twofold oversampling. The fourfold table costs one more FFT
-// pass and twice the points per chirp, the two taps it saves ~7% per (pixel,
-// chirp) (measured on 1024 x 512 x 512^2: staging 10 -> 15 us, the
-// correlate-sum 0.460 -> 0.422 ms), so it pays from ~3e4 pixels per chirp.
+// pass and twice the points per chirp (measured on 1024 x 512 x 512^2:
+// staging 10 -> 15 us); each two taps fewer save 8 of ~74 VALU instructions
+// per (pixel, chirp) (the correlate-sum there: (10, 2) 0.460 ms, (8, 4)
+// 0.410 ms, (6, 4) 0.372 ms), so the fourfold table pays from ~3e4 pixels
+// per chirp. (6, 4) keeps the image within ~3e-5 of the direct sum, a third
+// of the 1e-4 bound; (8, 4) and the twofold kernels are there for tighter
+// bounds.
 // False with a reason when the request is not a built kernel or its
 // transform is too long.
 constexpr int64_t kProfileOver4Pixels = 32768;
@@ -228,7 +235,7 @@ inline bool profile_choice(int ns, int64_t npx, int taps, int over, ProfileChoic
 {
     if (taps == 0 && over == 0) {
         if (profile_length(ns, 4) <= kProfileMaxM && npx >= kProfileOver4Pixels) {
-            taps = 8;
+            taps = 6;
             over = 4;
         } else {
             taps = 10;
@@ -237,7 +244,7 @@ inline bool profile_choice(int ns, int64_t npx, int taps, int over, ProfileChoic
     }
     if (!profile_kernel_valid(taps, over)) {
         why = "the range-profile engine's kernels are (taps, oversampling) = (8, 2), (10, 2), "
-              "(12, 2) and (8, 4)";
+              "(12, 2), (6, 4) and (8, 4)";
         return false;
     }
     const int M = profile_length(ns, over);

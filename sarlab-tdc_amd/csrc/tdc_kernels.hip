@@ -1208,7 +1208,10 @@ hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const doubl
 {
     const int nb = correlate_grid(g);
     *workgroups = nb;
-    if (pp.over == 4 && pp.taps == 8)
+    if (pp.over == 4 && pp.taps == 6)
+        hipLaunchKernelGGL((profile_kernel<6, 4>), dim3(nb), dim3(kBlock), 0, s, table, cpos, grid,
+                           pp, g, out);
+    else if (pp.over == 4 && pp.taps == 8)
         hipLaunchKernelGGL((profile_kernel<8, 4>), dim3(nb), dim3(kBlock), 0, s, table, cpos, grid,
                            pp, g, out);
     else if (pp.over == 2 && pp.taps == 8)

@@ -100,10 +100,10 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const StepTable *__restri
                 const auto *tab = static_cast<const float2 *>(j.raw);
                 double2 *out = static_cast<double2 *>(j.partial) +
                                static_cast<size_t>(tl.split) * j.npx;
-                // served jobs run the automatic kernel: (8, 4), or (10, 2) for small
+                // served jobs run the automatic kernel: (6, 4), or (10, 2) for small
                 // images and chirps whose fourfold transform would not fit
                 if (j.pp.over == 4)
-                    profile_tile<8, 4>(tab, j.cpos, j.grid64, j.pp, j.npx, n0, n1, tl.pix0, out);
+                    profile_tile<6, 4>(tab, j.cpos, j.grid64, j.pp, j.npx, n0, n1, tl.pix0, out);
                 else
                     profile_tile<10, 2>(tab, j.cpos, j.grid64, j.pp, j.npx, n0, n1, tl.pix0, out);
             }

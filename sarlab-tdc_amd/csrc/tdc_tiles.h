@@ -773,6 +773,13 @@ template <> struct Kernel<12, 2> { // k = 0.79
                                      0.02416651468988259F,   0.0013357150352831012F,
                                      3.227957702166608e-05F, 3.410798652928483e-07F};
 };
+template <> struct Kernel<6, 4> { // k = 0.87
+    static constexpr float kE0 = -1.3143844411298862F;
+    static constexpr float kE1 = 2.6287688822597723F;
+    static constexpr float kC[6] = {0.02614107418320684F, 0.40209702267679975F,
+                                    1.0F,                 0.40209702267679975F,
+                                    0.02614107418320684F, 0.0002747753163046619F};
+};
 template <> struct Kernel<8, 4> { // k = 0.88
     static constexpr float kE0 = -0.9971192312019825F;
     static constexpr float kE1 = 1.994238462403965F;
@@ -890,7 +897,8 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 for (int t = 0; t < TAPS; ++t) v[t] = r[t];
             }
             // up0 = exp2(g (kE0 g - lead kE1)), e1 = exp2(kE1 g): for g in
-            // [0, 1) up0 lies in (0.013, 1] and up0 e1^(TAPS - 1) below 6e3
+            // [0, 1) and every built kernel up0 lies in (0.0079, 1] and
+            // up0 e1^(TAPS - 1) below 126, far inside FP32's range
             using K = profile::Kernel<TAPS, OVER>;
             constexpr int kLead = TAPS / 2 - 1;
             float up = __builtin_amdgcn_exp2f(g * fmaf(K::kE0, g, -kLead * K::kE1));

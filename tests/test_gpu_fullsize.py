@@ -26,7 +26,7 @@ def arc():
     return scene.make("arc")
 
 
-def precision(tdc, s, raw=None, storage="c64", engine="auto", splits=0):
+def precision(tdc, s, raw=None, storage="c64", engine="auto", splits=0, kernel=(0, 0)):
     f = s.f64()
     if raw is not None:
         f["rawData"] = np.ascontiguousarray(raw, dtype=np.complex64)
@@ -38,6 +38,7 @@ def precision(tdc, s, raw=None, storage="c64", engine="auto", splits=0):
     p.setRawStorage(storage)
     p.setEngine(engine)
     p.setLaunch(0, splits)
+    p.setProfileKernel(*kernel)
     p.start(0.0, 1e30, False, False, False)
     return p.getFocusedImage().reshape(-1), p.stats()
 
@@ -77,16 +78,22 @@ def test_arc_linearity(tdc, arc):
 def test_arc_engines_and_storage(tdc, arc):
     m, _ = precision(tdc, arc, engine="matrix")
     v, _ = precision(tdc, arc, engine="vector")
-    p, _ = precision(tdc, arc, engine="profile")
+    p, st = precision(tdc, arc, engine="profile")
+    p8, st8 = precision(tdc, arc, engine="profile", kernel=(8, 4))
     h, _ = precision(tdc, arc, storage="f16")
+    assert (st["taps"], st["oversampling"]) == (6, 4)
+    assert (st8["taps"], st8["oversampling"]) == (8, 4)
     ev = np.linalg.norm(m - v) / np.linalg.norm(v)
     ep = np.linalg.norm(p - v) / np.linalg.norm(v)
+    ep8 = np.linalg.norm(p8 - v) / np.linalg.norm(v)
     eh = np.linalg.norm(h - p) / np.linalg.norm(p)
-    print(f"matrix vs vector {ev:.3e}; profile vs vector {ep:.3e}; f16 storage vs complex64 "
-          f"{eh:.3e}")
-    # each engine is within ~2e-6 of the direct sum here (the range-profile
-    # engine's automatic 8-tap kernel with fourfold oversampling the furthest)
-    assert ev < 5e-6 and ep < 5e-6
+    print(f"matrix vs vector {ev:.3e}; profile (6, 4) vs vector {ep:.3e}, (8, 4) {ep8:.3e}; "
+          f"f16 storage vs complex64 {eh:.3e}")
+    # the matrix and vector engines are within ~1e-6 of the direct sum here,
+    # the range-profile engine within its kernel's error: ~3e-5 for the
+    # automatic 6-tap kernel with fourfold oversampling, ~2e-6 with 8 taps
+    assert ev < 5e-6 and ep8 < 5e-6
+    assert ep < 5e-5
     assert eh < 3e-3
 
 
@@ -119,17 +126,20 @@ def test_arc_half_storage_vs_oracle(tdc, orc, arc, arc_subset, storage, tol):
     """FP16 and BF16 raw storage at the full arc size against the FP64 oracle
     on the complex64 input (4096 seeded pixels plus the 64 brightest): FP16
     meets the 1e-4 bound; BF16's 8-bit mantissa alone puts the image ~1.5e-4
-    away, above it. Against the oracle on the samples rounded as stored, both
-    are within 1e-5: the error is the storage rounding, carried exactly."""
+    away, above it. Against the oracle on the samples rounded as stored, the
+    8-tap kernel is within 1e-5: the error is the storage rounding, carried
+    exactly."""
     px, ref = arc_subset
     img, st = precision(tdc, arc, storage=storage)
     assert st["raw_storage"] == storage and st["engine"] == "profile"
     e = orc.rel_l2(img[px], ref)
+    img8, _ = precision(tdc, arc, storage=storage, kernel=(8, 4))
     rounded = round_storage(arc.raw, storage)
     ref_r = orc.image_f64(rounded, arc.sample_times, arc.track, arc.grid, arc.mod_rate, arc.f0,
                           pixels=px)
-    er = orc.rel_l2(img[px], ref_r)
-    print(f"arc {storage}: rel-L2 {e:.3e} vs the complex64 input, {er:.3e} vs the stored samples")
+    er = orc.rel_l2(img8[px], ref_r)
+    print(f"arc {storage}: rel-L2 {e:.3e} vs the complex64 input, {er:.3e} vs the stored samples "
+          f"(8-tap kernel)")
     assert e <= tol
     assert er <= 1e-5
 

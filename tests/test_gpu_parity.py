@@ -361,8 +361,8 @@ def test_profile_engine_window_and_gather_paths(tdc, orc, spacing):
     check(orc, img, oracle_f64(orc, s), PREC_TOL)
 
 
-@pytest.mark.parametrize("taps,over,ns", [(8, 2, 300), (10, 2, 300), (12, 2, 300), (8, 4, 300),
-                                          (8, 4, 4096), (10, 2, 8192)])
+@pytest.mark.parametrize("taps,over,ns", [(8, 2, 300), (10, 2, 300), (12, 2, 300), (6, 4, 300),
+                                          (8, 4, 300), (6, 4, 4096), (8, 4, 4096), (10, 2, 8192)])
 def test_profile_engine_kernels(tdc, orc, taps, over, ns):
     """Every built interpolation kernel of the range-profile engine (taps,
     oversampling) against the oracle, each at the longest chirps its transform
@@ -383,17 +383,17 @@ def test_profile_engine_kernels(tdc, orc, taps, over, ns):
 
 
 def test_profile_kernel_choice(tdc):
-    """The automatic kernel: (8, 4) up to 4096 samples per chirp for images of
+    """The automatic kernel: (6, 4) up to 4096 samples per chirp for images of
     at least 32768 pixels, (10, 2) for smaller images or longer chirps; a
     kernel that is not built, or fourfold oversampling beyond 4096 samples, is
     refused."""
     p = tdc.TdcProcessor(0)
     p.setLogging(False)
     with pytest.raises(ValueError):
-        p.setProfileKernel(6, 4)
+        p.setProfileKernel(6, 2)
     with pytest.raises(ValueError):
         p.setProfileKernel(10, 4)
-    for ns, side, want in ((4096, 182, (8, 4)), (4096, 181, (10, 2)), (4097, 182, (10, 2))):
+    for ns, side, want in ((4096, 182, (6, 4)), (4096, 181, (10, 2)), (4097, 182, (10, 2))):
         s = ragged_scene(npri=4, ns=ns, rows=side, cols=side, seed=ns + side)
         f = s.f64()
         p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"],
@@ -403,9 +403,10 @@ def test_profile_kernel_choice(tdc):
         p.start(0.0, 1e30, False, False, False)
         st = p.stats()
         assert (st["taps"], st["oversampling"]) == want
-    p.setProfileKernel(8, 4)
-    with pytest.raises(Exception, match="at most 4096 samples"):
-        p.start(0.0, 1e30, False, False, False)
+    for taps in (6, 8):
+        p.setProfileKernel(taps, 4)
+        with pytest.raises(Exception, match="at most 4096 samples"):
+            p.start(0.0, 1e30, False, False, False)
 
 
 def test_profile_incremental_range(tdc, orc):

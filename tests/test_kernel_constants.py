@@ -28,14 +28,17 @@ def _bfactor(taps, over):
     text = open(os.path.join(CSRC, "tdc_internal.h")).read()
     body = re.search(r"inline double profile_bfactor\(int taps, int over\)\s*\{(.*?)\n\}", text,
                      re.S).group(1)
-    m4 = float(re.search(r"if \(over == 4\) return ([0-9.]+);", body).group(1))
+    m6, m4 = map(float, re.search(r"if \(over == 4\) return taps == 6 \? ([0-9.]+) : ([0-9.]+);",
+                                  body).groups())
     m12, mother = map(float, re.search(r"return taps == 12 \? ([0-9.]+) : ([0-9.]+);",
                                        body).groups())
-    return m4 if over == 4 else (m12 if taps == 12 else mother)
+    if over == 4:
+        return m6 if taps == 6 else m4
+    return m12 if taps == 12 else mother
 
 
 def test_every_built_kernel_has_constants():
-    assert set(_kernels()) == {(8, 2), (10, 2), (12, 2), (8, 4)}
+    assert set(_kernels()) == {(8, 2), (10, 2), (12, 2), (6, 4), (8, 4)}
 
 
 def test_constants_follow_the_host_kernel():
