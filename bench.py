@@ -725,6 +725,10 @@ def main():
                                      "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64",
                                      "SQ_INSTS_VALU_CVT") if k in sq},
                         source=os.path.relpath(tfile, ROOT))
+            # a SIMD issues up to two VALU instructions per quad-cycle: the shares
+            # of its quad-cycles that issued two, one and none (third PMC pass)
+            if "issue" in tj:
+                simd["issue"] = tj["issue"]
         roof = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS,
                     unit="TFLOP/s", frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=traffic,
                     kernel=(f"profile_kernel ({head['storage']}, {taps} taps, "
