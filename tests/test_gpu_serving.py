@@ -93,13 +93,13 @@ def test_job_is_bit_identical_alone_and_in_a_mix(tdc):
     assert st["completed"] == 6 and st["failed"] == 0
 
 
-@pytest.mark.parametrize("nc,ns,side,tiles", [(2048, 128, 16, 128), (512, 256, 64, 512),
+@pytest.mark.parametrize("nc,ns,side,tiles", [(2048, 128, 16, 32), (512, 256, 64, 128),
                                               (600, 128, 256, 2048)])
 def test_jobs_with_few_pixel_tiles_are_cut_into_short_chirp_ranges(tdc, orc, nc, ns, side, tiles):
     """A job with few 256-pixel tiles is cut into chirp ranges short enough
-    for 8192 tiles, down to 16 chirps, while its partial images stay within
+    for 8192 tiles, down to 64 chirps, while its partial images stay within
     four times its input bytes: 16^2 pixels x 2048 chirps gives one pixel
-    tile x 128 ranges of 16 chirps, 64^2 x 512 gives 16 x 32, and 256^2 x 600
+    tile x 32 ranges of 64 chirps, 64^2 x 512 gives 16 x 8, and 256^2 x 600
     (2.2 MB of input, 1 MB per partial image) 256 x 8 ranges of 75 chirps.
     The image sums the partial images within the precision form's tolerance
     and is the same bit for bit alone and beside other jobs."""
