@@ -453,6 +453,37 @@ def serve_leg(tdc, args):
             f"{rec_s['makespan_s']:.3f} s (device busy {rec_s['device_busy_s']:.3f} s, executed "
             f"{rec_s['fp32_peak_frac']:.3f} of the FP32 peak over the makespan)")
         out["precision_saturation"] = rec_s
+    if args.serve_resident_jobs > 0:
+        # the same jobs formed one at a time on the resident path (a processor
+        # per job's inputs, the same engine and kernel): the device time the
+        # server's steps are measured against
+        n_res = min(len(scenes), args.serve_resident_jobs)
+        p = tdc.TdcProcessor(0)
+        p.setLogging(False)
+        win = (0.0, 1e30, False, False, False)
+        main_ms = pass_ms = 0.0
+        for sc in scenes[:n_res]:
+            f = sc.f64()
+            p.setRawDataF64(f["rawData"], f["priTimes"], f["sampleTimes"], f["position"],
+                            f["modRate"], f["startFreq"])
+            p.setFocusGridF64(f["focusGrid"])
+            p.prepare(*win)
+            p.run(1)
+            p.run(3)
+            st = p.stats()
+            main_ms += st["main_kernel_ms"]
+            pass_ms += st["kernel_ms"]
+        res_rec = dict(jobs=n_res, sum_correlate_ms=main_ms, sum_pass_ms=pass_ms)
+        if n_res == len(scenes) and "precision_saturation" in out:
+            busy_ms = out["precision_saturation"]["device_busy_s"] * 1e3
+            res_rec["saturated_steps_ms"] = busy_ms
+            res_rec["steps_vs_correlate"] = main_ms / busy_ms
+            res_rec["steps_vs_pass"] = pass_ms / busy_ms
+        out["resident_same_jobs"] = res_rec
+        log(f"serving: the same {n_res} jobs one at a time on the resident path: correlate-sum "
+            f"{main_ms:.1f} ms, whole passes {pass_ms:.1f} ms"
+            + (f" (saturated steps {res_rec['saturated_steps_ms']:.1f} ms)"
+               if "saturated_steps_ms" in res_rec else ""))
     n_plain = min(len(scenes), args.serve_plain_jobs)
     if n_plain > 0:
         rec_p, _ = serve_stream(tdc, srv, scenes[:n_plain], args.serve_rate, "plain")
@@ -520,6 +551,9 @@ def main():
     ap.add_argument("--serve-saturation-rate", type=float, default=100000.0,
                     help="offered jobs/s of a third precision stream of the same jobs, above the "
                          "device's capacity (0: skip)")
+    ap.add_argument("--serve-resident-jobs", type=int, default=1000,
+                    help="jobs of the stream also formed one at a time on the resident path, "
+                         "for the device time of the same work without the server (0: skip)")
     ap.add_argument("--serve-plain-jobs", type=int, default=200,
                     help="jobs of the stream also served in the plain form")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -739,6 +773,18 @@ def main():
                                   fp32_equivalent_per_pixel_chirp=fp32eq,
                                   pixel_chirp_pairs_per_launch=float(s.n_pixels) * s.n_chirps),
                     direct_sum_equiv=direct_equiv, simd_cycles=simd)
+        # the same kernel's arithmetic as the hardware counts it
+        # (SQ_INSTS_VALU_FLOPS_* of the committed PMC pass, per pair), over this
+        # run's kernel time: a check on the hand count above
+        hwf = tj.get("hw_flops") if os.path.exists(tfile) else None
+        if hwf and hwf.get("fp32_equivalent"):
+            hw_ach = pairs_per_s * hwf["fp32_equivalent"] / 1e12
+            roof["executed"]["hw_counted"] = dict(
+                fp32_per_pixel_chirp=hwf["fp32"], fp32_trans_per_pixel_chirp=hwf["fp32_trans"],
+                fp64_per_pixel_chirp=hwf["fp64"],
+                fp32_equivalent_per_pixel_chirp=hwf["fp32_equivalent"],
+                tflops=round(hw_ach, 3), frac=round(hw_ach / PEAK_FP32_TFLOPS, 4),
+                source=os.path.relpath(tfile, ROOT))
     elif head["engine"] == "matrix":
         # FP16 products on the matrix cores; the VALU part (phasors, nested
         # sums, anchors) issues on the same SIMDs in between
