@@ -768,7 +768,13 @@ size_t precision_chirp_bytes(const tdc_processor *h, const PrecisionInputs &in)
     return b;
 }
 
-size_t precision_pixel_bytes(int nsplit) { return 24 + 8 + sizeof(double2) * static_cast<size_t>(nsplit); }
+// Bytes of one pixel's partial image per chirp split: FP32 from the
+// range-profile engine, FP64 from the others.
+size_t partial_bytes(int engine) { return engine == TDC_ENGINE_PROFILE ? sizeof(float2) : sizeof(double2); }
+size_t precision_pixel_bytes(int nsplit, int engine)
+{
+    return 24 + 8 + partial_bytes(engine) * static_cast<size_t>(nsplit);
+}
 
 int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool centroid,
                       bool apply_rw, bool taper, const PrecisionInputs *given)
@@ -786,7 +792,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     const int nsplit = choose_split(ntiles, npri, h->force_split);
     if (h->budget > 0) {
         const double need = double(precision_chirp_bytes(h, in)) * npri +
-                            double(precision_pixel_bytes(nsplit)) * npx;
+                            double(precision_pixel_bytes(nsplit, in.engine)) * npx;
         if (need > double(h->budget))
             return fail(TDC_ERR_UNSUPPORTED,
                         "the image needs " + std::to_string(static_cast<long long>(need)) +
@@ -822,7 +828,7 @@ int prepare_precision(tdc_processor *h, float dop_center, float dop_bw, bool cen
     h->run_engine = in.engine;
     make_geom(h->geom, npx, npri, ns, h->ppl, nsplit);
     TDC_HIP(h->d_img.ensure(sizeof(float2) * static_cast<size_t>(npx)));
-    TDC_HIP(h->d_part.ensure(sizeof(double2) * static_cast<size_t>(npx) * nsplit));
+    TDC_HIP(h->d_part.ensure(partial_bytes(in.engine) * static_cast<size_t>(npx) * nsplit));
     h->run_form = TDC_FORM_PRECISION;
     h->run_storage = h->storage;
     return TDC_OK;
@@ -1337,7 +1343,7 @@ int launch_main(tdc_processor *h, const tdc::Geom &g, size_t p0, int *wgs)
     double2 *dst = h->d_part.as<double2>() + p0;
     if (h->run_engine == TDC_ENGINE_PROFILE) {
         TDC_HIP(tdc::launch_profile(h->d_ptab.as<float2>(), h->d_pcpos.as<tdc::ChirpPos>(), grid,
-                                    h->pp, g, dst, s, wgs));
+                                    h->pp, g, h->d_part.as<float2>() + p0, s, wgs));
     } else if (h->run_engine == TDC_ENGINE_MATRIX) {
         TDC_HIP(tdc::launch_precision_matrix(
             tdc::matrix_parts(h->run_storage), h->d_atab.p, h->d_ascale.as<float>(),
@@ -1359,6 +1365,9 @@ int launch_reduce(tdc_processor *h)
         if (g.nsplit > 1)
             TDC_HIP(tdc::launch_finalize_f32(h->d_part.as<float2>(), g.nsplit, g.npx,
                                              h->d_img.as<float2>(), h->stream));
+    } else if (h->run_engine == TDC_ENGINE_PROFILE) {
+        TDC_HIP(tdc::launch_finalize_p32(h->d_part.as<float2>(), g.nsplit, g.npx,
+                                         h->d_img.as<float2>(), h->stream));
     } else {
         TDC_HIP(tdc::launch_finalize_f64(h->d_part.as<double2>(), g.nsplit, g.npx,
                                          h->d_img.as<float2>(), h->stream));
@@ -1488,7 +1497,8 @@ int ring_plan(const tdc_processor *h, const PrecisionInputs &in, RingPlan &plan)
     const int cps_full = (npri + nsplit_full - 1) / nsplit_full;
     for (int sps = 16; sps >= 1; sps /= 2) {
         // grid, image, FP64 sum and the slab's partial images
-        const double fixed = (24.0 + 8.0 + 16.0 + 16.0 * sps) * npx + tables;
+        const double fixed =
+            (24.0 + 8.0 + 16.0 + double(partial_bytes(in.engine)) * sps) * npx + tables;
         const double chirps = (double(h->budget) - fixed) / (2.0 * per_chirp);
         int nsplit;
         if (h->force_split > 0) {
@@ -1547,7 +1557,7 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
         }
         if (!h->r_done[k]) TDC_HIP(hipEventCreateWithFlags(&h->r_done[k], hipEventDisableTiming));
     }
-    TDC_HIP(h->d_part.ensure(sizeof(double2) * static_cast<size_t>(npx) * plan.sps));
+    TDC_HIP(h->d_part.ensure(partial_bytes(in.engine) * static_cast<size_t>(npx) * plan.sps));
     TDC_HIP(h->d_acc.ensure(sizeof(double2) * static_cast<size_t>(npx)));
     TDC_HIP(h->d_img.ensure(sizeof(float2) * static_cast<size_t>(npx)));
     h->run_form = TDC_FORM_PRECISION;
@@ -1599,7 +1609,7 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
                 h->pp.taps, h->pp.reverse, h->r_tab[set].as<float2>(), s));
             TDC_HIP(tdc::launch_profile(h->r_tab[set].as<float2>(),
                                         h->r_cp[set].as<tdc::ChirpPos>(), h->d_grid.as<double>(),
-                                        h->pp, gs, h->d_part.as<double2>(), s, &w));
+                                        h->pp, gs, h->d_part.as<float2>(), s, &w));
         } else if (in.engine == TDC_ENGINE_MATRIX) {
             const int parts = tdc::matrix_parts(h->storage);
             TDC_HIP(tdc::launch_stage_matrix(h->storage, h->r_rec[set].p, nk, ns, parts,
@@ -1617,9 +1627,14 @@ int run_ring(tdc_processor *h, const PrecisionInputs &in, const RingPlan &plan)
         wgs += w;
         TDC_HIP(hipEventRecord(h->r_done[set], s));
         const bool last = n0 + slab >= npri;
-        TDC_HIP(tdc::launch_accumulate_f64(h->d_part.as<double2>(), gs.nsplit, npx, k == 0 ? 1 : 0,
-                                           h->d_acc.as<double2>(),
-                                           last ? h->d_img.as<float2>() : nullptr, s));
+        if (in.engine == TDC_ENGINE_PROFILE)
+            TDC_HIP(tdc::launch_accumulate_p32(h->d_part.as<float2>(), gs.nsplit, npx,
+                                               k == 0 ? 1 : 0, h->d_acc.as<double2>(),
+                                               last ? h->d_img.as<float2>() : nullptr, s));
+        else
+            TDC_HIP(tdc::launch_accumulate_f64(h->d_part.as<double2>(), gs.nsplit, npx,
+                                               k == 0 ? 1 : 0, h->d_acc.as<double2>(),
+                                               last ? h->d_img.as<float2>() : nullptr, s));
     }
     TDC_HIP(hipEventRecord(evt[2], s));
     TDC_HIP(hipEventRecord(evt[3], s));
@@ -1763,7 +1778,7 @@ int tdc_start(tdc_processor *h, float doppler_win_center, float doppler_bandwidt
         const int ntiles = (npx + tdc::kBlock * h->ppl - 1) / (tdc::kBlock * h->ppl);
         const int nsplit = choose_split(ntiles, h->npri, h->force_split);
         const double need = double(precision_chirp_bytes(h, pin)) * h->npri +
-                            double(precision_pixel_bytes(nsplit)) * npx;
+                            double(precision_pixel_bytes(nsplit, pin.engine)) * npx;
         if (need > double(h->budget) && npx > 0) {
             RingPlan plan;
             if ((rc = ring_plan(h, pin, plan)) != TDC_OK) return rc;

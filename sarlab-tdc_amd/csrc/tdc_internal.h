@@ -322,9 +322,15 @@ inline size_t matrix_table_bytes(int npri, int ns, int storage)
 inline int anchor_entries(int ns) { return 4 * matrix_quads(ns); }
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s);
+// The same FP64 sum over FP32 partial images (the range-profile engine's).
+hipError_t launch_finalize_p32(const float2 *partial, int nsplit, int npx, float2 *img,
+                               hipStream_t s);
 // acc = (first ? 0 : acc) + the nsplit partial images in split order; with img
-// non-null the sum is rounded into img instead of stored in acc.
+// non-null the sum is rounded into img instead of stored in acc (_p32: FP32
+// partial images).
 hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, int first,
+                                 double2 *acc, float2 *img, hipStream_t s);
+hipError_t launch_accumulate_p32(const float2 *partial, int nsplit, int npx, int first,
                                  double2 *acc, float2 *img, hipStream_t s);
 // Range-profile tables [npri][Mp] from the staged raw samples (in `storage`
 // format, range window folded in); corr[i] = 1 / psi_hat((i - h) / M);
@@ -342,9 +348,11 @@ hipError_t launch_stage_profile_folded(int storage, const float *raw, const floa
                                        const float *corr, const float2 *twiddle,
                                        const ChirpPos *cpos, int npri, int ns, int M, int Mp,
                                        int taps, int reverse, float2 *table, hipStream_t s);
-// `cpos`: chirp positions minus the origin (pp.ox, oy, oz), with |pos - o|^2 in pad.
+// `cpos`: chirp positions minus the origin (pp.ox, oy, oz), with |pos - o|^2 in
+// pad; `out`: one FP32 partial image per chirp split (sum them in FP64 with
+// launch_finalize_p32 / launch_accumulate_p32).
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
-                          const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,
+                          const ProfileParams &pp, const Geom &g, float2 *out, hipStream_t s,
                           int *workgroups);
 
 // A named roctx range over a scope: host phases (prepare, run, fetch, job

@@ -733,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void profile_kernel(const float2 *__restric
                                                          const ChirpPos *__restrict__ cpos,
                                                          const double *__restrict__ grid,
                                                          ProfileParams pp, Geom g,
-                                                         double2 *__restrict__ out)
+                                                         float2 *__restrict__ out)
 {
     int tile, split;
     map_block(g, tile, split);
@@ -914,7 +914,10 @@ __global__ __launch_bounds__(kBlock) void precision_matrix_kernel(
                        tile * kBlock, out + static_cast<size_t>(split) * g.pstride);
 }
 
-__global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const double2 *__restrict__ partial,
+// FP64 sum of the per-split partial images (FP64, or FP32 from the
+// range-profile engine) in split order.
+template <class PT>
+__global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const PT *__restrict__ partial,
                                                               int nsplit, int npx,
                                                               float2 *__restrict__ img)
 {
@@ -922,9 +925,9 @@ __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const double2 *__r
     if (p >= npx) return;
     double sr = 0.0, si = 0.0;
     for (int s = 0; s < nsplit; ++s) {
-        const double2 v = partial[static_cast<size_t>(s) * npx + p];
-        sr += v.x;
-        si += v.y;
+        const PT v = partial[static_cast<size_t>(s) * npx + p];
+        sr += static_cast<double>(v.x);
+        si += static_cast<double>(v.y);
     }
     img[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
 }
@@ -1203,7 +1206,7 @@ hipError_t launch_stage_profile_folded(int storage, const float *raw, const floa
 }
 
 hipError_t launch_profile(const float2 *table, const ChirpPos *cpos, const double *grid,
-                          const ProfileParams &pp, const Geom &g, double2 *out, hipStream_t s,
+                          const ProfileParams &pp, const Geom &g, float2 *out, hipStream_t s,
                           int *workgroups)
 {
     const int nb = correlate_grid(g);
@@ -1340,7 +1343,8 @@ hipError_t launch_precision_matrix(int parts, const void *atab, const float *inv
 // Slabs of an aperture streamed through the device: acc = (first ? 0 : acc)
 // plus the slab's partial images in split order, the order finalize_f64_kernel
 // adds a resident pass's splits in; the last slab rounds the sum into img.
-__global__ __launch_bounds__(kBlock) void accumulate_f64_kernel(const double2 *__restrict__ partial,
+template <class PT>
+__global__ __launch_bounds__(kBlock) void accumulate_f64_kernel(const PT *__restrict__ partial,
                                                                 int nsplit, int npx, int first,
                                                                 double2 *__restrict__ acc,
                                                                 float2 *__restrict__ img)
@@ -1354,9 +1358,9 @@ __global__ __launch_bounds__(kBlock) void accumulate_f64_kernel(const double2 *_
         si = a.y;
     }
     for (int s = 0; s < nsplit; ++s) {
-        const double2 v = partial[static_cast<size_t>(s) * npx + p];
-        sr += v.x;
-        si += v.y;
+        const PT v = partial[static_cast<size_t>(s) * npx + p];
+        sr += static_cast<double>(v.x);
+        si += static_cast<double>(v.y);
     }
     if (img) img[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
     else acc[p] = make_double2(sr, si);
@@ -1366,16 +1370,33 @@ hipError_t launch_accumulate_f64(const double2 *partial, int nsplit, int npx, in
                                  double2 *acc, float2 *img, hipStream_t s)
 {
     if (npx <= 0) return hipSuccess;
-    hipLaunchKernelGGL(accumulate_f64_kernel, dim3((npx + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       partial, nsplit, npx, first, acc, img);
+    hipLaunchKernelGGL(accumulate_f64_kernel<double2>, dim3((npx + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, partial, nsplit, npx, first, acc, img);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate_p32(const float2 *partial, int nsplit, int npx, int first,
+                                 double2 *acc, float2 *img, hipStream_t s)
+{
+    if (npx <= 0) return hipSuccess;
+    hipLaunchKernelGGL(accumulate_f64_kernel<float2>, dim3((npx + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, partial, nsplit, npx, first, acc, img);
     return hipGetLastError();
 }
 
 hipError_t launch_finalize_f64(const double2 *partial, int nsplit, int npx, float2 *img,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL(finalize_f64_kernel, dim3((npx + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       partial, nsplit, npx, img);
+    hipLaunchKernelGGL(finalize_f64_kernel<double2>, dim3((npx + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, partial, nsplit, npx, img);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_p32(const float2 *partial, int nsplit, int npx, float2 *img,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(finalize_f64_kernel<float2>, dim3((npx + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, partial, nsplit, npx, img);
     return hipGetLastError();
 }
 

@@ -87,7 +87,7 @@ Layout layout(const tdc::NoallocInputs &in, int nsplit)
     l.table = o;
     o += align256(sizeof(float2) * static_cast<size_t>(in.npri) * in.pp.Mp);
     l.part = o;
-    o += align256(sizeof(double2) * npx * nsplit);
+    o += align256(sizeof(float2) * npx * nsplit); // FP32 partial images, one per split
     l.img = o;
     o += align256(sizeof(float2) * npx);
     l.end = o;
@@ -151,7 +151,7 @@ extern "C" int tdc_start_noalloc(tdc_processor *h, float doppler_win_center,
     auto *pcpos = reinterpret_cast<tdc::ChirpPos *>(base + l.pcpos);
     auto *grid = reinterpret_cast<double *>(base + l.grid);
     auto *table = reinterpret_cast<float2 *>(base + l.table);
-    auto *part = reinterpret_cast<double2 *>(base + l.part);
+    auto *part = reinterpret_cast<float2 *>(base + l.part);
     auto *img = reinterpret_cast<float2 *>(base + l.img);
 
     // inputs from host memory (copies through the runtime's own staging)
@@ -176,7 +176,7 @@ extern "C" int tdc_start_noalloc(tdc_processor *h, float doppler_win_center,
                                             s));
     if (npx) {
         NA_HIP(tdc::launch_profile(table, pcpos, grid, in.pp, g, part, s, &wgs));
-        NA_HIP(tdc::launch_finalize_f64(part, nsplit, in.npx, img, s));
+        NA_HIP(tdc::launch_finalize_p32(part, nsplit, in.npx, img, s));
         NA_HIP(hipMemcpyAsync(image_iq, img, sizeof(float2) * npx, hipMemcpyDeviceToHost, s));
     }
     NA_HIP(hipStreamSynchronize(s));

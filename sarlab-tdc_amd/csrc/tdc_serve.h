@@ -28,7 +28,8 @@ struct alignas(16) ServeJob {
     const double *grid64;     // FP64 pixel coordinates
     const double *kanchor;    // matrix engine: 2 (f0 + K t_a) / c per 64-sample block;
                               // plain: the anchor wavenumbers k_b per 64-sample block
-    void *partial;            // [nsplit][npx]: double2 (precision) or float2 (plain)
+    void *partial;            // [nsplit][npx]: double2 (matrix engine) or float2 (plain form,
+                              // range-profile engine)
     float2 *image;            // [npx]
     double dk;                // precision, matrix engine: 2 K dt / c
     ProfileParams pp;         // precision, range-profile engine

@@ -98,8 +98,8 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const StepTable *__restri
                                                     static_cast<size_t>(tl.split) * j.npx);
             } else {
                 const auto *tab = static_cast<const float2 *>(j.raw);
-                double2 *out = static_cast<double2 *>(j.partial) +
-                               static_cast<size_t>(tl.split) * j.npx;
+                float2 *out = static_cast<float2 *>(j.partial) +
+                              static_cast<size_t>(tl.split) * j.npx;
                 // served jobs run the automatic kernel: (6, 4), or (10, 2) for small
                 // images and chirps whose fourfold transform would not fit
                 if (j.pp.over == 4)
@@ -143,6 +143,17 @@ __global__ __launch_bounds__(kBlock) void serve_finalize_kernel(const StepTable 
                 si += v.y;
             }
             j.image[p] = make_float2(sr, si);
+        } else if (j.profile) {
+            // FP32 partial images of the range-profile tiles, added in FP64
+            const float2 *part = static_cast<const float2 *>(j.partial);
+            double sr = 0.0, si = 0.0;
+#pragma unroll 8
+            for (int s = 0; s < j.nsplit; ++s) {
+                const float2 v = part[static_cast<size_t>(s) * j.npx + p];
+                sr += static_cast<double>(v.x);
+                si += static_cast<double>(v.y);
+            }
+            j.image[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
         } else {
             const double2 *part = static_cast<const double2 *>(j.partial);
             double sr = 0.0, si = 0.0;

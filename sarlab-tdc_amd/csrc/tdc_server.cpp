@@ -374,7 +374,8 @@ void plan(Job &j)
     j.cps = std::min(j.cps, fill_cps);
     j.cps = std::min(j.cps, d.n_pri);
     {
-        const double part_px = prec ? sizeof(double2) : sizeof(float2);
+        // FP64 partial images for the matrix engine, FP32 for the others
+        const double part_px = (prec && !j.profile) ? sizeof(double2) : sizeof(float2);
         const double input = 8.0 * static_cast<double>(nterm) + (prec ? 24.0 : 12.0) * npx;
         const int max_split =
             std::max(1, static_cast<int>(kMaxPartialFactor * input / (part_px * npx)));
@@ -432,7 +433,7 @@ void plan(Job &j)
         o += align256(sizeof(float) * d.n_pri);
     }
     j.off_part = o;
-    o += align256((prec ? sizeof(double2) : sizeof(float2)) * npx * j.nsplit);
+    o += align256(((prec && !j.profile) ? sizeof(double2) : sizeof(float2)) * npx * j.nsplit);
     j.bytes = arena_class(o);
 }
 

@@ -730,7 +730,8 @@ __device__ __forceinline__ void matrix_tile(const uintx4_t *__restrict__ atab,
 // into an integer bin f and a fraction g, the TAPS table entries around u
 // weighted by the Gaussian kernel (two exp2 per pair, two multiplies per
 // tap), and the anchor exp(j 2 pi k_h R); chirps accumulate in FP32 blocks
-// of kProfileBlock chirps and the blocks in FP64. R, u and the anchor phase
+// of kProfileBlock chirps and the blocks in FP64, and the tile's partial
+// image is stored rounded to FP32 (its splits are added in FP64). R, u and the anchor phase
 // are evaluated in FP64 for the first chirp of each block and, with pp.incr,
 // advanced by FP32 increments for the block's other chirps (else in FP64 for
 // every chirp).
@@ -810,7 +811,7 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                                              const ChirpPos *__restrict__ cpos,
                                              const double *__restrict__ grid,
                                              const ProfileParams &pp, int npx, int n0, int n1,
-                                             int pix0, double2 *__restrict__ out)
+                                             int pix0, float2 *__restrict__ out)
 {
     __shared__ float2 s_win[kBlock / 64][64];
     const int lane = static_cast<int>(threadIdx.x) & 63;
@@ -918,7 +919,7 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
         accr += static_cast<double>(br);
         acci += static_cast<double>(bi);
     }
-    if (pix < npx) out[pix] = make_double2(accr, acci);
+    if (pix < npx) out[pix] = make_float2(static_cast<float>(accr), static_cast<float>(acci));
 }
 
 } // namespace tdc

@@ -222,7 +222,7 @@ def test_budget_streams_chirp_slabs(tdc, orc, long_aperture, engine, storage):
 
 def test_budget_too_small_and_plain_form(tdc, long_aperture):
     """A budget below the per-pixel buffers (grid, image, FP64 sum and one
-    partial image: 64 B a pixel, 256 KiB here) holds no slab and fails with a
+    FP32 partial image: 56 B a pixel, 224 KiB here) holds no slab and fails with a
     message; 1 MiB already streams slabs of 8 chirps. The plain form keeps its
     aperture resident whatever the budget."""
     s = long_aperture

@@ -94,13 +94,14 @@ def test_job_is_bit_identical_alone_and_in_a_mix(tdc):
 
 
 @pytest.mark.parametrize("nc,ns,side,tiles", [(2048, 128, 16, 32), (512, 256, 64, 128),
-                                              (600, 128, 256, 2048)])
+                                              (600, 128, 256, 2560)])
 def test_jobs_with_few_pixel_tiles_are_cut_into_short_chirp_ranges(tdc, orc, nc, ns, side, tiles):
     """A job with few 256-pixel tiles is cut into chirp ranges short enough
     for 8192 tiles, down to 64 chirps, while its partial images stay within
     four times its input bytes: 16^2 pixels x 2048 chirps gives one pixel
     tile x 32 ranges of 64 chirps, 64^2 x 512 gives 16 x 8, and 256^2 x 600
-    (2.2 MB of input, 1 MB per partial image) 256 x 8 ranges of 75 chirps.
+    (2.2 MB of input, 0.5 MB per FP32 partial image) 256 x 10 ranges of 64
+    chirps.
     The image sums the partial images within the precision form's tolerance
     and is the same bit for bit alone and beside other jobs."""
     s = job_scene(21, nc, ns, side)
@@ -180,7 +181,8 @@ def test_few_pixel_jobs_under_a_tight_budget(tdc, orc):
     inputs = 8 * 386 * 128 + 24 * npx
     nsplit = info["tiles"] / ((npx + 255) // 256)
     print("footprint", one, "splits", nsplit, "inputs", inputs)
-    assert 16 * npx * nsplit <= 4 * inputs
+    # FP32 partial images (the range-profile engine's)
+    assert 8 * npx * nsplit <= 4 * inputs
     # arena: upload region, staged tables and partials, rounded up to a size class
     assert one <= 8 * inputs
     srv = tdc.Server(0, 2.5 * one)
