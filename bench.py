@@ -65,16 +65,16 @@ MFMA_FLOP_PER_TERM = {"c64": 24.0, "f16": 16.0, "bf16": 16.0}
 # A chirp whose range is evaluated in FP64 (every chirp, or the first of each
 # block of 8 with the incremental range) adds FP64 |pos|^2 - 2 pos.p + |p|^2
 # (7), the clamped Newton range step (5), the bin position (2) and the anchor
-# phase (1), and with the incremental range FP32 0.5 / R (2); the block's
-# other chirps add FP32 delta (6), e0 and t (2), the series (4), dR (1), R and
-# 0.5 / R (3), the bin position (3) and the anchor phase (4). Counted as
+# phase (1), and with the incremental range FP32 1 / R (1); the block's other
+# chirps add FP32 delta / 2 (6), e0 and u (2), the series (4), dR (1), R and
+# 1 / R (2), the bin position (3) and the anchor phase (3). Counted as
 # FP32-equivalent flops (an FP64 flop issues at half the FP32 rate) against
 # the FP32 peak.
 PROFILE_BLOCK = 8
 PROFILE_FP64_BLOCK_SUMS = 0.25
 PROFILE_FP64_EXACT_RANGE = 15.0
-PROFILE_FP32_INCR_FIRST = 2.0
-PROFILE_FP32_INCR_STEP = 23.0
+PROFILE_FP32_INCR_FIRST = 1.0
+PROFILE_FP32_INCR_STEP = 21.0
 
 
 def profile_fp32_per_pair(taps, incr=False):
@@ -122,7 +122,7 @@ def profile_incremental(sc):
     dkm = abs(2.0 * sc.mod_rate * dt / 299792458.0) * m
     eps = 2.0 ** -24
     e0 = dc * (1.0 + dc / rmin)
-    err = (eps * (5.0 * cn + 6.0 * pn) * dc / (2.0 * rmin) + 4.0 * eps * e0
+    err = (eps * (5.0 * cn + 6.0 * pn) * dc / (2.0 * rmin) + 5.0 * eps * e0
            + 0.625 * e0 ** 4 / rmin ** 3)
     return max(abs(kh), dkm) * err <= 2e-6
 

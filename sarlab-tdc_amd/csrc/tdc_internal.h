@@ -162,8 +162,10 @@ constexpr int kProfileDeltaEntry = kProfilePad - 2;
 // series dR = e0 (1 - t (1 - 2 t)), e0 = delta / (2 R_{n-1}), t = e0 / (2
 // R_{n-1}), whose remainder is below 0.625 e0^4 / R^3 with |e0| <= |dc|max
 // (1 + |dc|max / R); the FP32 products that form dR (the reciprocal of R,
-// e0 and the series) add at most 4 eps32 |e0|. Their sum, times the largest
-// wavenumber kmax, must stay below kIncrementalCycles per step; R is bounded below by the distance between the
+// e0 and the series) add at most 4 eps32 |e0|, and the FP32 roundings of k_h
+// and |dk| M that advance the phase and bin by dR one more eps32 |e0|. Their
+// sum, times the largest wavenumber kmax, must stay below kIncrementalCycles
+// per step; R is bounded below by the distance between the
 // bounding boxes of the chirp positions and the pixels. `c`: origin-relative
 // chirp positions; `gmin`/`gmax`: the grid's bounding box relative to the
 // same origin.
@@ -202,7 +204,7 @@ inline bool profile_incremental_ok(const ChirpPos *c, int npri, const double gmi
     const double eps32 = 1.0 / 16777216.0;
     const double e0 = dc * (1.0 + dc / rmin);
     const double err_delta = eps32 * (5.0 * std::sqrt(cnorm2) + 6.0 * std::sqrt(pnorm2)) * dc;
-    const double err_dr = err_delta / (2.0 * rmin) + 4.0 * eps32 * e0 +
+    const double err_dr = err_delta / (2.0 * rmin) + 5.0 * eps32 * e0 +
                           0.625 * e0 * e0 * e0 * e0 / (rmin * rmin * rmin);
     return kmax * err_dr <= kIncrementalCycles;
 }

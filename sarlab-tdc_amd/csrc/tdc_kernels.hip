@@ -572,7 +572,8 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 w)
 // Row n of the table from the transform x in LDS: row[j] = x[(j - lead) mod M]
 // (x[(lead - j) mod M] for down-chirps) for j < kProfileDeltaEntry past M,
 // then the chirp's position increment against chirp n - 1 (zero for the
-// first chirp) as one float4 in the last two entries, rounded from FP64.
+// first chirp) as one float4 in the last two entries, rounded from FP64:
+// (c_n - c_{n-1}, (|c_n|^2 - |c_{n-1}|^2) / 2).
 template <int BLOCK>
 __device__ __forceinline__ void write_profile_row(const float2 *x, const ChirpPos *__restrict__ cpos,
                                                   int n, int M, int Mp, int lead, int reverse,
@@ -588,7 +589,7 @@ __device__ __forceinline__ void write_profile_row(const float2 *x, const ChirpPo
         if (n > 0) {
             const ChirpPos c = cpos[n], p = cpos[n - 1];
             d = make_float4(static_cast<float>(c.x - p.x), static_cast<float>(c.y - p.y),
-                            static_cast<float>(c.z - p.z), static_cast<float>(c.pad - p.pad));
+                            static_cast<float>(c.z - p.z), static_cast<float>(0.5 * (c.pad - p.pad)));
         }
         *reinterpret_cast<float4 *>(dst + rows) = d;
     }

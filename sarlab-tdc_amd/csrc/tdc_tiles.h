@@ -825,22 +825,24 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     const double pp2 = fma(pz, pz, fma(py, py, px * px));
     const double qx = -2.0 * px, qy = -2.0 * py, qz = -2.0 * pz;
     const int mask = pp.M - 1;
-    // FP32 operands of the incremental range (pp.incr): -2 p', |dk| M, and k_h
-    // as the sum of two floats
-    const float qx32 = static_cast<float>(qx), qy32 = static_cast<float>(qy),
-                qz32 = static_cast<float>(qz);
+    // FP32 operands of the incremental range (pp.incr): -p', |dk| M and k_h
+    // (their rounding changes a step's bin and phase by a relative 2^-24 of
+    // |dk| M dR and k_h dR, inside profile_incremental_ok's bound)
+    const float qx32 = static_cast<float>(0.5 * qx), qy32 = static_cast<float>(0.5 * qy),
+                qz32 = static_cast<float>(0.5 * qz);
     const float dkm32 = static_cast<float>(pp.dkm);
-    const float kh_hi = static_cast<float>(pp.kh);
-    const float kh_lo = static_cast<float>(pp.kh - static_cast<double>(kh_hi));
+    const float kh32 = static_cast<float>(pp.kh);
     // chirp contributions are summed in FP32 over blocks of kProfileBlock
     // chirps and the blocks in FP64
     double accr = 0.0, acci = 0.0;
     for (int nb = n0; nb < n1; nb += profile::kProfileBlock) {
         const int ne = min(n1, nb + profile::kProfileBlock);
         float br = 0.0F, bi = 0.0F;
-        // the pixel's state within the block: FP32 range s and 0.5 / s, bin f
-        // and fraction g of its position, anchor phase a = frac(k_h R)
-        float s = 0.0F, hs = 0.0F, g = 0.0F, a = 0.0F;
+        // the pixel's state within the block: FP32 range s and 1 / s, bin f
+        // and fraction g of its position, anchor phase a = frac(k_h R); within
+        // the block f is kept modulo 2^32 (a multiple of M) and masked to the
+        // row where a row is indexed with it
+        float s = 0.0F, is = 0.0F, g = 0.0F, a = 0.0F;
         unsigned f = 0;
         for (int n = nb; n < ne; ++n) {
             const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
@@ -860,26 +862,28 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 a = static_cast<float>(fract64(pp.kh * rng));
                 if (pp.incr) {
                     s = static_cast<float>(rng);
-                    hs = 0.5F * __builtin_amdgcn_rcpf(s);
+                    is = __builtin_amdgcn_rcpf(s);
                 }
             } else {
-                // the next chirp of the block: delta = R_n^2 - R_{n-1}^2 from the
-                // chirp's position increment (the row's last entries), dR from
-                // its series, then s, the bin position and the anchor phase
-                // advanced by dR (bounds: profile_incremental_ok)
+                // the next chirp of the block: delta / 2 = (R_n^2 - R_{n-1}^2) / 2
+                // from the chirp's position increment (the row's last entries),
+                // dR = e0 (1 - u / 2 + u^2 / 2), e0 = delta / (2 R), u = e0 / R
+                // (the series of sqrt(R^2 + delta) - R), then s, the bin position
+                // and the anchor phase advanced by dR (bounds:
+                // profile_incremental_ok)
                 const float4 d = as_const(
                     reinterpret_cast<const float4 *>(row + pp.M + kProfileDeltaEntry))[0];
-                const float delta = fmaf(qz32, d.z, fmaf(qy32, d.y, fmaf(qx32, d.x, d.w)));
-                const float e0 = delta * hs;
-                const float t = e0 * hs;
-                const float dr = e0 * fmaf(-t, fmaf(-2.0F, t, 1.0F), 1.0F);
+                const float hdelta = fmaf(qz32, d.z, fmaf(qy32, d.y, fmaf(qx32, d.x, d.w)));
+                const float e0 = hdelta * is;
+                const float u = e0 * is;
+                const float dr = e0 * fmaf(u, fmaf(0.5F, u, -0.5F), 1.0F);
                 s += dr;
-                hs = 0.5F * __builtin_amdgcn_rcpf(s);
+                is = __builtin_amdgcn_rcpf(s);
                 const float gx = fmaf(dkm32, dr, g);
                 const float fl = __builtin_floorf(gx);
                 g = gx - fl;
-                f = (f + static_cast<unsigned>(static_cast<int>(fl))) & static_cast<unsigned>(mask);
-                a = __builtin_amdgcn_fractf(a + fmaf(kh_hi, dr, kh_lo * dr));
+                f += static_cast<unsigned>(static_cast<int>(fl));
+                a = __builtin_amdgcn_fractf(fmaf(kh32, dr, a));
             }
             constexpr unsigned kReach = (64 - TAPS) / 2;
             const unsigned lo = __builtin_amdgcn_readfirstlane(f) - kReach;
@@ -893,7 +897,7 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) v[t] = win[off + t];
             } else {
-                const float2 *r = row + f;
+                const float2 *r = row + (f & static_cast<unsigned>(mask));
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) v[t] = r[t];
             }

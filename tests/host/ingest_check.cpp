@@ -160,27 +160,27 @@ double incremental_error(const std::vector<tdc::ChirpPos> &c, const std::vector<
     double worst = 0.0;
     const int npri = static_cast<int>(c.size());
     for (size_t p = 0; p + 2 < px.size(); p += 3) {
-        const float qx = static_cast<float>(-2.0 * px[p]), qy = static_cast<float>(-2.0 * px[p + 1]),
-                    qz = static_cast<float>(-2.0 * px[p + 2]);
+        const float qx = static_cast<float>(-px[p]), qy = static_cast<float>(-px[p + 1]),
+                    qz = static_cast<float>(-px[p + 2]);
         auto range = [&](int n) {
             const double dx = c[n].x - px[p], dy = c[n].y - px[p + 1], dz = c[n].z - px[p + 2];
             return std::sqrt(dx * dx + dy * dy + dz * dz);
         };
         for (int nb = 0; nb < npri; nb += 8) {
             float s = static_cast<float>(range(nb));
-            float hs = 0.5F / s;
+            float is = 1.0F / s;
             for (int n = nb + 1; n < std::min(npri, nb + 8); ++n) {
                 const float dx = static_cast<float>(c[n].x - c[n - 1].x);
                 const float dy = static_cast<float>(c[n].y - c[n - 1].y);
                 const float dz = static_cast<float>(c[n].z - c[n - 1].z);
-                const float dw = static_cast<float>(c[n].pad - c[n - 1].pad);
-                const float delta = std::fmaf(qz, dz, std::fmaf(qy, dy, std::fmaf(qx, dx, dw)));
-                const float e0 = delta * hs;
-                const float t = e0 * hs;
-                const float dr = e0 * std::fmaf(-t, std::fmaf(-2.0F, t, 1.0F), 1.0F);
+                const float dw = static_cast<float>(0.5 * (c[n].pad - c[n - 1].pad));
+                const float hdelta = std::fmaf(qz, dz, std::fmaf(qy, dy, std::fmaf(qx, dx, dw)));
+                const float e0 = hdelta * is;
+                const float u = e0 * is;
+                const float dr = e0 * std::fmaf(u, std::fmaf(0.5F, u, -0.5F), 1.0F);
                 worst = std::max(worst, std::fabs(static_cast<double>(dr) - (range(n) - range(n - 1))));
                 s += dr;
-                hs = 0.5F / s;
+                is = 1.0F / s;
             }
         }
     }
