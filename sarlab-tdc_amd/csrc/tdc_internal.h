@@ -133,7 +133,10 @@ constexpr int kProfileStockhamMaxM = 8192;
 constexpr int kProfileMaxM = 16384;
 // FFT length from which the per-chirp transform runs 1024 threads
 constexpr int kProfileWideM = 4096;
-constexpr int kProfilePad = 16;    // wrap-around entries after the M bins of a table row
+// Entries after the M bins of a table row: wrap-around copies of its first
+// bins (70), so that a wave's 64-entry window starting at any bin lies in the
+// row, and the chirp's position increment (the last 2)
+constexpr int kProfilePad = 72;
 struct ProfileParams {
     double dk; // 2 K dt / c: cycles per sample per metre of range
     double kh; // 2 (f0 + K t_h) / c: wavenumber at the re-centring sample h
@@ -148,9 +151,11 @@ struct ProfileParams {
                  // by FP32 increments from the block's first chirp (profile_incremental_ok)
 };
 // The last two entries of a table row (never read as taps, which reach at
-// most M + kProfileTaps - 2) hold the chirp's position increment: float4
-// (c_n - c_{n-1}, |c_n|^2 - |c_{n-1}|^2) of the origin-relative positions
+// most M + kProfileTaps - 2, nor as window entries, which reach at most
+// M + 62) hold the chirp's position increment: float4
+// (c_n - c_{n-1}, (|c_n|^2 - |c_{n-1}|^2) / 2) of the origin-relative positions
 constexpr int kProfileDeltaEntry = kProfilePad - 2;
+static_assert(kProfileDeltaEntry >= 63, "a 64-entry window from bin M - 1 must end before the increment");
 
 // Whether the range-profile tile may advance a pixel's range by FP32
 // increments within a block of chirps (the block's first chirp is exact in

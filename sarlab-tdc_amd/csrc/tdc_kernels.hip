@@ -601,9 +601,10 @@ __device__ __forceinline__ void write_profile_row(const float2 *x, const ChirpPo
 // twiddles staged in LDS up to kProfileLdsTwiddleM (the output comes out in
 // natural order), written
 // shifted by lead = taps / 2 - 1 bins, row[j] = B[(j - lead) mod M], over M +
-// kProfilePad entries: a pixel whose bin position has integer part f reads
-// its taps B[f - lead .. f - lead + taps - 1] as row[f .. f + taps - 1],
-// which never wrap. For down-chirps (`reverse`) the row holds
+// kProfileDeltaEntry entries: a pixel whose bin position has integer part f
+// reads its taps B[f - lead .. f - lead + taps - 1] as row[f .. f + taps - 1],
+// and a wave's 64-entry window starting at bin l as row[l .. l + 63], which
+// never wrap. For down-chirps (`reverse`) the row holds
 // B[(lead - j) mod M]: the profile at frequency -u is read at u. The row's
 // last two entries hold the chirp's position increment (write_profile_row).
 template <class RAW, int BLOCK, bool FOLD, bool LDS_TW>

@@ -890,7 +890,10 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
             const unsigned off = f - lo;
             float2 v[TAPS];
             if (__all(off <= 64U - TAPS)) {
-                const float2 e = row[(lo + lane) & mask];
+                // entries lo .. lo + 63 modulo M: from bin lo mod M on, the
+                // row's wrap-around entries cover the window's end
+                const float2 *wrow = row + (lo & static_cast<unsigned>(mask));
+                const float2 e = wrow[lane];
                 wave_lds_sync(); // the previous chirp's reads of the slot are done
                 win[lane] = e;
                 wave_lds_sync();
