@@ -30,6 +30,10 @@ template <class T> __device__ __forceinline__ cptr<T> as_const(const T *p)
     return (cptr<T>) p;
 }
 
+// Word 3 of a CDNA buffer resource for untyped loads: data format 32 bits
+// (a zero format would read as zero), no swizzle, no index stride
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
 
@@ -59,6 +63,14 @@ __device__ __forceinline__ double range_newton(double r2)
 
 // x - floor(x) in FP64 (one instruction)
 __device__ __forceinline__ double fract64(double x) { return __builtin_amdgcn_fract(x); }
+// floor(x) as an int in one instruction (v_cvt_flr_i32_f32, which the compiler
+// forms from a floor and a conversion only where it may assume no NaNs)
+__device__ __forceinline__ int floor_to_int(float x)
+{
+    int i;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
 
 // sin and cos of 2 pi x for x in [0, 1): reduction to the nearest quarter
 // turn, |2 pi f| <= pi/4, then the Taylor series through a^9 (sin) and a^10
@@ -815,6 +827,7 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
 {
     __shared__ float2 s_win[kBlock / 64][64];
     const int lane = static_cast<int>(threadIdx.x) & 63;
+    const unsigned lane_bytes = static_cast<unsigned>(lane) * static_cast<unsigned>(sizeof(float2));
     float2 *win = s_win[threadIdx.x >> 6];
     const int pix = pix0 + static_cast<int>(threadIdx.x);
     const size_t q = static_cast<size_t>(min(pix, npx - 1));
@@ -860,10 +873,9 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 // lies in the row; window indices are masked to the row
                 f = static_cast<unsigned>(static_cast<int>(x)) & static_cast<unsigned>(mask);
                 a = static_cast<float>(fract64(pp.kh * rng));
-                if (pp.incr) {
-                    s = static_cast<float>(rng);
-                    is = __builtin_amdgcn_rcpf(s);
-                }
+                // (read by the next chirp's increment only when pp.incr)
+                s = static_cast<float>(rng);
+                is = __builtin_amdgcn_rcpf(s);
             } else {
                 // the next chirp of the block: delta / 2 = (R_n^2 - R_{n-1}^2) / 2
                 // from the chirp's position increment (the row's last entries),
@@ -880,20 +892,30 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 s += dr;
                 is = __builtin_amdgcn_rcpf(s);
                 const float gx = fmaf(dkm32, dr, g);
-                const float fl = __builtin_floorf(gx);
-                g = gx - fl;
-                f += static_cast<unsigned>(static_cast<int>(fl));
+                f += static_cast<unsigned>(floor_to_int(gx));
+                g = __builtin_amdgcn_fractf(gx);
                 a = __builtin_amdgcn_fractf(fmaf(kh32, dr, a));
             }
             constexpr unsigned kReach = (64 - TAPS) / 2;
-            const unsigned lo = __builtin_amdgcn_readfirstlane(f) - kReach;
+            unsigned lo = __builtin_amdgcn_readfirstlane(f) - kReach;
+            // (lo kept opaque, so that the window test's off also addresses the
+            // window instead of being formed again from lane 0's bin)
+            asm volatile("" : "+s"(lo));
             const unsigned off = f - lo;
             float2 v[TAPS];
             if (__all(off <= 64U - TAPS)) {
                 // entries lo .. lo + 63 modulo M: from bin lo mod M on, the
                 // row's wrap-around entries cover the window's end
-                const float2 *wrow = row + (lo & static_cast<unsigned>(mask));
-                const float2 e = wrow[lane];
+                // (a buffer load: the wave-uniform window start in a scalar
+                // resource, the lane's byte offset in a loop-invariant register,
+                // and entries past the window's 64 read as zero)
+                const float2 e = __builtin_bit_cast(
+                    float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                __builtin_amdgcn_make_buffer_rsrc(
+                                    const_cast<float2 *>(row + (lo & static_cast<unsigned>(mask))),
+                                    static_cast<short>(0), 64 * static_cast<int>(sizeof(float2)),
+                                    kBufferRsrcWord3),
+                                lane_bytes, 0, 0));
                 wave_lds_sync(); // the previous chirp's reads of the slot are done
                 win[lane] = e;
                 wave_lds_sync();
