@@ -556,6 +556,41 @@ __device__ __forceinline__ void profile_sample(const void *src, const float *wr,
     }
 }
 
+// The same sample in two steps, so that a thread issues the loads of several
+// samples before it uses any: the loaded words, then their values.
+template <class RAW, bool FOLD> struct ProfileWord {
+    float2 v;
+    float w;
+};
+template <class RAW> struct ProfileWord<RAW, false> {
+    typename RAW::T t;
+};
+template <class RAW, bool FOLD>
+__device__ __forceinline__ ProfileWord<RAW, FOLD> profile_load(const void *src, const float *wr,
+                                                               int n, int ns, int i)
+{
+    const size_t e = static_cast<size_t>(n) * ns + i;
+    ProfileWord<RAW, FOLD> r;
+    if constexpr (FOLD) {
+        r.v = static_cast<const float2 *>(src)[e];
+        r.w = wr[i];
+    } else {
+        r.t = static_cast<const typename RAW::T *>(src)[e];
+    }
+    return r;
+}
+template <class RAW, bool FOLD>
+__device__ __forceinline__ void profile_value(const ProfileWord<RAW, FOLD> &r, float &re, float &im)
+{
+    if constexpr (FOLD) {
+        re = r.v.x * r.w;
+        im = r.v.y * r.w;
+        RAW::round(re, im);
+    } else {
+        RAW::unpack(r.t, re, im);
+    }
+}
+
 // exp(+j 2 pi k / M) for k < 3 M / 4 from the half-turn table (k < M / 2):
 // the second half turn is the first negated.
 __device__ __forceinline__ float2 twiddle_at(const float2 *tw, int k, int half)
@@ -622,22 +657,36 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     float2 *x = buf;
     float2 *y = buf + M;
     const float2 *tw = twiddle;
+    const int mask = M - 1;
     if constexpr (LDS_TW) {
         float2 *t = buf + 2 * M;
+#pragma unroll 4
         for (int k = tid; k < half; k += BLOCK) t[k] = twiddle[k];
         tw = t;
     }
-    // b[j] = c_i corr[i] for j = (i - h) mod M, zero elsewhere, in one pass
-    for (int j = tid; j < M; j += BLOCK) {
-        const int i = j < ns - h ? j + h : (j >= M - h ? j - M + h : -1);
-        float2 v = make_float2(0.0F, 0.0F);
-        if (i >= 0) {
-            float re, im;
-            profile_sample<RAW, FOLD>(src, wr, n, ns, i, re, im);
-            const float c = corr[i];
-            v = make_float2(re * c, im * c);
+    // b[j] = c_i corr[i] for j = (i - h) mod M: zeros at j in [ns - h, M - h),
+    // then the ns samples, kUnroll per thread at a time with their loads
+    // issued before the first is used
+    for (int j = ns - h + tid; j < M - h; j += BLOCK) x[j] = make_float2(0.0F, 0.0F);
+    constexpr int kUnroll = 4;
+    for (int i0 = tid; i0 < ns; i0 += kUnroll * BLOCK) {
+        ProfileWord<RAW, FOLD> wd[kUnroll];
+        float c[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int i = min(i0 + u * BLOCK, ns - 1);
+            wd[u] = profile_load<RAW, FOLD>(src, wr, n, ns, i);
+            c[u] = corr[i];
         }
-        x[j] = v;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const int i = i0 + u * BLOCK;
+            if (i < ns) {
+                float re, im;
+                profile_value<RAW, FOLD>(wd[u], re, im);
+                x[(i - h) & mask] = make_float2(re * c[u], im * c[u]);
+            }
+        }
     }
     __syncthreads();
     // Stockham passes, B[m] = sum_j b[j] exp(+j 2 pi j m / M) in natural order:

@@ -113,21 +113,27 @@ __device__ __forceinline__ void two_sum_acc(float &acc, float &comp, float x)
 // staging kernels can take raw samples straight from complex64 input.
 struct RawC64 {
     using T = float2;
-    static __device__ __forceinline__ void get(cptr<T> p, int i, float &re, float &im)
+    static __device__ __forceinline__ void unpack(T v, float &re, float &im)
     {
-        const float2 v = p[i];
         re = v.x;
         im = v.y;
+    }
+    static __device__ __forceinline__ void get(cptr<T> p, int i, float &re, float &im)
+    {
+        unpack(p[i], re, im);
     }
     static __device__ __forceinline__ void round(float &, float &) {}
 };
 struct RawF16 {
     using T = unsigned int;
-    static __device__ __forceinline__ void get(cptr<T> p, int i, float &re, float &im)
+    static __device__ __forceinline__ void unpack(T v, float &re, float &im)
     {
-        const unsigned int v = p[i];
         re = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v & 0xffffU)));
         im = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<unsigned short>(v >> 16)));
+    }
+    static __device__ __forceinline__ void get(cptr<T> p, int i, float &re, float &im)
+    {
+        unpack(p[i], re, im);
         // convert once per sample and share the FP32 value across the lane's
         // pixels (left to itself the compiler folds the conversion into every
         // FMA as v_fma_mix_f32, which measured 40% slower)
@@ -141,11 +147,14 @@ struct RawF16 {
 };
 struct RawBF16 {
     using T = unsigned int;
-    static __device__ __forceinline__ void get(cptr<T> p, int i, float &re, float &im)
+    static __device__ __forceinline__ void unpack(T v, float &re, float &im)
     {
-        const unsigned int v = p[i];
         re = __uint_as_float(v << 16);
         im = __uint_as_float(v & 0xffff0000U);
+    }
+    static __device__ __forceinline__ void get(cptr<T> p, int i, float &re, float &im)
+    {
+        unpack(p[i], re, im);
     }
     // round to nearest even (finite inputs)
     static __device__ __forceinline__ void round(float &re, float &im)
