@@ -965,6 +965,28 @@ __global__ __launch_bounds__(kBlock) void precision_matrix_kernel(
                        tile * kBlock, out + static_cast<size_t>(split) * g.pstride);
 }
 
+// Adds pixel p of the nsplit partial images (npx apart) to (sr, si) in split
+// order, with the loads of kSplitBatch splits in flight at a time.
+template <class PT>
+__device__ __forceinline__ void add_splits(const PT *__restrict__ partial, int nsplit, int npx,
+                                           int p, double &sr, double &si)
+{
+    constexpr int kSplitBatch = 8;
+    for (int s0 = 0; s0 < nsplit; s0 += kSplitBatch) {
+        PT v[kSplitBatch];
+#pragma unroll
+        for (int u = 0; u < kSplitBatch; ++u)
+            v[u] = partial[static_cast<size_t>(min(s0 + u, nsplit - 1)) * npx + p];
+#pragma unroll
+        for (int u = 0; u < kSplitBatch; ++u) {
+            if (s0 + u < nsplit) {
+                sr += static_cast<double>(v[u].x);
+                si += static_cast<double>(v[u].y);
+            }
+        }
+    }
+}
+
 // FP64 sum of the per-split partial images (FP64, or FP32 from the
 // range-profile engine) in split order.
 template <class PT>
@@ -975,11 +997,7 @@ __global__ __launch_bounds__(kBlock) void finalize_f64_kernel(const PT *__restri
     const int p = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
     if (p >= npx) return;
     double sr = 0.0, si = 0.0;
-    for (int s = 0; s < nsplit; ++s) {
-        const PT v = partial[static_cast<size_t>(s) * npx + p];
-        sr += static_cast<double>(v.x);
-        si += static_cast<double>(v.y);
-    }
+    add_splits(partial, nsplit, npx, p, sr, si);
     img[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
 }
 
@@ -1408,11 +1426,7 @@ __global__ __launch_bounds__(kBlock) void accumulate_f64_kernel(const PT *__rest
         sr = a.x;
         si = a.y;
     }
-    for (int s = 0; s < nsplit; ++s) {
-        const PT v = partial[static_cast<size_t>(s) * npx + p];
-        sr += static_cast<double>(v.x);
-        si += static_cast<double>(v.y);
-    }
+    add_splits(partial, nsplit, npx, p, sr, si);
     if (img) img[p] = make_float2(static_cast<float>(sr), static_cast<float>(si));
     else acc[p] = make_double2(sr, si);
 }
