@@ -604,6 +604,39 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 w)
     return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x));
 }
 
+// o[m] = sum_r a[r] w^(r m), w = exp(+j 2 pi / 8): two 4-point transforms
+// of the even and odd inputs, joined by w^m
+__device__ __forceinline__ void dft4_pos(float2 b0, float2 b1, float2 b2, float2 b3, float2 *o)
+{
+    const float2 t0 = make_float2(b0.x + b2.x, b0.y + b2.y);
+    const float2 t1 = make_float2(b0.x - b2.x, b0.y - b2.y);
+    const float2 t2 = make_float2(b1.x + b3.x, b1.y + b3.y);
+    const float2 t3 = make_float2(b3.y - b1.y, b1.x - b3.x); // +j (b1 - b3)
+    o[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+    o[1] = make_float2(t1.x + t3.x, t1.y + t3.y);
+    o[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+    o[3] = make_float2(t1.x - t3.x, t1.y - t3.y);
+}
+__device__ __forceinline__ void dft8_pos(const float2 (&a)[8], float2 (&o)[8])
+{
+    constexpr float kS = 0.70710678118654752F; // 1 / sqrt(2)
+    float2 ev[4], od[4];
+    dft4_pos(a[0], a[2], a[4], a[6], ev);
+    dft4_pos(a[1], a[3], a[5], a[7], od);
+    // w^m od[m]: w = (1 + j) / sqrt 2, w^2 = j, w^3 = (-1 + j) / sqrt 2
+    const float2 w1 = make_float2(kS * (od[1].x - od[1].y), kS * (od[1].x + od[1].y));
+    const float2 w2 = make_float2(-od[2].y, od[2].x);
+    const float2 w3 = make_float2(-kS * (od[3].x + od[3].y), kS * (od[3].x - od[3].y));
+    o[0] = make_float2(ev[0].x + od[0].x, ev[0].y + od[0].y);
+    o[4] = make_float2(ev[0].x - od[0].x, ev[0].y - od[0].y);
+    o[1] = make_float2(ev[1].x + w1.x, ev[1].y + w1.y);
+    o[5] = make_float2(ev[1].x - w1.x, ev[1].y - w1.y);
+    o[2] = make_float2(ev[2].x + w2.x, ev[2].y + w2.y);
+    o[6] = make_float2(ev[2].x - w2.x, ev[2].y - w2.y);
+    o[3] = make_float2(ev[3].x + w3.x, ev[3].y + w3.y);
+    o[7] = make_float2(ev[3].x - w3.x, ev[3].y - w3.y);
+}
+
 // Row n of the table from the transform x in LDS: row[j] = x[(j - lead) mod M]
 // (x[(lead - j) mod M] for down-chirps) for j < kProfileDeltaEntry past M,
 // then the chirp's position increment against chirp n - 1 (zero for the
@@ -631,8 +664,8 @@ __device__ __forceinline__ void write_profile_row(const float2 *x, const ChirpPo
 }
 
 // One workgroup per chirp: b[(i - h) mod M] = c_i * corr[i] (zero elsewhere),
-// then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-4 Stockham passes (and
-// one radix-2 pass when log2 M is odd) between two LDS buffers, with the
+// then B[m] = sum_j b[j] exp(+j 2 pi j m / M) by radix-8 Stockham passes (and
+// one last radix-4 or radix-2 pass) between two LDS buffers, with the
 // twiddles staged in LDS up to kProfileLdsTwiddleM (the output comes out in
 // natural order), written
 // shifted by lead = taps / 2 - 1 bins, row[j] = B[(j - lead) mod M], over M +
@@ -690,36 +723,49 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     }
     __syncthreads();
     // Stockham passes, B[m] = sum_j b[j] exp(+j 2 pi j m / M) in natural order:
-    // radix 4 while a quarter of the remaining length L = M / s is left, then
-    // one radix-2 pass (twiddle 1) when log2 M is odd
+    // radix 8 while an eighth of the remaining length L = M / s is left, then
+    // one radix-4 or radix-2 pass for the last factor of 4 or 2
     int s = 1, lg = 0;
-    for (; (M >> lg) >= 4; s <<= 2, lg += 2) {
-        const int q = (M >> lg) >> 2; // L / 4
-        for (int idx = tid; idx < M / 4; idx += BLOCK) {
+    for (; (M >> lg) >= 8; s <<= 3, lg += 3) {
+        const int q = (M >> lg) >> 3; // L / 8
+        for (int idx = tid; idx < M / 8; idx += BLOCK) {
             const int j = idx >> lg;
             const int k = idx & (s - 1);
-            const float2 a0 = x[k + s * j];
-            const float2 a1 = x[k + s * (j + q)];
-            const float2 a2 = x[k + s * (j + 2 * q)];
-            const float2 a3 = x[k + s * (j + 3 * q)];
-            const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y);
-            const float2 t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
-            const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
-            // +j (a1 - a3): the quarter turn of the +j 2 pi transform
-            const float2 t3 = make_float2(a3.y - a1.y, a1.x - a3.x);
+            float2 a[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) a[r] = x[k + s * (j + r * q)];
+            float2 o[8];
+            dft8_pos(a, o);
             const int e = j * s;
-            float2 *o = y + k + 4 * s * j;
-            o[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
-            o[s] = cmulf(make_float2(t1.x + t3.x, t1.y + t3.y), twiddle_at(tw, e, half));
-            o[2 * s] = cmulf(make_float2(t0.x - t2.x, t0.y - t2.y), twiddle_at(tw, 2 * e, half));
-            o[3 * s] = cmulf(make_float2(t1.x - t3.x, t1.y - t3.y), twiddle_at(tw, 3 * e, half));
+            float2 *dst = y + k + 8 * s * j;
+            dst[0] = o[0];
+#pragma unroll
+            for (int m = 1; m < 8; ++m) dst[m * s] = cmulf(o[m], twiddle_at(tw, m * e, half));
         }
         __syncthreads();
         float2 *tmp = x;
         x = y;
         y = tmp;
     }
-    if ((M >> lg) == 2) {
+    if ((M >> lg) == 4) {
+        // the last radix-4 pass: L = 4, so every twiddle is 1
+        for (int k = tid; k < M / 4; k += BLOCK) {
+            const float2 a0 = x[k], a1 = x[k + s], a2 = x[k + 2 * s], a3 = x[k + 3 * s];
+            const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y);
+            const float2 t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
+            const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
+            // +j (a1 - a3): the quarter turn of the +j 2 pi transform
+            const float2 t3 = make_float2(a3.y - a1.y, a1.x - a3.x);
+            y[k] = make_float2(t0.x + t2.x, t0.y + t2.y);
+            y[k + s] = make_float2(t1.x + t3.x, t1.y + t3.y);
+            y[k + 2 * s] = make_float2(t0.x - t2.x, t0.y - t2.y);
+            y[k + 3 * s] = make_float2(t1.x - t3.x, t1.y - t3.y);
+        }
+        __syncthreads();
+        float2 *tmp = x;
+        x = y;
+        y = tmp;
+    } else if ((M >> lg) == 2) {
         for (int k = tid; k < half; k += BLOCK) {
             const float2 a = x[k], c = x[k + half];
             y[k] = make_float2(a.x + c.x, a.y + c.y);
