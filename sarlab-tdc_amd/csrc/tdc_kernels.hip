@@ -691,18 +691,19 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
     float2 *y = buf + M;
     const float2 *tw = twiddle;
     const int mask = M - 1;
+    // the thread's first kTwiddles twiddles, loaded beside its first samples
+    constexpr int kTwiddles = 4;
+    float2 twr[kTwiddles];
     if constexpr (LDS_TW) {
-        float2 *t = buf + 2 * M;
-#pragma unroll 4
-        for (int k = tid; k < half; k += BLOCK) t[k] = twiddle[k];
-        tw = t;
+#pragma unroll
+        for (int u = 0; u < kTwiddles; ++u) twr[u] = twiddle[min(tid + u * BLOCK, half - 1)];
     }
     // b[j] = c_i corr[i] for j = (i - h) mod M: zeros at j in [ns - h, M - h),
     // then the ns samples, kUnroll per thread at a time with their loads
     // issued before the first is used
     for (int j = ns - h + tid; j < M - h; j += BLOCK) x[j] = make_float2(0.0F, 0.0F);
     constexpr int kUnroll = 4;
-    for (int i0 = tid; i0 < ns; i0 += kUnroll * BLOCK) {
+    auto samples = [&](int i0) {
         ProfileWord<RAW, FOLD> wd[kUnroll];
         float c[kUnroll];
 #pragma unroll
@@ -720,6 +721,19 @@ __global__ __launch_bounds__(BLOCK) void stage_profile_kernel(
                 x[(i - h) & mask] = make_float2(re * c[u], im * c[u]);
             }
         }
+    };
+    // (the first batch outside the loop, so that its loads issue with the
+    // twiddles')
+    if (ns > 0) samples(tid);
+    for (int i0 = tid + kUnroll * BLOCK; i0 < ns; i0 += kUnroll * BLOCK) samples(i0);
+    if constexpr (LDS_TW) {
+        float2 *t = buf + 2 * M;
+#pragma unroll
+        for (int u = 0; u < kTwiddles; ++u) {
+            if (tid + u * BLOCK < half) t[tid + u * BLOCK] = twr[u];
+        }
+        for (int k = tid + kTwiddles * BLOCK; k < half; k += BLOCK) t[k] = twiddle[k];
+        tw = t;
     }
     __syncthreads();
     // Stockham passes, B[m] = sum_j b[j] exp(+j 2 pi j m / M) in natural order:
