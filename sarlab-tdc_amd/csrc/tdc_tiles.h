@@ -854,6 +854,22 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 qz32 = static_cast<float>(0.5 * qz);
     const float dkm32 = static_cast<float>(pp.dkm);
     const float kh32 = static_cast<float>(pp.kh);
+    // The table through one buffer resource and 32-bit byte offsets (a
+    // table holds at most 2^31 bytes, checked on the host): row n starts at
+    // n Mp 8 bytes, its position increment kProfileDeltaEntry 8 bytes past
+    // its M bins
+    const auto tab = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(table),
+                                                       static_cast<short>(0), 0x7fffffff,
+                                                       kBufferRsrcWord3);
+    const unsigned row_bytes = static_cast<unsigned>(pp.Mp) * static_cast<unsigned>(sizeof(float2));
+    const unsigned delta_bytes =
+        static_cast<unsigned>(pp.M + kProfileDeltaEntry) * static_cast<unsigned>(sizeof(float2));
+    const char *tbytes = reinterpret_cast<const char *>(table);
+    const char *cbytes = reinterpret_cast<const char *>(cpos);
+    // all ones when the block's other chirps advance by FP32 increments, else
+    // zero (an integer, so that the per-chirp choice is one scalar test)
+    unsigned incr_mask = __builtin_amdgcn_readfirstlane(pp.incr ? ~0U : 0U);
+    asm volatile("" : "+s"(incr_mask));
     // chirp contributions are summed in FP32 over blocks of kProfileBlock
     // chirps and the blocks in FP64
     double accr = 0.0, acci = 0.0;
@@ -867,9 +883,11 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
         float s = 0.0F, is = 0.0F, g = 0.0F, a = 0.0F;
         unsigned f = 0;
         for (int n = nb; n < ne; ++n) {
-            const float2 *row = table + static_cast<size_t>(n) * pp.Mp;
-            if (n == nb || !pp.incr) {
-                const ChirpPos c = as_const(cpos)[n];
+            const unsigned row0 = static_cast<unsigned>(n) * row_bytes;
+            if ((static_cast<unsigned>(n - nb) & incr_mask) == 0U) {
+                // the block's first chirp, or every chirp without pp.incr
+                const ChirpPos c = as_const(reinterpret_cast<const ChirpPos *>(
+                    cbytes + static_cast<unsigned>(n) * static_cast<unsigned>(sizeof(ChirpPos))))[0];
                 const double rng =
                     range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
                 // bin position u = x mod M, x = |dk| M R >= 0: integer part f (the
@@ -892,8 +910,8 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 // (the series of sqrt(R^2 + delta) - R), then s, the bin position
                 // and the anchor phase advanced by dR (bounds:
                 // profile_incremental_ok)
-                const float4 d = as_const(
-                    reinterpret_cast<const float4 *>(row + pp.M + kProfileDeltaEntry))[0];
+                const float4 d =
+                    as_const(reinterpret_cast<const float4 *>(tbytes + (row0 + delta_bytes)))[0];
                 const float hdelta = fmaf(qz32, d.z, fmaf(qy32, d.y, fmaf(qx32, d.x, d.w)));
                 const float e0 = hdelta * is;
                 const float u = e0 * is;
@@ -920,18 +938,18 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 // and entries past the window's 64 read as zero)
                 const float2 e = __builtin_bit_cast(
                     float2, __builtin_amdgcn_raw_buffer_load_b64(
-                                __builtin_amdgcn_make_buffer_rsrc(
-                                    const_cast<float2 *>(row + (lo & static_cast<unsigned>(mask))),
-                                    static_cast<short>(0), 64 * static_cast<int>(sizeof(float2)),
-                                    kBufferRsrcWord3),
-                                lane_bytes, 0, 0));
+                                tab, lane_bytes,
+                                static_cast<int>(row0 + (lo & static_cast<unsigned>(mask)) *
+                                                            static_cast<unsigned>(sizeof(float2))),
+                                0));
                 wave_lds_sync(); // the previous chirp's reads of the slot are done
                 win[lane] = e;
                 wave_lds_sync();
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) v[t] = win[off + t];
             } else {
-                const float2 *r = row + (f & static_cast<unsigned>(mask));
+                const float2 *r = reinterpret_cast<const float2 *>(tbytes + row0) +
+                                  (f & static_cast<unsigned>(mask));
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) v[t] = r[t];
             }
