@@ -854,18 +854,14 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 qz32 = static_cast<float>(0.5 * qz);
     const float dkm32 = static_cast<float>(pp.dkm);
     const float kh32 = static_cast<float>(pp.kh);
-    // The table through one buffer resource and 32-bit byte offsets (a
-    // table holds at most 2^31 bytes, checked on the host): row n starts at
-    // n Mp 8 bytes, its position increment kProfileDeltaEntry 8 bytes past
-    // its M bins
-    const auto tab = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(table),
-                                                       static_cast<short>(0), 0x7fffffff,
-                                                       kBufferRsrcWord3);
+    // Within a block the rows and chirp records are addressed by 32-bit byte
+    // offsets from the block's first (a row is at most (kProfileMaxM +
+    // kProfilePad) 8 bytes), the rows through one buffer resource per block:
+    // row n starts (n - nb) Mp 8 bytes past row nb, its position increment
+    // kProfileDeltaEntry 8 bytes past its M bins
     const unsigned row_bytes = static_cast<unsigned>(pp.Mp) * static_cast<unsigned>(sizeof(float2));
     const unsigned delta_bytes =
         static_cast<unsigned>(pp.M + kProfileDeltaEntry) * static_cast<unsigned>(sizeof(float2));
-    const char *tbytes = reinterpret_cast<const char *>(table);
-    const char *cbytes = reinterpret_cast<const char *>(cpos);
     // all ones when the block's other chirps advance by FP32 increments, else
     // zero (an integer, so that the per-chirp choice is one scalar test)
     unsigned incr_mask = __builtin_amdgcn_readfirstlane(pp.incr ? ~0U : 0U);
@@ -876,6 +872,12 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     for (int nb = n0; nb < n1; nb += profile::kProfileBlock) {
         const int ne = min(n1, nb + profile::kProfileBlock);
         float br = 0.0F, bi = 0.0F;
+        const float2 *brow = table + static_cast<size_t>(nb) * pp.Mp;
+        const auto tab = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(brow),
+                                                           static_cast<short>(0), 0x7fffffff,
+                                                           kBufferRsrcWord3);
+        const char *tbytes = reinterpret_cast<const char *>(brow);
+        const char *cbytes = reinterpret_cast<const char *>(cpos + nb);
         // the pixel's state within the block: FP32 range s and 1 / s, bin f
         // and fraction g of its position, anchor phase a = frac(k_h R); within
         // the block f is kept modulo 2^32 (a multiple of M) and masked to the
@@ -883,11 +885,12 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
         float s = 0.0F, is = 0.0F, g = 0.0F, a = 0.0F;
         unsigned f = 0;
         for (int n = nb; n < ne; ++n) {
-            const unsigned row0 = static_cast<unsigned>(n) * row_bytes;
+            const unsigned row0 = static_cast<unsigned>(n - nb) * row_bytes;
             if ((static_cast<unsigned>(n - nb) & incr_mask) == 0U) {
                 // the block's first chirp, or every chirp without pp.incr
                 const ChirpPos c = as_const(reinterpret_cast<const ChirpPos *>(
-                    cbytes + static_cast<unsigned>(n) * static_cast<unsigned>(sizeof(ChirpPos))))[0];
+                    cbytes +
+                    static_cast<unsigned>(n - nb) * static_cast<unsigned>(sizeof(ChirpPos))))[0];
                 const double rng =
                     range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
                 // bin position u = x mod M, x = |dk| M R >= 0: integer part f (the

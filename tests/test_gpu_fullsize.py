@@ -187,3 +187,38 @@ def test_helix_engine_vs_oracle(tdc, orc, engine):
     print(f"helix {engine}: rel-L2 {e:.3e} on {px.size} pixels, peak phase {ph:.3e} rad")
     assert e <= PREC_TOL
     assert ph <= PREC_TOL
+
+
+def test_profile_table_beyond_4gib(tdc, orc):
+    """A range-profile table past 2^32 bytes: 32800 chirps of 4096 samples at
+    fourfold oversampling give 32800 rows of 16456 entries (4.32 GB). The tile
+    addresses rows and chirp records by 32-bit offsets from each 8-chirp
+    block's first, so the rows past 2^32 bytes are read right: 16 pixels of
+    random raw data against the FP64 oracle."""
+    npri, ns = 32800, 4096
+    rng = np.random.default_rng(29)
+    raw = rng.standard_normal((npri, ns, 2), dtype=np.float32).view(np.complex64)[..., 0]
+    raw = np.ascontiguousarray(raw)
+    f0, chirp, bw = 77e9, 40e-6, 4e9
+    sample_times = np.arange(ns, dtype=np.float64) * (chirp / ns)
+    pri_times = np.arange(npri, dtype=np.float64) * 50e-6
+    track = np.zeros((npri, 3))
+    track[:, 0] = np.linspace(-0.25, 0.25, npri)
+    gx, gy = np.meshgrid(np.arange(4) * 3e-3, 1.0 + np.arange(4) * 3e-3)
+    grid = np.ascontiguousarray(np.stack([gx, gy, np.zeros_like(gx)], axis=-1))
+    p = tdc.TdcProcessor(0)
+    p.setLogging(False)
+    p.setRawDataF64(raw, pri_times, sample_times, track, bw / chirp, f0)
+    p.setFocusGridF64(grid)
+    p.setEngine("profile")
+    # (8 taps: on white, unwindowed samples the six-tap kernel's own error is
+    # ~1.4e-4; rows read from the wrong place would give ~7%)
+    p.setProfileKernel(8, 4)
+    p.start(0.0, 1e30, False, False, False)
+    st = p.stats()
+    assert (st["engine"], st["taps"], st["oversampling"]) == ("profile", 8, 4)
+    img = p.getFocusedImage().reshape(-1)
+    ref = orc.image_f64(raw, sample_times, track, grid, bw / chirp, f0).reshape(-1)
+    e = orc.rel_l2(img, ref)
+    print(f"table {npri * (4 * ns + 72) * 8 / 1e9:.2f} GB, rel-L2 {e:.3e}")
+    assert e <= PREC_TOL, e
