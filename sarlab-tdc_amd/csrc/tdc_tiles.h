@@ -863,7 +863,8 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     const unsigned delta_bytes =
         static_cast<unsigned>(pp.M + kProfileDeltaEntry) * static_cast<unsigned>(sizeof(float2));
     // all ones when the block's other chirps advance by FP32 increments, else
-    // zero (an integer, so that the per-chirp choice is one scalar test)
+    // zero: an integer kept opaque, so that the per-chirp choice stays one
+    // scalar AND and compare instead of being folded into lane-mask logic
     unsigned incr_mask = __builtin_amdgcn_readfirstlane(pp.incr ? ~0U : 0U);
     asm volatile("" : "+s"(incr_mask));
     // chirp contributions are summed in FP32 over blocks of kProfileBlock
@@ -936,9 +937,9 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
             if (__all(off <= 64U - TAPS)) {
                 // entries lo .. lo + 63 modulo M: from bin lo mod M on, the
                 // row's wrap-around entries cover the window's end
-                // (a buffer load: the wave-uniform window start in a scalar
-                // resource, the lane's byte offset in a loop-invariant register,
-                // and entries past the window's 64 read as zero)
+                // (a buffer load: the block's rows as a scalar resource, the
+                // window's start as its scalar offset and the lane's byte
+                // offset in a loop-invariant register)
                 const float2 e = __builtin_bit_cast(
                     float2, __builtin_amdgcn_raw_buffer_load_b64(
                                 tab, lane_bytes,
