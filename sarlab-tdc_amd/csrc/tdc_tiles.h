@@ -867,103 +867,139 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
     // scalar AND and compare instead of being folded into lane-mask logic
     unsigned incr_mask = __builtin_amdgcn_readfirstlane(pp.incr ? ~0U : 0U);
     asm volatile("" : "+s"(incr_mask));
-    // chirp contributions are summed in FP32 over blocks of kProfileBlock
-    // chirps and the blocks in FP64
+    // The chirps in order, each block of kProfileBlock from n0 with its first
+    // chirp exact; chirp n + 1's state and the load of its window are issued
+    // while chirp n's taps are summed. The block's rows and chirp records
+    // are addressed from its first: brow, its buffer resource, cbytes.
     double accr = 0.0, acci = 0.0;
-    for (int nb = n0; nb < n1; nb += profile::kProfileBlock) {
-        const int ne = min(n1, nb + profile::kProfileBlock);
-        float br = 0.0F, bi = 0.0F;
-        const float2 *brow = table + static_cast<size_t>(nb) * pp.Mp;
-        const auto tab = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(brow),
-                                                           static_cast<short>(0), 0x7fffffff,
-                                                           kBufferRsrcWord3);
-        const char *tbytes = reinterpret_cast<const char *>(brow);
-        const char *cbytes = reinterpret_cast<const char *>(cpos + nb);
-        // the pixel's state within the block: FP32 range s and 1 / s, bin f
-        // and fraction g of its position, anchor phase a = frac(k_h R); within
-        // the block f is kept modulo 2^32 (a multiple of M) and masked to the
-        // row where a row is indexed with it
-        float s = 0.0F, is = 0.0F, g = 0.0F, a = 0.0F;
+    if (n0 < n1) {
+        float s = 0.0F, is = 0.0F, g = 0.0F, a = 0.0F, br = 0.0F, bi = 0.0F;
         unsigned f = 0;
-        for (int n = nb; n < ne; ++n) {
-            const unsigned row0 = static_cast<unsigned>(n - nb) * row_bytes;
-            if ((static_cast<unsigned>(n - nb) & incr_mask) == 0U) {
-                // the block's first chirp, or every chirp without pp.incr
-                const ChirpPos c = as_const(reinterpret_cast<const ChirpPos *>(
-                    cbytes +
-                    static_cast<unsigned>(n - nb) * static_cast<unsigned>(sizeof(ChirpPos))))[0];
-                const double rng =
-                    range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
-                // bin position u = x mod M, x = |dk| M R >= 0: integer part f (the
-                // truncation of x, exact while x < 2^31, i.e. R B < ~8e16 m Hz at
-                // fourfold oversampling) and fraction g; down-chirps read reversed
-                // rows, so x is never negative
-                const double x = pp.dkm * rng;
-                g = static_cast<float>(fract64(x));
-                // indices in unsigned arithmetic: f < M, so row[f .. f + TAPS - 1]
-                // lies in the row; window indices are masked to the row
-                f = static_cast<unsigned>(static_cast<int>(x)) & static_cast<unsigned>(mask);
-                a = static_cast<float>(fract64(pp.kh * rng));
-                // (read by the next chirp's increment only when pp.incr)
-                s = static_cast<float>(rng);
-                is = __builtin_amdgcn_rcpf(s);
-            } else {
-                // the next chirp of the block: delta / 2 = (R_n^2 - R_{n-1}^2) / 2
-                // from the chirp's position increment (the row's last entries),
-                // dR = e0 (1 - u / 2 + u^2 / 2), e0 = delta / (2 R), u = e0 / R
-                // (the series of sqrt(R^2 + delta) - R), then s, the bin position
-                // and the anchor phase advanced by dR (bounds:
-                // profile_incremental_ok)
-                const float4 d =
-                    as_const(reinterpret_cast<const float4 *>(tbytes + (row0 + delta_bytes)))[0];
-                const float hdelta = fmaf(qz32, d.z, fmaf(qy32, d.y, fmaf(qx32, d.x, d.w)));
-                const float e0 = hdelta * is;
-                const float u = e0 * is;
-                const float dr = e0 * fmaf(u, fmaf(0.5F, u, -0.5F), 1.0F);
-                s += dr;
-                is = __builtin_amdgcn_rcpf(s);
-                const float gx = fmaf(dkm32, dr, g);
-                f += static_cast<unsigned>(floor_to_int(gx));
-                g = __builtin_amdgcn_fractf(gx);
-                a = __builtin_amdgcn_fractf(fmaf(kh32, dr, a));
-            }
-            constexpr unsigned kReach = (64 - TAPS) / 2;
-            unsigned lo = __builtin_amdgcn_readfirstlane(f) - kReach;
-            // (lo kept opaque, so that the window test's off also addresses the
-            // window instead of being formed again from lane 0's bin)
+        const float2 *brow = table + static_cast<size_t>(n0) * pp.Mp;
+        auto tab = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(brow),
+                                                     static_cast<short>(0), 0x7fffffff,
+                                                     kBufferRsrcWord3);
+        const char *cbytes = reinterpret_cast<const char *>(cpos + n0);
+        // the pixel's state at the chirp: FP32 range s and 1 / s, bin f and
+        // fraction g of its position, anchor phase a = frac(k_h R); within a
+        // block f is kept modulo 2^32 (a multiple of M) and masked to the row
+        // where a row is indexed with it
+        unsigned row0 = 0, k = 0; // the chirp's row offset and place in its block
+        // the state in FP64: a block's first chirp (k = 0), or every chirp
+        // without pp.incr
+        auto exact = [&]() {
+            const ChirpPos c = as_const(reinterpret_cast<const ChirpPos *>(
+                cbytes + (k & ~incr_mask) * static_cast<unsigned>(sizeof(ChirpPos))))[0];
+            const double rng = range_newton(fma(c.z, qz, fma(c.y, qy, fma(c.x, qx, c.pad + pp2))));
+            // bin position u = x mod M, x = |dk| M R >= 0: integer part f (the
+            // truncation of x, exact while x < 2^31, i.e. R B < ~8e16 m Hz at
+            // fourfold oversampling) and fraction g; down-chirps read reversed
+            // rows, so x is never negative
+            const double x = pp.dkm * rng;
+            g = static_cast<float>(fract64(x));
+            // indices in unsigned arithmetic: f < M, so row[f .. f + TAPS - 1]
+            // lies in the row; window indices are masked to the row
+            f = static_cast<unsigned>(static_cast<int>(x)) & static_cast<unsigned>(mask);
+            a = static_cast<float>(fract64(pp.kh * rng));
+            // (read by the next chirp's increment only when pp.incr)
+            s = static_cast<float>(rng);
+            is = __builtin_amdgcn_rcpf(s);
+        };
+        // the state advanced by one chirp within a block: delta / 2 = (R_n^2
+        // - R_{n-1}^2) / 2 from the chirp's position increment (the row's last
+        // entries), dR = e0 (1 - u / 2 + u^2 / 2), e0 = delta / (2 R), u = e0 /
+        // R (the series of sqrt(R^2 + delta) - R), then s, the bin position
+        // and the anchor phase advanced by dR (bounds: profile_incremental_ok)
+        auto advance = [&]() {
+            const float4 d = as_const(reinterpret_cast<const float4 *>(
+                reinterpret_cast<const char *>(brow) + (row0 + delta_bytes)))[0];
+            const float hdelta = fmaf(qz32, d.z, fmaf(qy32, d.y, fmaf(qx32, d.x, d.w)));
+            const float e0 = hdelta * is;
+            const float u = e0 * is;
+            const float dr = e0 * fmaf(u, fmaf(0.5F, u, -0.5F), 1.0F);
+            s += dr;
+            is = __builtin_amdgcn_rcpf(s);
+            const float gx = fmaf(dkm32, dr, g);
+            f += static_cast<unsigned>(floor_to_int(gx));
+            g = __builtin_amdgcn_fractf(gx);
+            a = __builtin_amdgcn_fractf(fmaf(kh32, dr, a));
+        };
+        // the wave's window of the chirp's row: entries lo .. lo + 63 modulo
+        // M, lo (64 - TAPS) / 2 bins below lane 0's first tap (from bin lo mod
+        // M on, the row's wrap-around entries cover the window's end), by a
+        // buffer load: the block's rows as a scalar resource, the window's
+        // start as its scalar offset, the lane's byte offset in a
+        // loop-invariant register
+        constexpr unsigned kReach = (64 - TAPS) / 2;
+        auto window = [&](unsigned &lo) {
+            lo = __builtin_amdgcn_readfirstlane(f) - kReach;
+            // (lo kept opaque, so that the window test's off also addresses
+            // the window instead of being formed again from lane 0's bin)
             asm volatile("" : "+s"(lo));
+            return __builtin_bit_cast(
+                float2, __builtin_amdgcn_raw_buffer_load_b64(
+                            tab, lane_bytes,
+                            static_cast<int>(row0 + (lo & static_cast<unsigned>(mask)) *
+                                                        static_cast<unsigned>(sizeof(float2))),
+                            0));
+        };
+        exact();
+        unsigned lo;
+        float2 e = window(lo);
+        for (int n = n0; n < n1; ++n) {
+            // chirp n's taps: from its window (loaded during chirp n - 1)
+            // through the wave's LDS slot, or, with a lane outside the
+            // window, gathered from the row
             const unsigned off = f - lo;
             float2 v[TAPS];
             if (__all(off <= 64U - TAPS)) {
-                // entries lo .. lo + 63 modulo M: from bin lo mod M on, the
-                // row's wrap-around entries cover the window's end
-                // (a buffer load: the block's rows as a scalar resource, the
-                // window's start as its scalar offset and the lane's byte
-                // offset in a loop-invariant register)
-                const float2 e = __builtin_bit_cast(
-                    float2, __builtin_amdgcn_raw_buffer_load_b64(
-                                tab, lane_bytes,
-                                static_cast<int>(row0 + (lo & static_cast<unsigned>(mask)) *
-                                                            static_cast<unsigned>(sizeof(float2))),
-                                0));
                 wave_lds_sync(); // the previous chirp's reads of the slot are done
                 win[lane] = e;
                 wave_lds_sync();
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) v[t] = win[off + t];
             } else {
-                const float2 *r = reinterpret_cast<const float2 *>(tbytes + row0) +
+                const float2 *r = reinterpret_cast<const float2 *>(
+                                      reinterpret_cast<const char *>(brow) + row0) +
                                   (f & static_cast<unsigned>(mask));
 #pragma unroll
                 for (int t = 0; t < TAPS; ++t) v[t] = r[t];
+                // (waited for here, so that the sums below wait for no
+                // vector-memory load and the next window's stays in flight)
+#pragma unroll
+                for (int t = 0; t < TAPS; ++t) asm volatile("" ::"v"(v[t].x), "v"(v[t].y));
             }
-            // up0 = exp2(g (kE0 g - lead kE1)), e1 = exp2(kE1 g): for g in
-            // [0, 1) and every built kernel up0 lies in (0.0079, 1] and
-            // up0 e1^(TAPS - 1) below 126, far inside FP32's range
+            // chirp n's weights and anchor rotation, formed before its state
+            // moves on: up0 = exp2(g (kE0 g - lead kE1)), e1 = exp2(kE1 g)
+            // (for g in [0, 1) and every built kernel up0 lies in (0.0079, 1]
+            // and up0 e1^(TAPS - 1) below 126, far inside FP32's range)
             using K = profile::Kernel<TAPS, OVER>;
             constexpr int kLead = TAPS / 2 - 1;
             float up = __builtin_amdgcn_exp2f(g * fmaf(K::kE0, g, -kLead * K::kE1));
             const float e1 = __builtin_amdgcn_exp2f(K::kE1 * g);
+            const float zr = cos_rev(a), zi = sin_rev(a);
+            asm volatile("" ::"v"(up), "v"(e1), "v"(zr), "v"(zi));
+            // chirp n + 1: its state and its window's load, in flight while
+            // chirp n's taps are summed
+            k = k + 1 == static_cast<unsigned>(profile::kProfileBlock) ? 0U : k + 1;
+            const bool last = n + 1 == n1;
+            if (!last) {
+                if (k == 0) {
+                    brow += static_cast<size_t>(profile::kProfileBlock) * pp.Mp;
+                    tab = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(brow),
+                                                            static_cast<short>(0), 0x7fffffff,
+                                                            kBufferRsrcWord3);
+                    cbytes += profile::kProfileBlock * sizeof(ChirpPos);
+                    row0 = 0;
+                } else {
+                    row0 += row_bytes;
+                }
+                if ((k & incr_mask) == 0U)
+                    exact();
+                else
+                    advance();
+                e = window(lo);
+            }
             float ar = 0.0F, ai = 0.0F;
 #pragma unroll
             for (int t = 0; t < TAPS; ++t) {
@@ -972,12 +1008,17 @@ __device__ __forceinline__ void profile_tile(const float2 *__restrict__ table,
                 ai = fmaf(w, v[t].y, ai);
                 up *= e1;
             }
-            const float zr = cos_rev(a), zi = sin_rev(a);
             br = fmaf(zr, ar, fmaf(-zi, ai, br));
             bi = fmaf(zr, ai, fmaf(zi, ar, bi));
+            // chirp contributions are summed in FP32 over a block and the
+            // blocks in FP64
+            if (k == 0 || last) {
+                accr += static_cast<double>(br);
+                acci += static_cast<double>(bi);
+                br = 0.0F;
+                bi = 0.0F;
+            }
         }
-        accr += static_cast<double>(br);
-        acci += static_cast<double>(bi);
     }
     if (pix < npx) out[pix] = make_float2(static_cast<float>(accr), static_cast<float>(acci));
 }
